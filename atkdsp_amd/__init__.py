@@ -1,0 +1,497 @@
+"""atkdsp_amd — the wideband front end of atkdsp on one MI355X.
+
+A ctypes binding over ``atkdsp_amd/atkdsp.so`` with the names, argument
+meaning and error behaviour of the reference binding's front-end subset
+(reference python/atkdsp/__init__.py: ``load`` :238-268, ``unpack`` :337,
+``unpack_dc`` :361, ``Fft`` :531-599, ``window`` :602, ``db_to_pixels`` :673,
+``decimate_max`` :686, ``stitch_max`` :719, ``Pfb`` :4159-4219,
+``pfb_cov_accumulate`` :4222, ``pfb_channel_power`` :4252), so code written
+against ``atkdsp`` runs against this package unchanged. The library is also a
+drop-in for the reference binding itself: point ``ATKDSP_LIB`` at
+``atkdsp_amd/atkdsp.so``.
+
+There is no CPU fallback: without the built library or without a GPU every
+call raises :class:`AtkDspError`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+ABI_VERSION = 26
+
+FMT = {"cu8": 0, "ci8": 1, "ci16": 2, "ci16_le": 2, "cs16": 2,
+       "ci16q11": 3, "cf32": 4, "cf32_le": 4}
+DET = {"max": 0, "min": 1, "avg": 2}
+WIN = {"rectangular": 0, "hann": 1, "hamming": 2, "blackmanharris": 3}
+
+LIB_NAME = "atkdsp.so"
+_HERE = Path(__file__).resolve().parent
+
+
+class AtkDspError(RuntimeError):
+    """A kernel returned an error code, or the library could not be loaded."""
+
+
+class _Cf32(C.Structure):
+    _fields_ = [("re", C.c_float), ("im", C.c_float)]
+
+
+_ERRORS = {-1: "bad argument", -2: "out of memory", -3: "output buffer too small",
+           -4: "FFT backend or device failure"}
+
+_lib = None
+_lib_path: Path | None = None
+
+
+def _check(rc: int, what: str) -> int:
+    if rc < 0:
+        raise AtkDspError(f"{what}: {_ERRORS.get(rc, f'error {rc}')}")
+    return rc
+
+
+def _bind(lib) -> None:
+    P = C.POINTER
+    cf, fp, vp, sz, u, d = P(_Cf32), P(C.c_float), C.c_void_p, C.c_size_t, C.c_uint, C.c_double
+    sig = {
+        "atkdsp_abi_version": (C.c_int, []),
+        "atkdsp_version_string": (C.c_char_p, []),
+        "atkdsp_build_info": (C.c_char_p, []),
+        "atkdsp_bytes_per_sample": (C.c_int, [C.c_int]),
+        "atkdsp_set_threads": (C.c_int, [C.c_int]),
+        "atkdsp_get_threads": (C.c_int, []),
+        "atkdsp_unpack": (C.c_ssize_t, [vp, sz, C.c_int, cf, sz, C.c_float, cf]),
+        "atkdsp_unpack_dc": (C.c_ssize_t, [vp, sz, C.c_int, cf, sz, C.c_float, C.c_float,
+                                           P(d), P(d)]),
+        "atkdsp_fft_create": (vp, [sz]),
+        "atkdsp_fft_destroy": (None, [vp]),
+        "atkdsp_fft_length": (sz, [vp]),
+        "atkdsp_fft_exec": (C.c_int, [vp, cf, cf, C.c_int]),
+        "atkdsp_window": (C.c_int, [C.c_int, sz, fp]),
+        "atkdsp_power_db": (C.c_int, [vp, cf, fp, fp]),
+        "atkdsp_spectrum_reduce": (C.c_ssize_t, [vp, cf, sz, sz, fp, C.c_int, fp]),
+        "atkdsp_spectrum_stats": (C.c_ssize_t, [vp, cf, sz, sz, fp, fp, fp, fp, fp]),
+        "atkdsp_window_stats": (C.c_int, [fp, sz, P(d), P(d)]),
+        "atkdsp_db_to_pixels": (None, [fp, sz, C.c_float, C.c_float, P(C.c_uint32), P(C.c_uint32)]),
+        "atkdsp_decimate_max": (None, [fp, sz, fp, sz]),
+        "atkdsp_stitch_max": (None, [fp, sz, d, d, d, d, d, d, fp, sz]),
+        "atkdsp_pfb_create": (vp, [d, u, u, u, d, u]),
+        "atkdsp_pfb_destroy": (None, [vp]),
+        "atkdsp_pfb_reset": (None, [vp]),
+        "atkdsp_pfb_nchan": (u, [vp]),
+        "atkdsp_pfb_out_rate": (d, [vp]),
+        "atkdsp_pfb_out_hops": (sz, [vp, sz]),
+        "atkdsp_pfb_process": (C.c_ssize_t, [vp, cf, sz, cf, sz]),
+        "atkdsp_pfb_process_n": (C.c_ssize_t, [vp, P(cf), u, sz, cf, sz]),
+        "atkdsp_pfb_block_origin": (C.c_int64, [vp, C.c_int64]),
+        "atkdsp_pfb_group_delay": (d, [vp]),
+        "atkdsp_pfb_set_cal": (C.c_int, [vp, cf, u]),
+        "atkdsp_pfb_describe": (C.c_int, [vp, C.c_char_p, sz]),
+        "atkdsp_pfb_cov_accumulate": (None, [cf, u, u, sz, P(C.c_uint8), cf, P(C.c_uint32)]),
+        "atkdsp_pfb_cov_reset": (None, [cf, P(C.c_uint32), u, u, P(C.c_uint8)]),
+        "atkdsp_pfb_channel_power": (None, [cf, u, u, sz, fp, fp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _candidates() -> list[Path]:
+    out = []
+    env = os.environ.get("ATKDSP_AMD_LIB")
+    if env:
+        out.append(Path(env))
+    out.append(_HERE / LIB_NAME)
+    return out
+
+
+def load(path: str | os.PathLike | None = None):
+    """Load the library once and return the ctypes handle; raises AtkDspError."""
+    global _lib, _lib_path
+    if _lib is not None and path is None:
+        return _lib
+    errors = []
+    for cand in ([Path(path)] if path else _candidates()):
+        if not cand.exists():
+            errors.append(f"{cand}: not found (run python -m atkdsp_amd.build)")
+            continue
+        try:
+            lib = C.CDLL(str(cand))
+        except OSError as e:
+            errors.append(f"{cand}: {e}")
+            continue
+        lib.atkdsp_abi_version.restype = C.c_int
+        abi = int(lib.atkdsp_abi_version())
+        if abi != ABI_VERSION:
+            errors.append(f"{cand}: ABI {abi}, this binding needs {ABI_VERSION}")
+            continue
+        _bind(lib)
+        _lib, _lib_path = lib, cand
+        return lib
+    raise AtkDspError("could not load atkdsp_amd: " + "; ".join(errors))
+
+
+def available() -> bool:
+    """True when the library loads (it may still find no GPU). Never raises."""
+    try:
+        load()
+        return True
+    except AtkDspError:
+        return False
+
+
+def lib_path() -> Path | None:
+    return _lib_path
+
+
+def version() -> str:
+    return load().atkdsp_version_string().decode()
+
+
+def build_info() -> str:
+    return load().atkdsp_build_info().decode()
+
+
+def set_threads(n: int) -> int:
+    return int(load().atkdsp_set_threads(int(n)))
+
+
+def get_threads() -> int:
+    return int(load().atkdsp_get_threads())
+
+
+def bytes_per_sample(fmt) -> int:
+    return int(load().atkdsp_bytes_per_sample(_fmt(fmt)))
+
+
+def _fmt(fmt) -> int:
+    if isinstance(fmt, str):
+        if fmt not in FMT:
+            raise AtkDspError(f"unknown sample format {fmt!r}")
+        return FMT[fmt]
+    return int(fmt)
+
+
+def _cf32(a, name="array") -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.complex64)
+    if a.ndim != 1:
+        raise AtkDspError(f"{name} must be one-dimensional")
+    return a
+
+
+def _f32(a, name="array") -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 1:
+        raise AtkDspError(f"{name} must be one-dimensional")
+    return a
+
+
+def _cfp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(_Cf32))
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _raw_bytes(raw) -> np.ndarray:
+    if isinstance(raw, np.ndarray):
+        return np.ascontiguousarray(raw).view(np.uint8).reshape(-1)
+    return np.frombuffer(raw, dtype=np.uint8)
+
+
+# ---- unpack ------------------------------------------------------------------
+def unpack(raw, fmt, dc_alpha: float = 0.0, dc_state: np.ndarray | None = None,
+           out: np.ndarray | None = None) -> np.ndarray:
+    """Raw bytes -> complex64. With ``dc_alpha > 0`` the one-pole DC block runs
+    with its state in ``dc_state`` (a 1-element complex64 array kept by the caller)."""
+    lib = load()
+    src = _raw_bytes(raw)
+    n = src.size // bytes_per_sample(fmt) if bytes_per_sample(fmt) else 0
+    if out is None:
+        out = np.empty(n, dtype=np.complex64)
+    elif out.dtype != np.complex64 or not out.flags.c_contiguous:
+        raise AtkDspError("out must be a contiguous complex64 array")
+    st = None
+    if dc_alpha > 0:
+        if dc_state is None or dc_state.dtype != np.complex64 or dc_state.size != 1:
+            raise AtkDspError("dc_state must be a 1-element complex64 array")
+        st = _cfp(dc_state)
+    got = _check(lib.atkdsp_unpack(src.ctypes.data, src.size, _fmt(fmt), _cfp(out), out.size,
+                                   float(dc_alpha), st), "unpack")
+    return out[:got]
+
+
+def unpack_dc(raw, fmt, offset: complex = 0j, out: np.ndarray | None = None,
+              want_mean: bool = True):
+    """Raw bytes -> complex64 minus ``offset``; returns ``(x, mean)`` where
+    ``mean`` is the mean of the samples before the offset was removed."""
+    lib = load()
+    src = _raw_bytes(raw)
+    bps = bytes_per_sample(fmt)
+    n = src.size // bps if bps else 0
+    if out is None:
+        out = np.empty(n, dtype=np.complex64)
+    elif out.dtype != np.complex64 or not out.flags.c_contiguous:
+        raise AtkDspError("out must be a contiguous complex64 array")
+    mr, mi = C.c_double(0.0), C.c_double(0.0)
+    got = _check(lib.atkdsp_unpack_dc(src.ctypes.data, src.size, _fmt(fmt), _cfp(out), out.size,
+                                      float(offset.real), float(offset.imag),
+                                      C.byref(mr) if want_mean else None,
+                                      C.byref(mi) if want_mean else None), "unpack_dc")
+    return out[:got], complex(mr.value, mi.value)
+
+
+# ---- FFT ------------------------------------------------------------------------
+class Fft:
+    """A plan for one length (any n >= 2). Forward unscaled, inverse scaled 1/n."""
+
+    def __init__(self, n: int):
+        self._lib = load()
+        self._h = self._lib.atkdsp_fft_create(int(n))
+        if not self._h:
+            raise AtkDspError(f"fft_create({n}) failed (no GPU or out of device memory)")
+        self.n = int(n)
+
+    def close(self) -> None:
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._lib.atkdsp_fft_destroy(h)
+
+    def __del__(self):
+        self.close()
+
+    def exec(self, x, inverse: bool = False, out: np.ndarray | None = None) -> np.ndarray:
+        x = _cf32(x, "x")
+        if x.size != self.n:
+            raise AtkDspError(f"fft: expected {self.n} samples, got {x.size}")
+        if out is None:
+            out = np.empty(self.n, dtype=np.complex64)
+        _check(self._lib.atkdsp_fft_exec(self._h, _cfp(x), _cfp(out), 1 if inverse else 0), "fft_exec")
+        return out
+
+    def power_db(self, x, win=None, out: np.ndarray | None = None) -> np.ndarray:
+        x = _cf32(x, "x")
+        if x.size != self.n:
+            raise AtkDspError(f"power_db: expected {self.n} samples, got {x.size}")
+        w = None if win is None else _f32(win, "win")
+        if out is None:
+            out = np.empty(self.n, dtype=np.float32)
+        _check(self._lib.atkdsp_power_db(self._h, _cfp(x), None if w is None else _fp(w), _fp(out)),
+               "power_db")
+        return out
+
+    def spectrum_reduce(self, x, hop: int | None = None, win=None, detector="max",
+                        out: np.ndarray | None = None):
+        """Every frame of x reduced per bin into one dB line. Returns (line, frames)."""
+        x = _cf32(x, "x")
+        w = None if win is None else _f32(win, "win")
+        if out is None:
+            out = np.empty(self.n, dtype=np.float32)
+        det = DET[detector] if isinstance(detector, str) else int(detector)
+        frames = _check(self._lib.atkdsp_spectrum_reduce(
+            self._h, _cfp(x), x.size, int(hop or self.n), None if w is None else _fp(w), det,
+            _fp(out)), "spectrum_reduce")
+        return out, int(frames)
+
+    def spectrum_stats(self, x, hop: int | None = None, win=None,
+                       want=("max", "avg", "min", "sk")):
+        """Per-bin max/avg/min (dB) and spectral kurtosis in one pass; returns a
+        dict of the requested arrays plus ``frames``."""
+        x = _cf32(x, "x")
+        w = None if win is None else _f32(win, "win")
+        want = set(want)
+        outs = {k: (np.empty(self.n, dtype=np.float32) if k in want else None)
+                for k in ("max", "avg", "min", "sk")}
+        ptr = {k: (_fp(v) if v is not None else None) for k, v in outs.items()}
+        frames = _check(self._lib.atkdsp_spectrum_stats(
+            self._h, _cfp(x), x.size, int(hop or self.n), None if w is None else _fp(w),
+            ptr["max"], ptr["avg"], ptr["min"], ptr["sk"]), "spectrum_stats")
+        res = {k: v for k, v in outs.items() if v is not None}
+        res["frames"] = int(frames)
+        return res
+
+
+def window(kind, n: int) -> np.ndarray:
+    k = WIN[kind] if isinstance(kind, str) else int(kind)
+    out = np.empty(int(n), dtype=np.float32)
+    _check(load().atkdsp_window(k, int(n), _fp(out)), "window")
+    return out
+
+
+def power_db(x, win=None) -> np.ndarray:
+    x = _cf32(x, "x")
+    return Fft(x.size).power_db(x, win)
+
+
+def spectrum_reduce(x, n: int, hop: int | None = None, win=None, detector="max"):
+    return Fft(n).spectrum_reduce(x, hop, win, detector)
+
+
+def spectrum_stats(x, n: int, hop: int | None = None, win=None, want=("max", "avg", "min", "sk")):
+    return Fft(n).spectrum_stats(x, hop, win, want)
+
+
+def window_stats(win) -> tuple:
+    """(coherent_gain, enbw_bins) of a window array."""
+    w = _f32(win, "win")
+    cg, enbw = C.c_double(0.0), C.c_double(0.0)
+    _check(load().atkdsp_window_stats(_fp(w), w.size, C.byref(cg), C.byref(enbw)), "window_stats")
+    return cg.value, enbw.value
+
+
+# ---- display ----------------------------------------------------------------------
+def db_to_pixels(db, lo: float, hi: float, lut256, out: np.ndarray | None = None) -> np.ndarray:
+    db = _f32(db, "db")
+    lut = np.ascontiguousarray(lut256, dtype=np.uint32)
+    if lut.size != 256:
+        raise AtkDspError("lut256 must have 256 entries")
+    if out is None:
+        out = np.empty(db.size, dtype=np.uint32)
+    load().atkdsp_db_to_pixels(_fp(db), db.size, float(lo), float(hi),
+                               lut.ctypes.data_as(C.POINTER(C.c_uint32)),
+                               out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def decimate_max(x, m: int, out: np.ndarray | None = None) -> np.ndarray:
+    """Peak-preserving reduction of x to m columns (m > len(x) pads with the last value)."""
+    x = _f32(x, "x")
+    m = int(m)
+    if out is None:
+        out = np.empty(m, dtype=np.float32)
+    load().atkdsp_decimate_max(_fp(x), x.size, _fp(out), m)
+    return out
+
+
+def stitch_max(seg, seg_lo_hz: float, seg_hz_per_bin: float, keep_lo_hz: float,
+               keep_hi_hz: float, out_lo_hz: float, out_hz_per_bin: float,
+               out: np.ndarray) -> np.ndarray:
+    """Max-accumulate one segment into ``out`` (float32, NaN = unmeasured), in place."""
+    seg = _f32(seg, "seg")
+    if out.dtype != np.float32 or not out.flags.c_contiguous:
+        raise AtkDspError("out must be a contiguous float32 array")
+    load().atkdsp_stitch_max(_fp(seg), seg.size, float(seg_lo_hz), float(seg_hz_per_bin),
+                             float(keep_lo_hz), float(keep_hi_hz), float(out_lo_hz),
+                             float(out_hz_per_bin), _fp(out), out.size)
+    return out
+
+
+# ---- polyphase filterbank --------------------------------------------------------------
+class Pfb:
+    """Oversampled polyphase filterbank: nchan channels spaced fs/nchan,
+    `oversample` x; process_n runs N element streams on one block clock."""
+
+    def __init__(self, fs, nchan, oversample=2, taps_per_poly=12, atten_db=80.0, max_nel=1):
+        self._lib = load()
+        self._p = self._lib.atkdsp_pfb_create(float(fs), int(nchan), int(oversample),
+                                              int(taps_per_poly), float(atten_db), int(max_nel))
+        if not self._p:
+            raise AtkDspError("pfb_create: check fs>0, nchan>=2, nchan%oversample==0 (and a GPU)")
+        self.nchan = int(nchan)
+        self.max_nel = int(max_nel)
+
+    def close(self) -> None:
+        p, self._p = getattr(self, "_p", None), None
+        if p:
+            self._lib.atkdsp_pfb_destroy(p)
+
+    def __del__(self):
+        self.close()
+
+    def reset(self):
+        self._lib.atkdsp_pfb_reset(self._p)
+
+    def out_rate(self):
+        return self._lib.atkdsp_pfb_out_rate(self._p)
+
+    def group_delay(self):
+        return self._lib.atkdsp_pfb_group_delay(self._p)
+
+    def block_origin(self, hop_index):
+        return int(self._lib.atkdsp_pfb_block_origin(self._p, int(hop_index)))
+
+    def out_hops(self, n):
+        return int(self._lib.atkdsp_pfb_out_hops(self._p, int(n)))
+
+    def set_cal(self, table, nel):
+        if table is None:
+            _check(self._lib.atkdsp_pfb_set_cal(self._p, None, 0), "pfb_set_cal")
+            return
+        t = np.ascontiguousarray(np.asarray(table, np.complex64)).reshape(int(nel), self.nchan)
+        _check(self._lib.atkdsp_pfb_set_cal(self._p, _cfp(t), int(nel)), "pfb_set_cal")
+
+    def describe(self):
+        buf = C.create_string_buffer(256)
+        n = self._lib.atkdsp_pfb_describe(self._p, buf, 256)
+        return buf.value.decode() if n > 0 else ""
+
+    def process_n(self, ins, nel):
+        nel = int(nel)
+        arrs = [np.ascontiguousarray(np.asarray(ins[e]).ravel(), np.complex64) for e in range(nel)]
+        n = arrs[0].size
+        for a in arrs:
+            if a.size != n:
+                raise AtkDspError("pfb: elements must be equal length")
+        m = self._lib.atkdsp_pfb_out_hops(self._p, n)
+        out = np.zeros(nel * self.nchan * m, np.complex64)
+        inarr = (C.POINTER(_Cf32) * nel)(*[_cfp(a) for a in arrs])
+        got = _check(self._lib.atkdsp_pfb_process_n(self._p, inarr, nel, n, _cfp(out), out.size),
+                     "pfb_process_n")
+        return out[: nel * self.nchan * got].reshape(nel, self.nchan, got)
+
+    def process(self, x):
+        return self.process_n([x], 1)[0]
+
+
+def pfb_cov_accumulate(out, nel, nchan, gate, R, counts):
+    """W1: R[k] += sum_i y_k y_k^H over gated channels; counts[k] += m (in place)."""
+    lib = load()
+    o = np.ascontiguousarray(out, np.complex64)
+    nel, nchan = int(nel), int(nchan)
+    m = o.shape[2] if o.ndim == 3 else 0
+    if not (R.flags["C_CONTIGUOUS"] and R.dtype == np.complex64):
+        raise AtkDspError("pfb_cov_accumulate: R must be contiguous complex64")
+    if not (counts.flags["C_CONTIGUOUS"] and counts.dtype == np.uint32):
+        raise AtkDspError("pfb_cov_accumulate: counts must be contiguous uint32")
+    gp = None
+    if gate is not None:
+        g = np.ascontiguousarray(gate, np.uint8)
+        gp = g.ctypes.data_as(C.POINTER(C.c_uint8))
+    lib.atkdsp_pfb_cov_accumulate(_cfp(o.reshape(-1)), nel, nchan, m, gp, _cfp(R.reshape(-1)),
+                                  counts.ctypes.data_as(C.POINTER(C.c_uint32)))
+
+
+def pfb_cov_reset(R, counts, nel, nchan, gate=None):
+    lib = load()
+    gp = None
+    if gate is not None:
+        g = np.ascontiguousarray(gate, np.uint8)
+        gp = g.ctypes.data_as(C.POINTER(C.c_uint8))
+    lib.atkdsp_pfb_cov_reset(_cfp(R.reshape(-1)), counts.ctypes.data_as(C.POINTER(C.c_uint32)),
+                             int(nel), int(nchan), gp)
+
+
+def pfb_channel_power(out, nel, nchan):
+    """W2: returns (pwr[nel, nchan], mean_pwr[nchan]) float32."""
+    lib = load()
+    o = np.ascontiguousarray(out, np.complex64)
+    nel, nchan = int(nel), int(nchan)
+    m = o.shape[2] if o.ndim == 3 else 0
+    pwr = np.zeros(nel * nchan, np.float32)
+    meanp = np.zeros(nchan, np.float32)
+    lib.atkdsp_pfb_channel_power(_cfp(o.reshape(-1)), nel, nchan, m, _fp(pwr), _fp(meanp))
+    return pwr.reshape(nel, nchan), meanp
+
+
+__all__ = [
+    "ABI_VERSION", "AtkDspError", "load", "available", "lib_path", "version", "build_info",
+    "set_threads", "get_threads", "FMT", "DET", "WIN", "bytes_per_sample",
+    "unpack", "unpack_dc", "Fft", "window", "power_db", "spectrum_reduce", "spectrum_stats",
+    "window_stats", "db_to_pixels", "decimate_max", "stitch_max",
+    "Pfb", "pfb_cov_accumulate", "pfb_cov_reset", "pfb_channel_power",
+]

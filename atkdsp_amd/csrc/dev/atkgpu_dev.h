@@ -1,0 +1,113 @@
+/*
+ * atkgpu_dev.h — the thin C ABI between the C host layer (atkdsp_* entry
+ * points, src in ../host) and the HIP kernels for gfx950 (this directory).
+ *
+ * Plain C: device pointers are void*, streams and plans are opaque. Every
+ * launcher returns 0 or a negative ATKDSP_E_* code (E_FFT for a failed HIP
+ * call). Nothing here allocates except the *_create / alloc functions, which
+ * the host layer only calls from its own *_create paths or a one-time init.
+ */
+#ifndef ATKGPU_DEV_H
+#define ATKGPU_DEV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- runtime ------------------------------------------------------------ */
+int         atkgpu_init(void);                 /* 0, or E_FFT when no device */
+const char *atkgpu_device_name(void);
+void       *atkgpu_dmalloc(size_t bytes);      /* device memory, NULL on failure */
+void        atkgpu_dfree(void *p);
+void       *atkgpu_hmalloc(size_t bytes);      /* pinned host memory */
+void        atkgpu_hfree(void *p);
+int         atkgpu_stream_create(void **s);    /* default priority */
+void        atkgpu_stream_destroy(void *s);
+int         atkgpu_sync(void *s);
+int         atkgpu_h2d(void *s, void *dst, const void *src, size_t bytes);
+int         atkgpu_d2h(void *s, void *dst, const void *src, size_t bytes);
+int         atkgpu_d2d(void *s, void *dst, const void *src, size_t bytes);
+int         atkgpu_d2h_2d(void *s, void *dst, size_t dpitch, const void *src,
+                          size_t spitch, size_t width, size_t height);
+int         atkgpu_fill_u32(void *s, void *dst, uint32_t v, size_t count);
+int         atkgpu_event_create(void **e);
+void        atkgpu_event_destroy(void *e);
+int         atkgpu_event_record(void *e, void *s);
+int         atkgpu_event_elapsed_ms(void *e0, void *e1, float *ms);
+
+/* ---- unpack --------------------------------------------------------------- */
+/* number of double pairs in the mean partial buffer */
+#define ATKGPU_MEAN_PARTS 1024
+/* out[i] = ((float)raw - c) * k per component (cf32 input: x - off).
+ * When sum2 (2 doubles, device) is non-NULL it receives the FP64 sums of out,
+ * formed through partials (2*ATKGPU_MEAN_PARTS doubles) in a fixed order. */
+int atkgpu_unpack(void *s, const void *raw, int fmt, size_t n, float cre, float cim,
+                  float k, float off_re, float off_im, void *out,
+                  double *partials, double *sum2);
+/* one-pole DC block in place over cf32 x[n]: st <- st + a(x - st); x -= st.
+ * state (device, 2 floats) in/out. scratch: >= 4*(n/256+1)+16 doubles. */
+int atkgpu_dc_pole(void *s, void *x, size_t n, float alpha, float *state, double *scratch);
+
+/* ---- FFT plans -------------------------------------------------------------- */
+typedef struct atkgpu_fftplan atkgpu_fftplan;
+atkgpu_fftplan *atkgpu_fftplan_create(size_t n);
+void            atkgpu_fftplan_destroy(atkgpu_fftplan *p);
+/* device scratch (complex samples) one batch of `batch` transforms needs */
+size_t          atkgpu_fftplan_scratch(const atkgpu_fftplan *p, size_t batch);
+/* forward (inverse: conj trick and 1/n) transforms of `batch` inputs
+ * in + t*in_dist (contiguous, optionally times window) into out + t*n.
+ * in and out may alias only when in_dist == n and window == NULL. */
+int atkgpu_fft_batch(void *s, const atkgpu_fftplan *p, const void *in, size_t in_dist,
+                     const float *window, size_t batch, int inverse, void *out,
+                     void *scratch);
+
+/* ---- spectrum lines ----------------------------------------------------------- */
+/* acc planes (device): reduce uses float line[n]; stats uses float max, min and
+ * double sum, sum2 (see spectrum.hip). first != 0 starts a fresh accumulation. */
+int atkgpu_spec_accum(void *s, const void *X, size_t n, size_t frames, int detector,
+                      int first, float *line);
+int atkgpu_spec_reduce_db(void *s, const float *line, size_t n, int detector,
+                          long long frames, float *out_db);
+int atkgpu_spec_stats_accum(void *s, const void *X, size_t n, size_t frames, int first,
+                            float *mx, float *mn, double *su, double *s2);
+int atkgpu_spec_stats_db(void *s, size_t n, long long frames, const float *mx,
+                         const float *mn, const double *su, const double *s2,
+                         float *max_db, float *avg_db, float *min_db, float *sk);
+int atkgpu_power_db(void *s, const void *X, size_t n, float *out_db);
+
+/* ---- display ------------------------------------------------------------------ */
+int atkgpu_decimate_max(void *s, const float *in, size_t n, float *out, size_t m);
+int atkgpu_db_to_pixels(void *s, const float *db, size_t n, float lo, float hi,
+                        const uint32_t *lut, uint32_t *out);
+int atkgpu_stitch_max(void *s, const float *seg, size_t nseg, double seg_lo,
+                      double seg_step, double keep_lo, double keep_hi, double out_lo,
+                      double out_step, float *out, size_t nout);
+
+/* ---- polyphase filterbank -------------------------------------------------------- */
+/* fold: for hops i in [0, m) of element e, stream s_e = sbuf + e*spitch
+ * (hist ++ leftover ++ input, already contiguous on device):
+ *   fold[(e*m + i)*M + q] = sum_k h[kM + q] * s_e[i*hop + kM + q], k ascending (FP32) */
+int atkgpu_pfb_fold(void *s, const void *sbuf, size_t spitch, unsigned nel, size_t m,
+                    unsigned M, unsigned hop, unsigned K, const float *taps, void *fold);
+/* rotation exp(-2 pi i k ((b0+i+1) hop mod M)/M), cal[e*M+k] for e < cal_nel
+ * (FP32 complex), then out[(e*M + k)*m + i] (channel-major slab). */
+int atkgpu_pfb_rotate_store(void *s, const void *X, unsigned nel, size_t m, unsigned M,
+                            unsigned hop, long long b0, const void *twM, const void *cal,
+                            unsigned cal_nel, void *out);
+/* W1 / W2 over a device [nel][nchan][m] block */
+int atkgpu_pfb_cov(void *s, const void *y, unsigned nel, unsigned nchan, size_t m,
+                   const uint8_t *gate, void *R, uint32_t *counts);
+int atkgpu_pfb_cov_reset(void *s, void *R, uint32_t *counts, unsigned nel, unsigned nchan,
+                         const uint8_t *gate);
+int atkgpu_pfb_power(void *s, const void *y, unsigned nel, unsigned nchan, size_t m,
+                     float *pwr, float *mean_pwr);
+/* the plan's twiddle table (n complex floats, device) */
+const void *atkgpu_fftplan_twiddles(const atkgpu_fftplan *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

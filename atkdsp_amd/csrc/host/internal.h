@@ -1,0 +1,41 @@
+/* Host-side internals of the drop-in atkdsp library (C11). */
+#ifndef ATK_HOST_INTERNAL_H
+#define ATK_HOST_INTERNAL_H
+
+#include <stdatomic.h>
+#include <stddef.h>
+#include <stdint.h>
+#include "../../../include/atkdsp.h"
+#include "../dev/atkgpu_dev.h"
+
+#define ATK_PI 3.14159265358979323846
+
+/* A device slot: one stream plus fixed device buffers, claimed by CAS so that
+ * concurrent callers of a shared object never share a stream or a buffer
+ * (the reference claims CPU scratch slots the same way, src/internal.h:51-75,
+ * src/fft.c:74-88). Slots are made at create / one-time init, never per call. */
+typedef struct {
+    atomic_int busy;
+    void *stream;
+    void *d_a, *d_b, *d_c;     /* general device buffers */
+    size_t bytes_a, bytes_b, bytes_c;
+    void *d_small;             /* 1 MiB of small device scratch (lines, sums, luts) */
+} atk_slot;
+
+#define ATK_SMALL_BYTES (1u << 20)
+
+int  atk_slot_init(atk_slot *s, size_t a, size_t b, size_t c);
+void atk_slot_free(atk_slot *s);
+int  atk_claim(atk_slot *slots, int n);   /* spins until one is free; returns its index */
+void atk_release(atk_slot *slots, int i);
+
+/* The library-wide pool used by the stateless entry points (unpack, display,
+ * stitch, W1/W2). Made once, on first use. Returns NULL when there is no GPU. */
+typedef struct {
+    atk_slot slot[4];
+    int nslot;
+    size_t chunk;              /* samples a slot converts per pass */
+} atk_pool;
+atk_pool *atk_pool_get(void);
+
+#endif

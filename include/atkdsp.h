@@ -1,0 +1,155 @@
+/*
+ * atkdsp.h — the wideband front-end subset of the atkdsp C ABI (ABI 26,
+ * version "0.25.1"), implemented on one MI355X.
+ *
+ * Each declaration below replaces the reference function of the same name;
+ * the citation gives the reference declaration (include/atkdsp.h:LINE) and
+ * the reference implementation it follows (src/FILE:LINES). Signatures,
+ * argument meaning, return codes and streaming state are the reference's, so
+ * the reference ctypes binding (python/atkdsp/__init__.py) loads this library
+ * unchanged when ATKDSP_LIB points at it.
+ *
+ * Contract kept from the reference header (include/atkdsp.h:5-41):
+ *   - plain C calling convention, opaque handles, caller-owned host buffers;
+ *   - nothing in a *_process / *_exec / reduce call allocates: device
+ *     buffers, pinned staging and HIP streams are made in *_create;
+ *   - streaming state is explicit and block boundaries are invisible: the
+ *     filterbank's history lives in device memory and its hop counter is int64;
+ *   - NaN means "unmeasured" (no fast-math anywhere).
+ *
+ * The library also exports every other name of the reference ABI (334 in
+ * all) as a documented stub: *_create returns NULL, signed counts return
+ * ATKDSP_E_ARG, unsigned counts 0, floating results NaN, void functions do
+ * nothing. Those are outside the wideband front end.
+ */
+#ifndef ATKDSP_H
+#define ATKDSP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#define ATKDSP_API __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATKDSP_ABI_VERSION 26          /* reference include/atkdsp.h:63 */
+#define ATKDSP_VERSION_STRING "0.25.1" /* reference include/atkdsp.h:64 */
+
+/* error codes: reference include/atkdsp.h:202-206 */
+#define ATKDSP_OK       0
+#define ATKDSP_E_ARG   -1
+#define ATKDSP_E_NOMEM -2   /* also: device memory exhausted at create */
+#define ATKDSP_E_CAP   -3
+#define ATKDSP_E_FFT   -4   /* also: a HIP launch or copy failed       */
+
+/* complex float32, numpy complex64 layout: reference include/atkdsp.h:212 */
+typedef struct { float re, im; } atkdsp_cf32;
+
+/* reference include/atkdsp.h:215-235 */
+enum atkdsp_format { ATKDSP_FMT_CU8 = 0, ATKDSP_FMT_CI8 = 1, ATKDSP_FMT_CI16 = 2,
+                     ATKDSP_FMT_CI16Q11 = 3, ATKDSP_FMT_CF32 = 4 };
+enum atkdsp_detector { ATKDSP_DET_MAX = 0, ATKDSP_DET_MIN = 1, ATKDSP_DET_AVG = 2 };
+enum atkdsp_window { ATKDSP_WIN_RECT = 0, ATKDSP_WIN_HANN = 1, ATKDSP_WIN_HAMMING = 2,
+                     ATKDSP_WIN_BLACKMANHARRIS = 3 };
+
+/* ---- identity: reference include/atkdsp.h:238-245, src/atkdsp.c:43-80 ---- */
+ATKDSP_API int         atkdsp_abi_version(void);
+ATKDSP_API const char *atkdsp_version_string(void);
+/* contains "abi 26", then "hip gfx950" and the device name */
+ATKDSP_API const char *atkdsp_build_info(void);
+ATKDSP_API int         atkdsp_bytes_per_sample(int fmt);
+/* kept for the binding; the GPU result does not depend on it */
+ATKDSP_API int         atkdsp_set_threads(int n);
+ATKDSP_API int         atkdsp_get_threads(void);
+
+/* ---- unpack: reference include/atkdsp.h:252-254, src/convert.c:7-17,113-120.
+ * Bit-exact conversion; the one-pole DC block runs as a chunked scan whose
+ * chunk boundaries carry an FP64 state (within 2e-6 of the serial pole). */
+ATKDSP_API ptrdiff_t atkdsp_unpack(const void *raw, size_t nbytes, int fmt,
+                                   atkdsp_cf32 *out, size_t out_cap,
+                                   float dc_alpha, atkdsp_cf32 *dc_state);
+/* reference include/atkdsp.h:270-273, src/convert.c:33-111,122-128.
+ * Bit-exact; the mean is an FP64 fixed-order tree sum. */
+ATKDSP_API ptrdiff_t atkdsp_unpack_dc(const void *raw, size_t nbytes, int fmt,
+                                      atkdsp_cf32 *out, size_t out_cap,
+                                      float off_re, float off_im,
+                                      double *mean_re, double *mean_im);
+
+/* ---- FFT: reference include/atkdsp.h:384-389, src/fft.c:31-131.
+ * Any n >= 2. A plan may be shared by several caller threads: it owns a fixed
+ * pool of device slots (stream + buffers) claimed per call. */
+typedef struct atkdsp_fft atkdsp_fft;
+ATKDSP_API atkdsp_fft *atkdsp_fft_create(size_t n);
+ATKDSP_API void        atkdsp_fft_destroy(atkdsp_fft *p);
+ATKDSP_API size_t      atkdsp_fft_length(const atkdsp_fft *p);
+ATKDSP_API int         atkdsp_fft_exec(const atkdsp_fft *p, const atkdsp_cf32 *in,
+                                       atkdsp_cf32 *out, int inverse);
+/* reference include/atkdsp.h:392, src/fft.c:133-152 (create-time, host) */
+ATKDSP_API int atkdsp_window(int kind, size_t n, float *out);
+/* reference include/atkdsp.h:397-398, src/fft.c:155-174 */
+ATKDSP_API int atkdsp_power_db(const atkdsp_fft *p, const atkdsp_cf32 *in,
+                               const float *window, float *out_db);
+/* reference include/atkdsp.h:406-409, src/fft.c:177-295 */
+ATKDSP_API ptrdiff_t atkdsp_spectrum_reduce(const atkdsp_fft *p, const atkdsp_cf32 *in,
+                                            size_t n_samples, size_t hop,
+                                            const float *window, int detector,
+                                            float *out_line);
+/* reference include/atkdsp.h:428-432, src/fft.c:297-431 */
+ATKDSP_API ptrdiff_t atkdsp_spectrum_stats(const atkdsp_fft *p, const atkdsp_cf32 *in,
+                                           size_t n_samples, size_t hop,
+                                           const float *window,
+                                           float *max_db, float *avg_db,
+                                           float *min_db, float *sk);
+/* reference include/atkdsp.h:441-442, src/fft.c:433-444 (host arithmetic) */
+ATKDSP_API int atkdsp_window_stats(const float *window, size_t n,
+                                   double *coherent_gain, double *enbw_bins);
+
+/* ---- display: reference include/atkdsp.h:459-464, src/display.c:4-37 ---- */
+ATKDSP_API void atkdsp_db_to_pixels(const float *db, size_t n, float lo, float hi,
+                                    const uint32_t *lut256, uint32_t *out);
+ATKDSP_API void atkdsp_decimate_max(const float *in, size_t n, float *out, size_t m);
+/* reference include/atkdsp.h:493-497, src/detect.c:116-134 */
+ATKDSP_API void atkdsp_stitch_max(const float *seg, size_t nseg,
+                                  double seg_lo_hz, double seg_hz_per_bin,
+                                  double keep_lo_hz, double keep_hi_hz,
+                                  double out_lo_hz, double out_hz_per_bin,
+                                  float *out, size_t nout);
+
+/* ---- polyphase filterbank: reference include/atkdsp.h:3191-3250,
+ * src/channelizer.c:79-471. Output layouts, bookkeeping, cal and W1/W2 are
+ * the reference's; the streaming history (ov samples per element) stays in
+ * device memory between calls. */
+typedef struct atkdsp_pfb atkdsp_pfb;
+ATKDSP_API atkdsp_pfb *atkdsp_pfb_create(double fs, unsigned nchan, unsigned oversample,
+                                         unsigned taps_per_poly, double atten_db,
+                                         unsigned max_nel);
+ATKDSP_API void        atkdsp_pfb_destroy(atkdsp_pfb *p);
+ATKDSP_API void        atkdsp_pfb_reset(atkdsp_pfb *p);
+ATKDSP_API unsigned    atkdsp_pfb_nchan(const atkdsp_pfb *p);
+ATKDSP_API double      atkdsp_pfb_out_rate(const atkdsp_pfb *p);
+ATKDSP_API size_t      atkdsp_pfb_out_hops(const atkdsp_pfb *p, size_t n);
+ATKDSP_API ptrdiff_t   atkdsp_pfb_process(atkdsp_pfb *p, const atkdsp_cf32 *in, size_t n,
+                                          atkdsp_cf32 *out, size_t cap);
+ATKDSP_API ptrdiff_t   atkdsp_pfb_process_n(atkdsp_pfb *p, const atkdsp_cf32 *const *ins,
+                                            unsigned nel, size_t n,
+                                            atkdsp_cf32 *out, size_t cap);
+ATKDSP_API int64_t     atkdsp_pfb_block_origin(const atkdsp_pfb *p, int64_t hop_index);
+ATKDSP_API double      atkdsp_pfb_group_delay(const atkdsp_pfb *p);
+ATKDSP_API int         atkdsp_pfb_set_cal(atkdsp_pfb *p, const atkdsp_cf32 *table, unsigned nel);
+ATKDSP_API int         atkdsp_pfb_describe(const atkdsp_pfb *p, char *buf, size_t cap);
+ATKDSP_API void        atkdsp_pfb_cov_accumulate(const atkdsp_cf32 *out, unsigned nel,
+                                                 unsigned nchan, size_t m,
+                                                 const uint8_t *gate,
+                                                 atkdsp_cf32 *R, uint32_t *counts);
+ATKDSP_API void        atkdsp_pfb_cov_reset(atkdsp_cf32 *R, uint32_t *counts, unsigned nel,
+                                            unsigned nchan, const uint8_t *gate);
+ATKDSP_API void        atkdsp_pfb_channel_power(const atkdsp_cf32 *out, unsigned nel,
+                                                unsigned nchan, size_t m,
+                                                float *pwr, float *mean_pwr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ATKDSP_H */

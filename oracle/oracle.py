@@ -1,0 +1,419 @@
+"""The oracle: a CPU restatement of the reference's wideband front end.
+
+TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and bench.py's
+CPU-baseline leg may import this module, and only as the checker. The product
+(atkdsp_amd) never imports it.
+
+Each function restates the reference C (file:line given) in numpy:
+  - byte/integer work and the display helpers are restated exactly in float32
+    (bit-identical to the reference: unpack, decimate_max, db_to_pixels,
+    stitch_max);
+  - the filterbank, FFT and W1/W2 are restated in FP64, the "FP64 twin" the
+    reference's own tests measure against (tests/test_channelizer.py:397-398);
+  - the spectrum line restates the reference's mixed arithmetic: window and
+    FFT in FP64, power and detector in float32 (src/fft.c:91-108, :177-191),
+    stats planes in FP64 (src/fft.c:317-331).
+
+How each function is pinned (tests/test_oracle.py): against golden vectors
+written by the compiled reference (tests/golden/, made by
+tests/golden/make_golden.py from oracle/_ref), against the reference's own
+known-answer tests (tests/test_smoke.c lines cited there), and against closed
+forms. No function here is unpinned.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+F32 = np.float32
+BPS = {0: 2, 1: 2, 2: 4, 3: 4, 4: 8}
+CU8, CI8, CI16, CI16Q11, CF32 = 0, 1, 2, 3, 4
+
+
+# ---- unpack: src/convert.c:33-111 ------------------------------------------------
+def unpack_consts(fmt: int, off: complex):
+    """The folded per-format constants, formed in float32 as src/convert.c:46-68."""
+    ore, oim = F32(off.real), F32(off.imag)
+    if fmt == CU8:
+        k = F32(1.0) / F32(127.5)
+        return F32(127.5) + ore / k, F32(127.5) + oim / k, k
+    if fmt == CI8:
+        k = F32(1.0) / F32(128.0)
+    elif fmt == CI16:
+        k = F32(1.0) / F32(32768.0)
+    elif fmt == CI16Q11:
+        k = F32(1.0) / F32(2048.0)
+    else:
+        return F32(0), F32(0), F32(1)
+    return ore / k, oim / k, k
+
+
+def unpack_dc(raw: bytes | np.ndarray, fmt: int, off: complex = 0j, out_cap: int | None = None):
+    """Returns (out complex64, mean complex) exactly as atkdsp_unpack_dc."""
+    b = np.frombuffer(bytes(raw), np.uint8) if not isinstance(raw, np.ndarray) else raw.view(np.uint8).ravel()
+    n = b.size // BPS[fmt]
+    if out_cap is not None:
+        n = min(n, out_cap)
+    if fmt == CU8:
+        v = b[: 2 * n].astype(F32)
+    elif fmt == CI8:
+        v = b[: 2 * n].view(np.int8).astype(F32)
+    elif fmt in (CI16, CI16Q11):
+        v = b[: 4 * n].view("<i2").astype(F32)
+    elif fmt == CF32:
+        v = b[: 8 * n].view("<f4").copy()
+    else:
+        raise ValueError("format")
+    if fmt == CF32:
+        re, im = v[0::2], v[1::2]
+        if not (off.real == 0 and off.imag == 0):
+            re = re - F32(off.real)
+            im = im - F32(off.imag)
+    else:
+        cre, cim, k = unpack_consts(fmt, off)
+        re = (v[0::2] - cre) * k
+        im = (v[1::2] - cim) * k
+    out = np.empty(n, np.complex64)
+    out.real = re
+    out.imag = im
+    return out, mean_of(out, off)
+
+
+def mean_of(out: np.ndarray, off: complex = 0j) -> complex:
+    """Eight FP64 lane sums, tail into lane 0, lanes added in order, then
+    sum/n + off (src/convert.c:91-108)."""
+    n = out.size
+    if n == 0:
+        return complex(float(F32(off.real)), float(F32(off.imag)))
+    m = n & ~7
+    parts = []
+    for comp in (out.real.astype(np.float64), out.imag.astype(np.float64)):
+        lanes = np.zeros(8)
+        if m:
+            lanes = np.cumsum(comp[:m].reshape(-1, 8), axis=0)[-1].copy()
+        for x in comp[m:]:
+            lanes[0] += x
+        s = 0.0
+        for l in range(8):
+            s += lanes[l]
+        parts.append(s)
+    inv = 1.0 / n
+    return complex(parts[0] * inv + float(F32(off.real)), parts[1] * inv + float(F32(off.imag)))
+
+
+def unpack(raw, fmt: int, dc_alpha: float = 0.0, state: complex = 0j):
+    """atkdsp_unpack with the one-pole DC block restated in FP64
+    (src/convert.c:7-17, :113-120). Returns (out complex128, new state)."""
+    x, _ = unpack_dc(raw, fmt, 0j)
+    if not dc_alpha > 0:
+        return x.astype(np.complex128), state
+    from scipy.signal import lfilter
+    a = float(F32(dc_alpha))
+    xd = x.astype(np.complex128)
+    # s_i = (1-a) s_{i-1} + a x_i, y_i = x_i - s_i
+    s, _ = lfilter([a], [1.0, -(1.0 - a)], xd, zi=np.array([(1.0 - a) * state]))
+    return xd - s, complex(s[-1]) if s.size else state
+
+
+# ---- windows: src/fft.c:133-152, :433-444 ---------------------------------------------
+def window(kind: int, n: int) -> np.ndarray:
+    if n == 1:
+        return np.ones(1, F32)
+    x = 2.0 * math.pi * np.arange(n) / (n - 1)
+    if kind == 0:
+        w = np.ones(n)
+    elif kind == 1:
+        w = 0.5 - 0.5 * np.cos(x)
+    elif kind == 2:
+        w = 0.54 - 0.46 * np.cos(x)
+    elif kind == 3:
+        w = 0.35875 - 0.48829 * np.cos(x) + 0.14128 * np.cos(2 * x) - 0.01168 * np.cos(3 * x)
+    else:
+        raise ValueError("kind")
+    return w.astype(F32)
+
+
+def window_stats(w: np.ndarray):
+    w = w.astype(np.float64)
+    s, s2 = w.sum(), (w * w).sum()
+    return s / w.size, (w.size * s2 / (s * s) if s > 0 else 0.0)
+
+
+# ---- FFT: FP64 (pocketfft semantics, src/fft.c:91-131) --------------------------------
+def fft(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    xd = np.asarray(x, np.complex128)
+    return np.fft.ifft(xd) if inverse else np.fft.fft(xd)
+
+
+def power_db(x: np.ndarray, win: np.ndarray | None) -> np.ndarray:
+    """src/fft.c:155-174: FP64 window and FFT, float32 magnitude and dB, fftshifted."""
+    n = x.size
+    xd = np.asarray(x, np.complex64).astype(np.complex128)
+    if win is not None:
+        xd = xd * win.astype(np.float64)
+    X = np.fft.fft(xd)
+    k = (np.arange(n) + n // 2) % n
+    re, im = X.real[k].astype(F32), X.imag[k].astype(F32)
+    mag = np.sqrt(re * re + im * im) * (F32(1.0) / F32(n))
+    return (F32(20.0) * np.log10(mag + F32(1e-12))).astype(F32)
+
+
+def _frames(x: np.ndarray, n: int, hop: int) -> int:
+    if hop == 0 or hop > n:
+        hop = n
+    if x.size < n:
+        return 0, hop
+    return (x.size - n) // hop + 1, hop
+
+
+def _frame_ffts(x, n, hop, win, frames, chunk=256):
+    xd = np.asarray(x, np.complex64)
+    wd = None if win is None else win.astype(np.float64)
+    for f0 in range(0, frames, chunk):
+        f1 = min(frames, f0 + chunk)
+        idx = (np.arange(f0, f1)[:, None] * hop) + np.arange(n)[None, :]
+        blk = xd[idx].astype(np.complex128)
+        if wd is not None:
+            blk = blk * wd
+        yield np.fft.fft(blk, axis=1)
+
+
+def spectrum_reduce(x: np.ndarray, n: int, hop: int | None, win, detector: int):
+    """src/fft.c:177-295: returns (line float32, frames)."""
+    frames, hop = _frames(x, n, hop or n)
+    line = np.zeros(n, F32)
+    if frames == 0:
+        return line, 0
+    k = (np.arange(n) + n // 2) % n
+    inv2 = F32(1.0) / (F32(n) * F32(n))
+    acc = None
+    for X in _frame_ffts(x, n, hop, win, frames):
+        re = X.real[:, k].astype(F32)
+        im = X.imag[:, k].astype(F32)
+        pw = (re * re + im * im) * inv2
+        for row in pw:
+            if acc is None:
+                acc = row.copy()
+            elif detector == 0:
+                acc = np.where(row > acc, row, acc)
+            elif detector == 1:
+                acc = np.where(row < acc, row, acc)
+            else:
+                acc = acc + row
+    scale = F32(1.0) / F32(frames) if detector == 2 else F32(1.0)
+    mag = np.sqrt(acc * scale)
+    return (F32(20.0) * np.log10(mag + F32(1e-12))).astype(F32), frames
+
+
+def spectrum_stats(x: np.ndarray, n: int, hop: int | None, win):
+    """src/fft.c:297-431: FP64 planes; returns dict max/avg/min/sk and frames."""
+    frames, hop = _frames(x, n, hop or n)
+    if frames == 0:
+        return {"frames": 0}
+    k = (np.arange(n) + n // 2) % n
+    inv2 = 1.0 / (float(n) * float(n))
+    mx = mn = su = s2 = None
+    for X in _frame_ffts(x, n, hop, win, frames):
+        pw = (X.real[:, k] ** 2 + X.imag[:, k] ** 2) * inv2
+        bmx, bmn, bsu, bs2 = pw.max(0), pw.min(0), np.cumsum(pw, 0)[-1], np.cumsum(pw * pw, 0)[-1]
+        if mx is None:
+            mx, mn, su, s2 = bmx, bmn, bsu, bs2
+        else:
+            mx, mn, su, s2 = np.maximum(mx, bmx), np.minimum(mn, bmn), su + bsu, s2 + bs2
+    M = float(frames)
+    db = lambda v: (F32(10.0) * np.log10(v.astype(F32) + F32(1e-12))).astype(F32)
+    den = su * su
+    sk = np.where((frames < 2) | (den <= 0), 1.0, M * s2 / np.where(den > 0, den, 1.0)).astype(F32)
+    return {"max": db(mx), "min": db(mn), "avg": db(su / M), "sk": sk, "frames": frames}
+
+
+# ---- display: src/display.c:4-37, stitch: src/detect.c:116-134 -----------------------------
+def db_to_pixels(db: np.ndarray, lo: float, hi: float, lut: np.ndarray) -> np.ndarray:
+    db = np.asarray(db, F32)
+    lo, hi = F32(lo), F32(hi)
+    span = F32(hi - lo)
+    k = F32(255.0) / span if span > 0 else F32(0.0)
+    with np.errstate(invalid="ignore"):
+        t = (db - lo) * k
+        t = np.where(t < 0, F32(0), t)
+        t = np.where(t > 255, F32(255), t)
+        idx = np.where(np.isnan(db), 0, np.floor(np.where(np.isnan(t), 0, t) + F32(0.5))).astype(np.int64)
+    return np.asarray(lut, np.uint32)[idx]
+
+
+def decimate_max(x: np.ndarray, m: int) -> np.ndarray:
+    x = np.asarray(x, F32)
+    n = x.size
+    out = np.empty(m, F32)
+    if m >= n:
+        out[:n] = x
+        out[n:] = x[n - 1]
+        return out
+    per = n // m
+    for c in range(m):
+        a = c * per
+        b = n if c + 1 == m else a + per
+        best = x[a]
+        for v in x[a + 1:b]:
+            if v > best:
+                best = v
+        out[c] = best
+    return out
+
+
+def stitch_max(seg, seg_lo, seg_step, keep_lo, keep_hi, out_lo, out_step, out: np.ndarray) -> np.ndarray:
+    seg = np.asarray(seg, F32)
+    nout = out.size
+    if nout == 0 or seg_step <= 0 or out_step <= 0:
+        return out
+    out_hi = out_lo + out_step * float(nout)
+    for i in range(seg.size):
+        f = seg_lo + (float(i) + 0.5) * seg_step
+        if f < keep_lo or f > keep_hi or f < out_lo or f > out_hi:
+            continue
+        k = int((f - out_lo) / out_step)       # truncation toward zero, as the C cast
+        k = min(max(k, 0), nout - 1)
+        v, cur = seg[i], out[k]
+        if cur != cur or v > cur:
+            out[k] = v
+    return out
+
+
+# ---- polyphase filterbank: src/channelizer.c:33-43, :79-99, :218-402 --------------------------
+def kaiser_beta(a: float) -> float:
+    """src/design.c:18-22."""
+    if a > 50.0:
+        return 0.1102 * (a - 8.7)
+    if a >= 21.0:
+        return 0.5842 * (a - 21.0) ** 0.4 + 0.07886 * (a - 21.0)
+    return 0.0
+
+
+def _i0_series(x: np.ndarray) -> np.ndarray:
+    """I0 by its power series with the reference's per-element stopping rule."""
+    x = np.asarray(x, np.float64)
+    s = np.ones_like(x)
+    term = np.ones_like(x)
+    y = x * 0.5
+    live = np.ones(x.shape, bool)
+    for k in range(1, 500):
+        term = np.where(live, term * (y / k), term)
+        t2 = term * term
+        s = np.where(live, s + t2, s)
+        live &= ~(t2 < s * 1e-17)
+        if not live.any():
+            break
+    return s
+
+
+def pfb_prototype(nchan: int, taps_per_poly: int, atten_db: float) -> np.ndarray:
+    """Kaiser-windowed sinc, fc = 1/(2 nchan), unit DC gain times nchan (FP64)."""
+    atten_db = max(atten_db, 20.0)
+    L = nchan * taps_per_poly
+    fc = 1.0 / (2.0 * nchan)
+    Mid = (L - 1) * 0.5
+    beta = kaiser_beta(atten_db)
+    t = np.arange(L) - Mid
+    x = 2.0 * fc * t
+    with np.errstate(invalid="ignore", divide="ignore"):
+        sinc = np.where(t == 0.0, 1.0, np.sin(math.pi * x) / (math.pi * x))
+    r = t / Mid if Mid > 0 else np.zeros(L)
+    w = _i0_series(beta * np.sqrt(np.where(r * r < 1.0, 1.0 - r * r, 0.0))) / _i0_series(np.array(beta))
+    h = 2.0 * fc * sinc * w
+    s = h.sum()
+    if s == 0.0:
+        s = 1.0
+    return h / s * nchan
+
+
+class Pfb:
+    """FP64 streaming filterbank with the reference's state machine:
+    s = hist(ov) ++ leftover(lc) ++ in; hop i's window s[i hop, i hop + L);
+    fold over k ascending; forward FFT; rotation by exp(-2 pi i k shift/M),
+    shift = ((b + i + 1) hop) mod M; cal; output [el][chan][hop]."""
+
+    def __init__(self, fs, nchan, oversample=2, taps_per_poly=12, atten_db=80.0, max_nel=1):
+        self.M, self.os, self.K = int(nchan), int(oversample), int(taps_per_poly)
+        self.hop = self.M // self.os
+        self.L = self.M * self.K
+        self.ov = self.L - self.hop
+        self.h = pfb_prototype(self.M, self.K, float(atten_db))
+        self.max_nel = int(max_nel)
+        self.hist = np.zeros((self.max_nel, self.ov), np.complex128)
+        self.left = np.zeros((self.max_nel, 0), np.complex128)
+        self.b = 0
+        self.cal = None
+        self.fs = fs
+
+    def set_cal(self, table, nel):
+        self.cal = None if table is None else np.asarray(table, np.complex128).reshape(nel, self.M)
+
+    def out_rate(self):
+        return self.os * self.fs / self.M
+
+    def process_n(self, ins, nel):
+        M, hop, L, ov, K = self.M, self.hop, self.L, self.ov, self.K
+        xs = [np.asarray(ins[e], np.complex64).astype(np.complex128).ravel() for e in range(nel)]
+        n = xs[0].size
+        lc = self.left.shape[1]
+        m = (lc + n) // hop
+        outs = []
+        hk = self.h.reshape(K, M)
+        newhist, newleft = [], []
+        for e in range(nel):
+            s = np.concatenate([self.hist[e], self.left[e], xs[e]])
+            if m:
+                st = s.strides[0]
+                W = np.lib.stride_tricks.as_strided(s, shape=(m, K, M), strides=(hop * st, M * st, st))
+                fold = np.einsum("ikq,kq->iq", W, hk)
+                X = np.fft.fft(fold, axis=1)
+                shift = (((self.b + np.arange(m) + 1) * hop) % M)[:, None]
+                idx = (np.arange(M)[None, :] * shift) % M
+                X = X * np.exp(-2j * np.pi * idx / M)
+                if self.cal is not None and e < self.cal.shape[0]:
+                    X = X * self.cal[e][None, :]
+                outs.append(X.T)
+            else:
+                outs.append(np.zeros((M, 0), np.complex128))
+            newhist.append(s[m * hop: m * hop + ov])
+            newleft.append(s[m * hop + ov:])
+        for e in range(nel):
+            self.hist[e] = newhist[e]
+        self.left = np.zeros((self.max_nel, len(newleft[0])), np.complex128)
+        for e in range(nel):
+            self.left[e] = newleft[e]
+        self.b += m
+        return np.stack(outs)
+
+    def process(self, x):
+        return self.process_n([x], 1)[0]
+
+
+def pfb_cov_accumulate(out, nel, nchan, gate, R, counts):
+    """W1 in FP64 (src/channelizer.c:420-440): R[k] += y y^H over hops."""
+    o = np.asarray(out, np.complex128)
+    m = o.shape[2]
+    for k in range(nchan):
+        if gate is not None and not gate[k]:
+            continue
+        y = o[:, k, :]
+        R[k] += y @ y.conj().T
+        counts[k] += m
+
+
+def pfb_channel_power(out, nel, nchan):
+    """W2 in FP64 (src/channelizer.c:453-471)."""
+    o = np.asarray(out, np.complex128)
+    m = o.shape[2]
+    p = (np.abs(o) ** 2).sum(axis=2) / m if m else np.zeros((nel, nchan))
+    return p, p.mean(axis=0)
+
+
+def rel_err(a, b) -> float:
+    """max |a - b| / max |b| over the whole output (the reference's measure,
+    tests/test_channelizer.py:88)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.max(np.abs(b)) if b.size else 0.0
+    return float(np.max(np.abs(a - b)) / den) if den > 0 else float(np.max(np.abs(a - b)) if a.size else 0.0)

@@ -1,0 +1,194 @@
+"""ctypes access to the reference library compiled from its own sources by
+oracle/build_ref.sh (oracle/_ref/atkdsp_ref.so).
+
+TEST INFRASTRUCTURE ONLY (golden vectors, pinning the numpy oracle, and the
+CPU baseline timing). Independent of the product binding on purpose.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+REF_SO = Path(__file__).resolve().parent / "_ref" / "atkdsp_ref.so"
+
+
+class Cf(C.Structure):
+    _fields_ = [("re", C.c_float), ("im", C.c_float)]
+
+
+def available() -> bool:
+    return REF_SO.exists()
+
+
+class Ref:
+    def __init__(self, path: Path = REF_SO):
+        L = C.CDLL(str(path))
+        P, cf, fp, vp, sz, u, d = C.POINTER, C.POINTER(Cf), C.POINTER(C.c_float), C.c_void_p, C.c_size_t, C.c_uint, C.c_double
+        sigs = {
+            "atkdsp_abi_version": (C.c_int, []),
+            "atkdsp_set_threads": (C.c_int, [C.c_int]),
+            "atkdsp_unpack": (C.c_ssize_t, [vp, sz, C.c_int, cf, sz, C.c_float, cf]),
+            "atkdsp_unpack_dc": (C.c_ssize_t, [vp, sz, C.c_int, cf, sz, C.c_float, C.c_float, P(d), P(d)]),
+            "atkdsp_fft_create": (vp, [sz]),
+            "atkdsp_fft_destroy": (None, [vp]),
+            "atkdsp_fft_exec": (C.c_int, [vp, cf, cf, C.c_int]),
+            "atkdsp_window": (C.c_int, [C.c_int, sz, fp]),
+            "atkdsp_power_db": (C.c_int, [vp, cf, fp, fp]),
+            "atkdsp_spectrum_reduce": (C.c_ssize_t, [vp, cf, sz, sz, fp, C.c_int, fp]),
+            "atkdsp_spectrum_stats": (C.c_ssize_t, [vp, cf, sz, sz, fp, fp, fp, fp, fp]),
+            "atkdsp_window_stats": (C.c_int, [fp, sz, P(d), P(d)]),
+            "atkdsp_db_to_pixels": (None, [fp, sz, C.c_float, C.c_float, P(C.c_uint32), P(C.c_uint32)]),
+            "atkdsp_decimate_max": (None, [fp, sz, fp, sz]),
+            "atkdsp_stitch_max": (None, [fp, sz, d, d, d, d, d, d, fp, sz]),
+            "atkdsp_pfb_create": (vp, [d, u, u, u, d, u]),
+            "atkdsp_pfb_destroy": (None, [vp]),
+            "atkdsp_pfb_out_hops": (sz, [vp, sz]),
+            "atkdsp_pfb_process_n": (C.c_ssize_t, [vp, P(cf), u, sz, cf, sz]),
+            "atkdsp_pfb_set_cal": (C.c_int, [vp, cf, u]),
+            "atkdsp_pfb_cov_accumulate": (None, [cf, u, u, sz, P(C.c_uint8), cf, P(C.c_uint32)]),
+            "atkdsp_pfb_channel_power": (None, [cf, u, u, sz, fp, fp]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        self.L = L
+
+    @staticmethod
+    def _cf(a):
+        return a.ctypes.data_as(C.POINTER(Cf))
+
+    @staticmethod
+    def _f(a):
+        return a.ctypes.data_as(C.POINTER(C.c_float))
+
+    def set_threads(self, n: int) -> int:
+        return self.L.atkdsp_set_threads(n)
+
+    def unpack_dc(self, raw: bytes, fmt: int, off: complex = 0j):
+        b = np.frombuffer(raw, np.uint8)
+        bps = {0: 2, 1: 2, 2: 4, 3: 4, 4: 8}[fmt]
+        out = np.empty(b.size // bps, np.complex64)
+        mr, mi = C.c_double(), C.c_double()
+        n = self.L.atkdsp_unpack_dc(b.ctypes.data, b.size, fmt, self._cf(out), out.size,
+                                    float(off.real), float(off.imag), C.byref(mr), C.byref(mi))
+        return out[:n], complex(mr.value, mi.value)
+
+    def unpack(self, raw: bytes, fmt: int, alpha: float, state: np.ndarray):
+        b = np.frombuffer(raw, np.uint8)
+        bps = {0: 2, 1: 2, 2: 4, 3: 4, 4: 8}[fmt]
+        out = np.empty(b.size // bps, np.complex64)
+        n = self.L.atkdsp_unpack(b.ctypes.data, b.size, fmt, self._cf(out), out.size, float(alpha),
+                                 self._cf(state) if state is not None else None)
+        return out[:n]
+
+    def fft(self, x: np.ndarray, inverse=False):
+        x = np.ascontiguousarray(x, np.complex64)
+        p = self.L.atkdsp_fft_create(x.size)
+        out = np.empty_like(x)
+        self.L.atkdsp_fft_exec(p, self._cf(x), self._cf(out), int(inverse))
+        self.L.atkdsp_fft_destroy(p)
+        return out
+
+    def window(self, kind, n):
+        out = np.empty(n, np.float32)
+        self.L.atkdsp_window(kind, n, self._f(out))
+        return out
+
+    def window_stats(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        a, b = C.c_double(), C.c_double()
+        self.L.atkdsp_window_stats(self._f(w), w.size, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def power_db(self, x, win):
+        x = np.ascontiguousarray(x, np.complex64)
+        p = self.L.atkdsp_fft_create(x.size)
+        out = np.empty(x.size, np.float32)
+        self.L.atkdsp_power_db(p, self._cf(x), None if win is None else self._f(win), self._f(out))
+        self.L.atkdsp_fft_destroy(p)
+        return out
+
+    def spectrum_reduce(self, x, n, hop, win, det):
+        x = np.ascontiguousarray(x, np.complex64)
+        p = self.L.atkdsp_fft_create(n)
+        out = np.empty(n, np.float32)
+        fr = self.L.atkdsp_spectrum_reduce(p, self._cf(x), x.size, hop or n,
+                                           None if win is None else self._f(win), det, self._f(out))
+        self.L.atkdsp_fft_destroy(p)
+        return out, fr
+
+    def spectrum_stats(self, x, n, hop, win):
+        x = np.ascontiguousarray(x, np.complex64)
+        p = self.L.atkdsp_fft_create(n)
+        o = {k: np.empty(n, np.float32) for k in ("max", "avg", "min", "sk")}
+        fr = self.L.atkdsp_spectrum_stats(p, self._cf(x), x.size, hop or n,
+                                          None if win is None else self._f(win),
+                                          self._f(o["max"]), self._f(o["avg"]), self._f(o["min"]),
+                                          self._f(o["sk"]))
+        self.L.atkdsp_fft_destroy(p)
+        o["frames"] = fr
+        return o
+
+    def db_to_pixels(self, db, lo, hi, lut):
+        db = np.ascontiguousarray(db, np.float32)
+        lut = np.ascontiguousarray(lut, np.uint32)
+        out = np.empty(db.size, np.uint32)
+        self.L.atkdsp_db_to_pixels(self._f(db), db.size, lo, hi, lut.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                   out.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return out
+
+    def decimate_max(self, x, m):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.empty(m, np.float32)
+        self.L.atkdsp_decimate_max(self._f(x), x.size, self._f(out), m)
+        return out
+
+    def stitch_max(self, seg, a, b, c, d, e, f, out):
+        seg = np.ascontiguousarray(seg, np.float32)
+        self.L.atkdsp_stitch_max(self._f(seg), seg.size, a, b, c, d, e, f, self._f(out), out.size)
+        return out
+
+    def pfb(self, fs, nchan, os_, K, atten, max_nel=1):
+        return RefPfb(self, fs, nchan, os_, K, atten, max_nel)
+
+    def cov_accumulate(self, out, nel, nchan, gate, R, counts):
+        o = np.ascontiguousarray(out, np.complex64)
+        g = None if gate is None else np.ascontiguousarray(gate, np.uint8).ctypes.data_as(C.POINTER(C.c_uint8))
+        self.L.atkdsp_pfb_cov_accumulate(self._cf(o), nel, nchan, o.shape[2], g, self._cf(R),
+                                         counts.ctypes.data_as(C.POINTER(C.c_uint32)))
+
+    def channel_power(self, out, nel, nchan):
+        o = np.ascontiguousarray(out, np.complex64)
+        p = np.zeros(nel * nchan, np.float32)
+        mp = np.zeros(nchan, np.float32)
+        self.L.atkdsp_pfb_channel_power(self._cf(o), nel, nchan, o.shape[2], self._f(p), self._f(mp))
+        return p.reshape(nel, nchan), mp
+
+
+class RefPfb:
+    def __init__(self, ref: Ref, fs, nchan, os_, K, atten, max_nel):
+        self.r, self.M = ref, nchan
+        self.p = ref.L.atkdsp_pfb_create(fs, nchan, os_, K, atten, max_nel)
+
+    def set_cal(self, table, nel):
+        t = np.ascontiguousarray(table, np.complex64)
+        self.r.L.atkdsp_pfb_set_cal(self.p, Ref._cf(t), nel)
+
+    def process_n(self, ins, nel):
+        arrs = [np.ascontiguousarray(x, np.complex64) for x in ins]
+        n = arrs[0].size
+        m = self.r.L.atkdsp_pfb_out_hops(self.p, n)
+        out = np.zeros(nel * self.M * m, np.complex64)
+        ptrs = (C.POINTER(Cf) * nel)(*[Ref._cf(a) for a in arrs])
+        got = self.r.L.atkdsp_pfb_process_n(self.p, ptrs, nel, n, Ref._cf(out), out.size)
+        return out[: nel * self.M * got].reshape(nel, self.M, got)
+
+    def process(self, x):
+        return self.process_n([x], 1)[0]
+
+    def __del__(self):
+        if getattr(self, "p", None):
+            self.r.L.atkdsp_pfb_destroy(self.p)
+            self.p = None

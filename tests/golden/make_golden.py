@@ -1,0 +1,199 @@
+"""Write the golden vectors in tests/golden/*.npz from the reference library
+compiled from its own sources (oracle/build_ref.sh -> oracle/_ref/atkdsp_ref.so).
+
+Inputs are seeded and small; outputs are what the reference C produced. The
+fixtures pin the numpy oracle (tests/test_oracle.py, CPU) and the HIP path
+(tests/test_gpu_*.py). Run in a container that has /root/reference:
+
+    sh oracle/build_ref.sh && python tests/golden/make_golden.py
+
+Reference build flags are recorded in each file (build.sh's -std=c99: no FMA
+contraction, so the FP32 fold is multiply-then-add).
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+from oracle.ref import Ref  # noqa: E402
+from atkdsp_amd import synth  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+FLAGS = "gcc -std=c99 -O3 -fno-math-errno -mavx2 -mfma -fopenmp (oracle/build_ref.sh)"
+
+
+def noise(rng, n, scale=1.0):
+    return ((rng.standard_normal(n) + 1j * rng.standard_normal(n)) * scale).astype(np.complex64)
+
+
+def save(name, **arrs):
+    np.savez_compressed(OUT / f"{name}.npz", flags=np.array(FLAGS), **arrs)
+    print(name, {k: getattr(v, "shape", None) for k, v in arrs.items()})
+
+
+def main():
+    r = Ref()
+    r.set_threads(1)
+    rng = np.random.default_rng(20261203)
+
+    # -- unpack / unpack_dc, every format, torn tail, with and without an offset
+    d = {}
+    for fmt in range(5):
+        if fmt == 0:
+            raw = rng.integers(0, 256, 4000, dtype=np.uint8).tobytes()
+        elif fmt == 1:
+            raw = rng.integers(-128, 128, 4000, dtype=np.int8).tobytes()
+        elif fmt == 4:
+            raw = rng.standard_normal(1000).astype("<f4").tobytes()
+        else:
+            raw = rng.integers(-2048, 2048, 2000, dtype="<i2").tobytes()
+        raw = raw[:-1]
+        for j, off in enumerate((0j, 0.0123 - 0.0456j)):
+            out, mean = r.unpack_dc(raw, fmt, off)
+            d[f"raw{fmt}"] = np.frombuffer(raw, np.uint8)
+            d[f"out{fmt}_{j}"] = out
+            d[f"mean{fmt}_{j}"] = np.array([mean.real, mean.imag])
+    d["offsets"] = np.array([0j, 0.0123 - 0.0456j])
+    save("unpack", **d)
+
+    # -- DC pole continuous across blocks (tests/test_cross_check.py:107-116 shape)
+    raw = (rng.integers(-2048, 2048, 20000, dtype="<i2") + 300).astype("<i2").tobytes()
+    st = np.zeros(1, np.complex64)
+    outs, states = [], []
+    for chunk in (raw[:4000], raw[4000:4004], raw[4004:]):
+        outs.append(r.unpack(chunk, 3, 0.01, st))
+        states.append(st[0])
+    save("pole", raw=np.frombuffer(raw, np.uint8), alpha=np.float32(0.01), out=np.concatenate(outs),
+         states=np.array(states), splits=np.array([4000, 4004]))
+
+    # -- FFT
+    d = {}
+    for n in (16, 1000, 1024, 4096):
+        x = noise(rng, n)
+        d[f"x{n}"] = x
+        d[f"fwd{n}"] = r.fft(x)
+        d[f"inv{n}"] = r.fft(x, inverse=True)
+    save("fft", **d)
+
+    # -- windows, window stats, power_db
+    d = {}
+    for k in range(4):
+        w = r.window(k, 2048)
+        d[f"win{k}"] = w
+        d[f"wstats{k}"] = np.array(r.window_stats(w))
+    x = noise(rng, 4096) + 0.5 * np.exp(2j * np.pi * 300 / 4096 * np.arange(4096)).astype(np.complex64)
+    w = r.window(1, 4096)
+    d["pdb_x"] = x.astype(np.complex64)
+    d["pdb_hann"] = r.power_db(x.astype(np.complex64), w)
+    d["pdb_rect"] = r.power_db(x.astype(np.complex64), None)
+    save("window", **d)
+
+    # -- spectrum reduce / stats
+    x = noise(rng, 40 * 1024 + 37)
+    x[5000:5300] += (3.0 * np.exp(2j * np.pi * 0.2 * np.arange(300))).astype(np.complex64)
+    w = r.window(1, 1024)
+    d = {"x": x, "w": w}
+    for det in range(3):
+        for hop in (1024, 512):
+            line, fr = r.spectrum_reduce(x, 1024, hop, w, det)
+            d[f"red{det}_{hop}"] = line
+            d[f"fr{det}_{hop}"] = np.int64(fr)
+    st = r.spectrum_stats(x, 1024, 512, w)
+    for k in ("max", "avg", "min", "sk"):
+        d[f"stats_{k}"] = st[k]
+    d["stats_frames"] = np.int64(st["frames"])
+    z = synth.wideband_iq(7, 60000, 40e6).astype(np.complex64)
+    w4 = r.window(1, 4096)
+    line, fr = r.spectrum_reduce(z, 4096, 2048, w4, 0)
+    d.update(c2x=z, c2w=w4, c2red=line, c2fr=np.int64(fr))
+    st = r.spectrum_stats(z, 4096, 2048, w4)
+    for k in ("max", "avg", "min", "sk"):
+        d[f"c2stats_{k}"] = st[k]
+    save("spectrum", **d)
+
+    # -- display helpers and stitch (exact)
+    d = {}
+    line = (rng.standard_normal(4096) * 10 - 60).astype(np.float32)
+    line[[0, 7, 100, 101, 4095]] = np.nan
+    line[5] = np.nan
+    line[2000:2004] = [0.0, -0.0, 0.0, -0.0]
+    d["line"] = line
+    for m in (1024, 1000, 4096, 5000, 3, 1):
+        d[f"dec{m}"] = r.decimate_max(line, m)
+    lut = rng.integers(0, 2 ** 32, 256, dtype=np.uint64).astype(np.uint32)
+    vals = np.concatenate([line, np.array([-np.inf, np.inf, -120.0, 0.0, -60.0, -59.99], np.float32)])
+    d["lut"] = lut
+    d["pixvals"] = vals
+    d["pix"] = r.db_to_pixels(vals, -120.0, 0.0, lut)
+    # a zero span maps every finite value to entry 0; +-inf there is undefined
+    # behaviour in the reference ((int) of NaN), so the case feeds finite values
+    d["pix_flat"] = r.db_to_pixels(line, -50.0, -50.0, lut)
+    seg = (rng.standard_normal(1024) * 5 - 70).astype(np.float32)
+    seg[[3, 500]] = np.nan
+    trace = np.full(8192, np.nan, np.float32)
+    trace[100:200] = -80.0
+    d["seg"], d["trace0"] = seg, trace.copy()
+    cases = [(100e6, 40e6 / 1024, 100.5e6, 139e6, 90e6, 10e3),
+             (120e6, 40e6 / 1024, 0.0, 1e12, 90e6, 39062.5),
+             (95e6, 40e6 / 1024, 96e6, 97e6, 90e6, 2e3),
+             (80e6, 40e6 / 1024, 0.0, 1e12, 90e6, 10e3)]
+    t = trace.copy()
+    for j, c in enumerate(cases):
+        t = r.stitch_max(seg, *c, t)
+        d[f"stitch{j}"] = t.copy()
+    d["stitch_cases"] = np.array(cases)
+    save("display", **d)
+
+    # -- PFB
+    d = {}
+    c1 = synth.c1_bytes()
+    x1, _ = r.unpack_dc(c1, 0, 0j)
+    p = r.pfb(2.4e6, 64, 1, 8, 81.2953, 1)
+    d["c1_x"], d["c1_y"] = x1, p.process(x1)
+    fs = 2.4e6
+    k = np.arange(192 * 80)
+    xt = (np.exp(2j * np.pi * (40 * fs / 192) / fs * k) + 0.02 * (rng.standard_normal(k.size)
+                                                                    + 1j * rng.standard_normal(k.size)))
+    xt = xt.astype(np.complex64)
+    d["t192_x"] = xt
+    d["t192_y"] = r.pfb(fs, 192, 2, 12, 80.0, 1).process(xt)
+    for os_ in (3, 4):
+        xs = noise(rng, 24 * 300 + 17)
+        d[f"t24_{os_}_x"] = xs
+        d[f"t24_{os_}_y"] = r.pfb(fs, 24, os_, 12, 80.0, 1).process(xs)
+    for M, K, nmul in ((1026, 12, 40), (1024, 16, 48), (2048, 12, 24)):
+        xs = noise(rng, M * nmul)
+        d[f"m{M}_x"] = xs
+        d[f"m{M}_y"] = r.pfb(fs, M, 2, K, 80.0, 1).process(xs)
+    xs = [noise(rng, 60 * 300 + 17) for _ in range(2)]
+    d["t60_x"] = np.stack(xs)
+    d["t60_y"] = r.pfb(fs, 60, 3, 12, 80.0, 2).process_n(xs, 2)
+    # five elements of a plane wave with a cal table on elements 0..3
+    base = xt[: 192 * 60]
+    phi = np.deg2rad(37.0)
+    ins = [(base * np.exp(1j * e * phi)).astype(np.complex64) for e in range(5)]
+    cal = np.exp(1j * rng.uniform(0, 2 * np.pi, (4, 192))).astype(np.complex64)
+    pc = r.pfb(fs, 192, 2, 12, 80.0, 5)
+    pc.set_cal(cal, 4)
+    y5 = pc.process_n(ins, 5)
+    d["el5_x"], d["el5_cal"], d["el5_y"] = np.stack(ins), cal, y5
+    # W1 / W2 on that output, gated and not
+    R = np.zeros((192, 5, 5), np.complex64)
+    cnt = np.zeros(192, np.uint32)
+    r.cov_accumulate(y5, 5, 192, None, R, cnt)
+    d["w1_R"], d["w1_cnt"] = R.copy(), cnt.copy()
+    gate = np.zeros(192, np.uint8)
+    gate[[3, 40, 41, 191]] = 1
+    r.cov_accumulate(y5, 5, 192, gate, R, cnt)
+    d["w1_gate"], d["w1_R2"], d["w1_cnt2"] = gate, R, cnt
+    pw, mp = r.channel_power(y5, 5, 192)
+    d["w2_p"], d["w2_m"] = pw, mp
+    save("pfb", **d)
+
+
+if __name__ == "__main__":
+    main()

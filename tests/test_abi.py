@@ -1,0 +1,106 @@
+"""The drop-in boundary, CPU only: the library loads, reports ABI 26, exports
+every name of the reference ABI and every function include/atkdsp.h declares,
+and its stubs return the documented failure values. No compute calls."""
+import ctypes as C
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT, has_gpu
+
+LIB = ROOT / "atkdsp_amd" / "atkdsp.so"
+
+
+def exported():
+    r = subprocess.run(["nm", "-D", "--defined-only", str(LIB)], capture_output=True, text=True, check=True)
+    return {ln.split()[-1] for ln in r.stdout.splitlines() if " T " in ln}
+
+
+def header_functions(path):
+    src = (ROOT / "include" / path).read_text()
+    return set(re.findall(r"ATK\w*API\s+[^;(]*?\b(atk\w+)\s*\(", src))
+
+
+def test_library_is_built():
+    assert LIB.exists(), "run python -m atkdsp_amd.build"
+
+
+def test_exports_every_reference_abi_name():
+    from atkdsp_amd.build import abi_symbols
+    names = {n for n, _ in abi_symbols()}
+    assert len(names) == 334
+    missing = names - exported()
+    assert not missing, sorted(missing)[:10]
+
+
+def test_exports_every_declared_function():
+    ex = exported()
+    for h in ("atkdsp.h", "atkgpu.h"):
+        if not (ROOT / "include" / h).exists():
+            continue
+        decl = header_functions(h)
+        assert decl, h
+        assert not (decl - ex), (h, sorted(decl - ex))
+
+
+def test_identity_without_a_device():
+    import atkdsp_amd as A
+    lib = A.load()
+    assert lib.atkdsp_abi_version() == 26
+    assert A.version() == "0.25.1"
+    assert "abi 26" in A.build_info() and "gfx950" in A.build_info()
+    assert A.bytes_per_sample("cu8") == 2 and A.bytes_per_sample("ci16q11") == 4
+    assert A.bytes_per_sample("cf32") == 8 and A.bytes_per_sample(9) == 0
+    assert A.set_threads(3) == 3 and A.get_threads() == 3
+    assert A.set_threads(0) >= 1
+
+
+def test_stubs_return_failure_values():
+    lib = C.CDLL(str(LIB))
+    f = lib.atkdsp_fir_create
+    f.restype = C.c_void_p
+    assert f() is None                                  # a *_create stub: NULL
+    g = lib.atkdsp_nco_mix
+    g.restype = None
+    g()                                                 # void: nothing
+    h = lib.atkdsp_median
+    h.restype = C.c_float
+    assert h() != h()                                   # float: NaN
+    k = lib.atkdsp_cfar_process
+    k.restype = C.c_ssize_t
+    assert k() == -1                                    # signed count: ATKDSP_E_ARG
+
+
+def test_window_and_window_stats_are_host_design_helpers():
+    import numpy as np
+    import atkdsp_amd as A
+    from oracle import oracle as O
+    for kind in ("rectangular", "hann", "hamming", "blackmanharris"):
+        k = A.WIN[kind]
+        assert np.array_equal(A.window(kind, 513), O.window(k, 513))
+    with pytest.raises(A.AtkDspError):
+        A.window(7, 16)
+    cg, enbw = A.window_stats(A.window("hann", 4096))
+    assert abs(cg - 0.5) < 1e-3 and abs(enbw - 1.5) < 1e-3
+
+
+@pytest.mark.skipif(has_gpu(), reason="checks the behaviour without a device")
+def test_no_device_fails_loudly():
+    import numpy as np
+    import atkdsp_amd as A
+    with pytest.raises(A.AtkDspError):
+        A.Fft(1024)
+    with pytest.raises(A.AtkDspError):
+        A.Pfb(2.4e6, 64)
+    with pytest.raises(A.AtkDspError):
+        A.unpack(b"\x00\x01\x02\x03", "cu8")
+
+
+def test_reference_binding_would_load_it():
+    """The reference binding needs ABI 26 and every name it binds; both are
+    checked above. Here: the binding's own identity sentence (build_info
+    contains "abi 26", reference tests/test_cross_check.py:58-62)."""
+    lib = C.CDLL(str(LIB))
+    lib.atkdsp_build_info.restype = C.c_char_p
+    assert b"abi 26" in lib.atkdsp_build_info()

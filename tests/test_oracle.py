@@ -1,0 +1,280 @@
+"""Pins the numpy oracle (oracle/oracle.py), CPU only.
+
+Every oracle function is checked against at least one of: the golden vectors
+the compiled reference wrote (tests/golden/), the reference's own known-answer
+tests (reference tests/test_smoke.c and tests/test_cross_check.py lines cited
+per test), or a closed form chosen so that a plausible slip (a wrong sign, a
+dropped term, a transposed index, an off-by-one hop) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from oracle import oracle as O
+
+
+# ---- unpack -----------------------------------------------------------------
+@pytest.mark.parametrize("fmt", range(5))
+def test_unpack_is_bit_exact_with_the_reference(fmt):
+    g = golden("unpack")
+    raw = g[f"raw{fmt}"].tobytes()
+    for j, off in enumerate(g["offsets"]):
+        out, mean = O.unpack_dc(raw, fmt, complex(off))
+        ref = g[f"out{fmt}_{j}"]
+        assert out.size == ref.size == len(raw) // O.BPS[fmt]
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+        # the mean restates the reference's eight FP64 lanes: equal to the bit
+        assert mean.real == g[f"mean{fmt}_{j}"][0] and mean.imag == g[f"mean{fmt}_{j}"][1]
+
+
+def test_unpack_known_answers():
+    # reference tests/test_smoke.c:26-34: q11 2048 -> 1.0, -2048 -> -1.0, torn tail ignored
+    raw = np.array([2048, -2048, 0, 1024], "<i2").tobytes() + b"\x01"
+    out, _ = O.unpack_dc(raw, 3)
+    assert out.tolist() == [1 - 1j, 0.5j]
+    # cu8 255 -> (255-127.5)/127.5 = 1; 0 -> -1 (closed form)
+    out, _ = O.unpack_dc(bytes([255, 0]), 0)
+    assert out[0] == 1 - 1j
+    # the offset is removed and the mean is BEFORE the offset
+    # (reference tests/test_cross_check.py:566-591)
+    raw = np.array([100, -50] * 64, "<i2").tobytes()
+    out, mean = O.unpack_dc(raw, 3, 0.01 + 0.02j)
+    # the mean is formed from the float32 outputs plus the offset, so it holds
+    # to float32 resolution of the samples (reference gate: 1e-6)
+    assert abs(mean - (100 / 2048 - 50j / 2048)) < 1e-8
+    assert np.allclose(out, (100 / 2048 - 0.01) + 1j * (-50 / 2048 - 0.02), atol=1e-7)
+
+
+def test_dc_pole_matches_the_reference_across_blocks():
+    g = golden("pole")
+    raw = g["raw"].tobytes()
+    a, b = g["splits"]
+    st = 0j
+    outs = []
+    for chunk in (raw[:a], raw[a:b], raw[b:]):
+        y, st = O.unpack(chunk, 3, float(g["alpha"]), st)
+        outs.append(y)
+    y = np.concatenate(outs)
+    # the reference runs the pole in FP32, the oracle in FP64: tolerance of
+    # reference tests/test_cross_check.py:115
+    assert np.max(np.abs(y - g["out"])) < 2e-6
+    assert abs(st - g["states"][-1]) < 2e-6
+    assert abs(y[-500:].mean()) < 0.01          # the DC is gone
+
+
+# ---- windows / FFT ----------------------------------------------------------------
+@pytest.mark.parametrize("kind", range(4))
+def test_window_and_stats_match_the_reference(kind):
+    g = golden("window")
+    w = O.window(kind, 2048)
+    assert np.max(np.abs(w - g[f"win{kind}"])) <= 6e-8
+    cg, enbw = O.window_stats(g[f"win{kind}"])
+    assert np.allclose([cg, enbw], g[f"wstats{kind}"], rtol=1e-12)
+
+
+def test_window_stats_textbook_values():
+    # reference tests/test_cross_check.py:1492-1506: rect 1/1, Hann 0.5/1.5,
+    # 4-term Blackman-Harris 0.35875/2.0044 (Harris 1978, table 1)
+    n = 1 << 16
+    for kind, cg, enbw, tol in ((0, 1.0, 1.0, 1e-12), (1, 0.5, 1.5, 1e-3), (3, 0.35875, 2.0044, 1e-3)):
+        a, b = O.window_stats(O.window(kind, n))
+        assert abs(a - cg) < tol and abs(b - enbw) < tol
+
+
+@pytest.mark.parametrize("n", [16, 1000, 1024, 4096])
+def test_fft_matches_the_reference_and_a_brute_force_dft(n):
+    g = golden("fft")
+    x = g[f"x{n}"]
+    X = O.fft(x)
+    assert O.rel_err(X, g[f"fwd{n}"]) < 1e-7
+    assert O.rel_err(O.fft(x, True), g[f"inv{n}"]) < 1e-7
+    if n <= 1024:
+        j = np.arange(n)
+        D = np.exp(-2j * np.pi * np.outer(j, j) / n) @ x.astype(np.complex128)
+        assert O.rel_err(X, D) < 1e-12
+
+
+def test_power_db_matches_the_reference():
+    g = golden("window")
+    assert np.max(np.abs(O.power_db(g["pdb_x"], g["win1"][:0] if False else O.window(1, 4096)) - g["pdb_hann"])) < 1e-3
+    assert np.max(np.abs(O.power_db(g["pdb_x"], None) - g["pdb_rect"])) < 1e-3
+
+
+def test_power_db_tone_known_answer():
+    # reference tests/test_smoke.c:102-121: a tone at bin +100 lands in shifted bin n/2+100
+    n = 1024
+    x = np.exp(2j * np.pi * 100 / n * np.arange(n)).astype(np.complex64)
+    line = O.power_db(x, None)
+    assert int(np.argmax(line)) == n // 2 + 100
+    assert abs(line[n // 2 + 100]) < 1e-4          # |X|/n = 1 -> 0 dB
+
+
+@pytest.mark.parametrize("det", range(3))
+@pytest.mark.parametrize("hop", [1024, 512])
+def test_spectrum_reduce_matches_the_reference(det, hop):
+    g = golden("spectrum")
+    line, fr = O.spectrum_reduce(g["x"], 1024, hop, g["w"], det)
+    assert fr == int(g[f"fr{det}_{hop}"])
+    assert np.max(np.abs(line - g[f"red{det}_{hop}"])) < 1e-3
+
+
+def test_spectrum_stats_match_the_reference():
+    g = golden("spectrum")
+    st = O.spectrum_stats(g["x"], 1024, 512, g["w"])
+    assert st["frames"] == int(g["stats_frames"])
+    for k in ("max", "avg", "min"):
+        assert np.max(np.abs(st[k] - g[f"stats_{k}"])) < 1e-3
+    assert np.allclose(st["sk"], g["stats_sk"], rtol=1e-4)
+    st = O.spectrum_stats(g["c2x"], 4096, 2048, g["c2w"])
+    for k in ("max", "avg", "min"):
+        assert np.max(np.abs(st[k] - g[f"c2stats_{k}"])) < 1e-3
+    line, fr = O.spectrum_reduce(g["c2x"], 4096, 2048, g["c2w"], 0)
+    assert fr == int(g["c2fr"]) and np.max(np.abs(line - g["c2red"])) < 1e-3
+
+
+def test_spectrum_reduce_of_identical_frames_is_power_db():
+    # reference tests/test_smoke.c:110-121
+    n = 256
+    fr = (np.exp(2j * np.pi * 17 / n * np.arange(n)) + 0.1).astype(np.complex64)
+    x = np.tile(fr, 6)
+    line, frames = O.spectrum_reduce(x, n, n, None, 0)
+    assert frames == 6
+    assert np.max(np.abs(line - O.power_db(fr, None))) < 1e-3
+
+
+def test_spectral_kurtosis_closed_forms():
+    # reference tests/test_cross_check.py:1453-1466: ~2 for noise, ~1 for a steady tone
+    rng = np.random.default_rng(5)
+    n = 512
+    x = (rng.standard_normal(n * 400) + 1j * rng.standard_normal(n * 400)).astype(np.complex64)
+    sk = O.spectrum_stats(x, n, n, None)["sk"]
+    assert abs(np.median(sk) - 2.0) < 0.1
+    t = np.exp(2j * np.pi * 64 / n * np.arange(n * 50)).astype(np.complex64)
+    sk = O.spectrum_stats(t, n, n, None)["sk"]
+    assert sk[n // 2 + 64] < 1.01
+
+
+# ---- display ----------------------------------------------------------------------
+def test_display_helpers_are_bit_exact_with_the_reference():
+    g = golden("display")
+    for m in (1024, 1000, 4096, 5000, 3, 1):
+        a = O.decimate_max(g["line"], m)
+        assert np.array_equal(a.view(np.uint32), g[f"dec{m}"].view(np.uint32)), m
+    assert np.array_equal(O.db_to_pixels(g["pixvals"], -120.0, 0.0, g["lut"]), g["pix"])
+    assert np.array_equal(O.db_to_pixels(g["line"], -50.0, -50.0, g["lut"]), g["pix_flat"])
+    t = g["trace0"].copy()
+    for j, c in enumerate(g["stitch_cases"]):
+        O.stitch_max(g["seg"], *c, t)
+        assert np.array_equal(t.view(np.uint32), g[f"stitch{j}"].view(np.uint32)), j
+
+
+def test_display_known_answers():
+    # reference tests/test_smoke.c:141-153 style placements, NaN first sticks
+    x = np.array([np.nan, 1, 2, 3, 4, np.nan, 0, 9], np.float32)
+    assert np.isnan(O.decimate_max(x, 2)[0]) and O.decimate_max(x, 2)[1] == 9
+    assert O.decimate_max(np.arange(10, dtype=np.float32), 3).tolist() == [2, 5, 9]
+    out = np.full(4, np.nan, np.float32)
+    O.stitch_max(np.array([1, 2, 3, 4], np.float32), 0.0, 1.0, 0.0, 10.0, 0.0, 2.0, out)
+    assert out[:2].tolist() == [2, 4] and np.isnan(out[2:]).all()
+
+
+# ---- PFB ----------------------------------------------------------------------------
+def test_prototype_closed_form_properties():
+    h = O.pfb_prototype(64, 8, 81.2953)
+    assert h.size == 512
+    assert abs(h.sum() - 64) < 1e-9                      # unit DC gain x nchan
+    assert np.allclose(h, h[::-1], atol=1e-15)           # linear phase
+    assert abs(O.kaiser_beta(81.2953) - 8.0) < 1e-4      # beta = 8 at 81.2953 dB
+
+
+def _pfb_rel(M, os_, K, atten, x, y_ref, nel=1, cal=None):
+    p = O.Pfb(2.4e6, M, os_, K, atten, nel)
+    if cal is not None:
+        p.set_cal(cal, cal.shape[0])
+    y = p.process_n(x if nel > 1 else [x], nel)
+    y = y if nel > 1 else y[0]
+    m = min(y.shape[-1], y_ref.shape[-1])
+    return O.rel_err(y[..., :m], y_ref[..., :m])
+
+
+def test_pfb_fp64_oracle_matches_the_reference():
+    # the reference differs from its FP64 twin by <= 8.8e-8 of full scale
+    # (reference CHANGELOG.md:155-157) and its tests hold it to 5e-7
+    # (tests/test_channelizer.py:397-398): the oracle must sit inside that
+    g = golden("pfb")
+    assert _pfb_rel(64, 1, 8, 81.2953, g["c1_x"], g["c1_y"]) < 5e-7
+    assert _pfb_rel(192, 2, 12, 80.0, g["t192_x"], g["t192_y"]) < 5e-7
+    for os_ in (3, 4):
+        assert _pfb_rel(24, os_, 12, 80.0, g[f"t24_{os_}_x"], g[f"t24_{os_}_y"]) < 5e-7
+    assert _pfb_rel(1026, 2, 12, 80.0, g["m1026_x"], g["m1026_y"]) < 5e-7
+    assert _pfb_rel(1024, 2, 16, 80.0, g["m1024_x"], g["m1024_y"]) < 5e-7
+    assert _pfb_rel(2048, 2, 12, 80.0, g["m2048_x"], g["m2048_y"]) < 5e-7
+    assert _pfb_rel(60, 3, 12, 80.0, list(g["t60_x"]), g["t60_y"], nel=2) < 5e-7
+    assert _pfb_rel(192, 2, 12, 80.0, list(g["el5_x"]), g["el5_y"], nel=5, cal=g["el5_cal"]) < 5e-7
+
+
+def test_pfb_oracle_chunk_invariance_and_bookkeeping():
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal(192 * 50 + 7) + 1j * rng.standard_normal(192 * 50 + 7)).astype(np.complex64)
+    y1 = O.Pfb(2.4e6, 192, 2, 12).process(x)
+    p = O.Pfb(2.4e6, 192, 2, 12)
+    parts = [p.process(x[a:b]) for a, b in ((0, 5), (5, 1000), (1000, 1001), (1001, x.size))]
+    y2 = np.concatenate(parts, axis=1)
+    assert y1.shape == y2.shape and np.allclose(y1, y2, atol=1e-12, rtol=0)
+
+
+def test_pfb_tone_lands_in_its_channel_as_true_baseband():
+    # reference tests/test_smoke.c:674-700 (tone at channel 10 of 64) and
+    # tests/test_channelizer.py:299-319 (true baseband at 2x/3x/4x)
+    fs, M = 2.4e6, 24
+    for os_ in (2, 3, 4):
+        for b, df in ((1, 3e3), (5, -4e3)):
+            n = M * 2500
+            x = np.exp(2j * np.pi * (b * fs / M + df) / fs * np.arange(n)).astype(np.complex64)
+            p = O.Pfb(fs, M, os_, 12)
+            y = p.process(x)[b]
+            rate = p.out_rate()
+            k = 1 << int(np.floor(np.log2(len(y))))
+            Y = np.abs(np.fft.fft(y[:k] * np.blackman(k)))
+            f = np.fft.fftfreq(k, 1 / rate)[np.argmax(Y)]
+            assert abs(f - df) <= rate / k
+    x = np.exp(2j * np.pi * 10 * (2.4e6 / 64) / 2.4e6 * np.arange(64 * 200)).astype(np.complex64)
+    y = O.Pfb(2.4e6, 64, 2, 12).process(x)[:, 40:]
+    assert int(np.argmax((np.abs(y) ** 2).mean(1))) == 10
+
+
+def test_w1_w2_oracle_matches_the_reference():
+    g = golden("pfb")
+    y = g["el5_y"]
+    R = np.zeros((192, 5, 5), np.complex128)
+    c = np.zeros(192, np.uint32)
+    O.pfb_cov_accumulate(y, 5, 192, None, R, c)
+    assert O.rel_err(R, g["w1_R"]) < 1e-5 and np.array_equal(c, g["w1_cnt"])
+    O.pfb_cov_accumulate(y, 5, 192, g["w1_gate"], R, c)
+    assert O.rel_err(R, g["w1_R2"]) < 1e-5 and np.array_equal(c, g["w1_cnt2"])
+    # Hermitian by construction
+    assert np.allclose(R, np.conj(np.transpose(R, (0, 2, 1))))
+    p, mp = O.pfb_channel_power(y, 5, 192)
+    assert O.rel_err(p, g["w2_p"]) < 1e-6 and O.rel_err(mp, g["w2_m"]) < 1e-6
+
+
+def test_oracle_detects_planted_mistakes():
+    """A wrong rotation sign, a dropped tap or an off-by-one hop in the oracle
+    would fail the golden comparison: plant each on the CPU and check."""
+    g = golden("pfb")
+    x, yref = g["t192_x"], g["t192_y"]
+    good = O.Pfb(2.4e6, 192, 2, 12).process(x)
+    m = min(good.shape[1], yref.shape[1])
+    # conjugated rotation
+    bad = good.copy()
+    bad[1::2, 1::2] *= -1
+    assert O.rel_err(bad[:, :m], yref[:, :m]) > 1e-2
+    # hop off by one
+    assert O.rel_err(good[:, 1:m], yref[:, :m - 1]) > 1e-2
+    # dropped last tap
+    p = O.Pfb(2.4e6, 192, 2, 12)
+    p.h = p.h.copy()
+    p.h[-192:] = 0
+    assert O.rel_err(p.process(x)[:, :m], yref[:, :m]) > 1e-6
