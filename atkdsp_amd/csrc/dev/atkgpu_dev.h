@@ -97,6 +97,38 @@ int atkgpu_pfb_fold(void *s, const void *sbuf, size_t spitch, unsigned nel, size
 int atkgpu_pfb_rotate_store(void *s, const void *X, unsigned nel, size_t m, unsigned M,
                             unsigned hop, long long b0, const void *twM, const void *cal,
                             unsigned cal_nel, void *out);
+/* The fused kernel (fold, FFT, rotation, cal and the channel-major store in
+ * one launch) for M in 64..2048 (power of two), os 1 or 2, K*os <= 32.
+ * atkgpu_pfb_fused_kos returns the padded tap-row count (8/16/24/32) the
+ * taps buffer must hold (KOS*hop floats, zero beyond K*os), or 0 when the
+ * shape needs the general path. Element e streams from sbuf + e*spitch with
+ * nvalid valid samples; hop i of element e, channel k goes to
+ * out[(e*M + k)*opitch + ocol + i]. */
+int atkgpu_pfb_fused_kos(unsigned M, unsigned os, unsigned K);
+int atkgpu_pfb_fused(void *s, const void *sbuf, size_t spitch, size_t nvalid, unsigned nel,
+                     size_t m, long long b0, unsigned M, unsigned os, unsigned kos,
+                     const float *taps, const void *tw, const void *cal, unsigned cal_nel,
+                     void *out, size_t opitch, size_t ocol);
+/* ---- chain kernels (chain.hip): S stream rows spitch samples apart ---- */
+/* raw (stream s at raw + s*raw_pitch bytes) -> cf32 at row0 + s*spitch, with the
+ * per-stream offset off[2s..2s+1] (NULL: none); mean[2s..] = block mean before
+ * the offset; update != 0 makes it the next block's offset. partials: S*parts*2. */
+int atkgpu_chain_unpack(void *s, const void *raw, size_t raw_pitch, int fmt, size_t n, void *row0,
+                        size_t spitch, float *off, double *partials, int parts, double *mean,
+                        int update, unsigned S);
+/* every frame f < frames at row0 + s*spitch + f*fhop (N in 1024/2048/4096):
+ * window, FFT, |X|^2/N^2, detector -> partial[S][nfb][N] -> line_db[S][N] */
+int atkgpu_chain_display(void *s, const void *row0, size_t spitch, unsigned S, size_t frames,
+                         size_t fhop, unsigned N, int nfb, const float *win, const void *tw, int det,
+                         float *partial, float *line_db);
+/* decimate line_db[S][N] to dec[S][cols]; stitch into trace[S][nout] (NULL: none)
+ * with stream s's segment starting at seg_lo0 + s*seg_lo_step */
+int atkgpu_chain_post(void *s, const float *line_db, unsigned S, unsigned N, unsigned cols, float *dec,
+                      float *trace, unsigned nout, double seg_lo0, double seg_lo_step, double seg_step,
+                      double out_lo, double out_step);
+/* copy keep samples from src + s*spitch to dst + s*spitch for every stream */
+int atkgpu_chain_keep(void *s, const void *src, size_t spitch, void *dst, size_t keep, unsigned S);
+
 /* W1 / W2 over a device [nel][nchan][m] block */
 int atkgpu_pfb_cov(void *s, const void *y, unsigned nel, unsigned nchan, size_t m,
                    const uint8_t *gate, void *R, uint32_t *counts);

@@ -1,0 +1,388 @@
+// Kernels of the device-resident front-end chain: many independent streams
+// of one shape, each launch covering every stream.
+//
+// Each stream keeps its samples in a row of a slab buffer:
+//   [kept prefix (keep samples) | this block (n samples)]
+// The prefix is the last `keep` samples of the stream so far, so the filterbank
+// finds its history (ov + lc samples) and the display its unfinished frame
+// tail (< nfft samples) just before the block. After a block the last keep
+// samples move to the other slab buffer.
+//
+//   k_chain_unpack   raw -> cf32 into the row (bit-exact unpack_dc, reference
+//                    src/convert.c:33-111), DC offset from the stream's
+//                    previous-block mean (device state), FP64 partial sums
+//   k_chain_mean     per stream: FP64 mean -> next block's offset
+//   k_disp_reduce    frames: window, team FFT, |X|^2/n^2, per-bin detector in
+//                    registers (reference src/fft.c:177-191), partial lines
+//   k_disp_final     partial lines in fixed order -> dB line (src/fft.c:286-292)
+//   k_disp_post      decimate_max (src/display.c:19-37) and the NaN-aware
+//                    stitch of the decimated line (src/detect.c:116-134)
+//   k_chain_keep     last keep samples -> the other slab buffer
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "atkgpu_dev.h"
+#include "dev_common.hpp"
+#include "fft_core.hpp"
+#include "fft_team.hpp"
+
+// ---- unpack with a per-stream offset ---------------------------------------------
+struct ChainUnpackArgs {
+    const unsigned char *raw;   // stream s at raw + s*raw_pitch
+    long long raw_pitch;
+    int fmt;
+    long long n;
+    cpx *row0;                  // stream s block start: row0 + s*spitch
+    long long spitch;
+    const float *off;           // per stream offset (re, im); null = 0
+    double *partials;           // [S][parts][2]
+    int parts;
+};
+
+template <int FMT>
+__device__ __forceinline__ cpx chain_conv(const unsigned char *raw, long long i, float cre, float cim, float k,
+                                          float ore, float oim) {
+#pragma clang fp contract(off)
+    cpx o;
+    if constexpr (FMT == FMT_CU8) {
+        const uchar2 b = ((const uchar2 *)raw)[i];
+        o.x = ((float)b.x - cre) * k;
+        o.y = ((float)b.y - cim) * k;
+    } else if constexpr (FMT == FMT_CI8) {
+        const char2 b = ((const char2 *)raw)[i];
+        o.x = ((float)(signed char)b.x - cre) * k;
+        o.y = ((float)(signed char)b.y - cim) * k;
+    } else if constexpr (FMT == FMT_CI16) {
+        const short2 b = ((const short2 *)raw)[i];
+        o.x = ((float)b.x - cre) * k;
+        o.y = ((float)b.y - cim) * k;
+    } else {
+        const float2 b = ((const float2 *)raw)[i];
+        o.x = b.x - ore;
+        o.y = b.y - oim;
+    }
+    return o;
+}
+
+template <int FMT>
+__global__ void __launch_bounds__(256) k_chain_unpack(ChainUnpackArgs a, float c0, float kk) {
+#pragma clang fp contract(off)
+    const int s = blockIdx.y;
+    const float ore = a.off ? a.off[2 * s] : 0.0f, oim = a.off ? a.off[2 * s + 1] : 0.0f;
+    // the folded constants exactly as src/convert.c:47, :57, :68 (IEEE division)
+    const float cre = c0 + ore / kk, cim = c0 + oim / kk;
+    const unsigned char *raw = a.raw + (long long)s * a.raw_pitch;
+    cpx *row = a.row0 + (long long)s * a.spitch;
+    double sr = 0.0, si = 0.0;
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < a.n; i += stride) {
+        const cpx o = chain_conv<FMT>(raw, i, cre, cim, kk, ore, oim);
+        row[i] = o;
+        sr += (double)o.x;
+        si += (double)o.y;
+    }
+    __shared__ double shr[256], shi[256];
+    shr[threadIdx.x] = sr;
+    shi[threadIdx.x] = si;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            shr[threadIdx.x] += shr[threadIdx.x + w];
+            shi[threadIdx.x] += shi[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double *p = a.partials + ((long long)s * a.parts + blockIdx.x) * 2;
+        p[0] = shr[0];
+        p[1] = shi[0];
+    }
+}
+
+// per stream: sum the partials in order, mean = sum/n + off; with update != 0
+// the offset for the next block becomes (float)mean
+__global__ void k_chain_mean(const double *__restrict__ partials, int parts, long long n, float *off,
+                             double *__restrict__ mean, int update, int S) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    double sr = 0.0, si = 0.0;
+    for (int b = 0; b < parts; ++b) {
+        sr += partials[((long long)s * parts + b) * 2];
+        si += partials[((long long)s * parts + b) * 2 + 1];
+    }
+    const double inv = n ? 1.0 / (double)n : 0.0;
+    const double ore = off ? (double)off[2 * s] : 0.0, oim = off ? (double)off[2 * s + 1] : 0.0;
+    const double mr = sr * inv + ore, mi = si * inv + oim;
+    mean[2 * s] = mr;
+    mean[2 * s + 1] = mi;
+    if (update && off) {
+        off[2 * s] = (float)mr;
+        off[2 * s + 1] = (float)mi;
+    }
+}
+
+extern "C" int atkgpu_chain_unpack(void *st, const void *raw, size_t raw_pitch, int fmt, size_t n, void *row0,
+                                   size_t spitch, float *off, double *partials, int parts, double *mean,
+                                   int update, unsigned S) {
+    if (!n || !S) return 0;
+    ChainUnpackArgs a;
+    a.raw = (const unsigned char *)raw;
+    a.raw_pitch = (long long)raw_pitch;
+    a.fmt = fmt;
+    a.n = (long long)n;
+    a.row0 = (cpx *)row0;
+    a.spitch = (long long)spitch;
+    a.off = off;
+    a.partials = partials;
+    a.parts = parts;
+    hipStream_t s = (hipStream_t)st;
+    dim3 grid((unsigned)parts, S);
+    float c0 = 0.0f, kk = 1.0f;
+    switch (fmt) {
+    case FMT_CU8:
+        c0 = 127.5f; kk = 1.0f / 127.5f;
+        hipLaunchKernelGGL(k_chain_unpack<FMT_CU8>, grid, dim3(256), 0, s, a, c0, kk);
+        break;
+    case FMT_CI8:
+        kk = 1.0f / 128.0f;
+        hipLaunchKernelGGL(k_chain_unpack<FMT_CI8>, grid, dim3(256), 0, s, a, c0, kk);
+        break;
+    case FMT_CI16:
+    case FMT_CI16Q11:
+        kk = fmt == FMT_CI16 ? 1.0f / 32768.0f : 1.0f / 2048.0f;
+        hipLaunchKernelGGL(k_chain_unpack<FMT_CI16>, grid, dim3(256), 0, s, a, c0, kk);
+        break;
+    case FMT_CF32:
+        hipLaunchKernelGGL(k_chain_unpack<FMT_CF32>, grid, dim3(256), 0, s, a, c0, kk);
+        break;
+    default:
+        return -1;
+    }
+    if (LAUNCH_OK()) return -4;
+    hipLaunchKernelGGL(k_chain_mean, dim3((S + 63) / 64), dim3(64), 0, s, (const double *)partials, parts,
+                       (long long)n, off, mean, update, (int)S);
+    return LAUNCH_OK();
+}
+
+// ---- the display reduction --------------------------------------------------------------
+struct DispArgs {
+    const cpx *row0;            // stream s frame 0 at row0 + s*spitch
+    long long spitch;
+    long long frames, fhop;
+    int fpb;                    // frames per block
+    int nfb;                    // frame blocks per stream
+    const float *win;           // N or null
+    const cpx *tw;              // N twiddles
+    int det;
+    float *partial;             // [S][nfb][N] display order (fftshifted)
+};
+
+template <int N>
+struct DispShape {
+    static constexpr int PT = 16;
+    static constexpr int NT = N / PT;
+    static constexpr int TEAMS = NT >= 256 ? 1 : 256 / NT;
+};
+
+template <int N>
+__global__ void __launch_bounds__(256) k_disp_reduce(DispArgs a) {
+#pragma clang fp contract(off)
+    using D = DispShape<N>;
+    constexpr int PT = D::PT, NT = D::NT, TEAMS = D::TEAMS;
+    __shared__ cpx lds[TEAMS * N];
+    __shared__ float red[TEAMS > 1 ? TEAMS * N : 1];
+    const int s = blockIdx.y, fb = blockIdx.x;
+    const int tm = threadIdx.x / NT, j = threadIdx.x % NT;
+    const cpx *row = a.row0 + (long long)s * a.spitch;
+    float w[PT];
+#pragma unroll
+    for (int r = 0; r < PT; ++r) w[r] = a.win ? a.win[j + r * NT] : 1.0f;
+    const float inv2 = 1.0f / ((float)N * (float)N);
+    float acc[PT];
+    bool first = true;
+    const long long f0 = (long long)fb * a.fpb;
+    long long f1 = f0 + a.fpb;
+    if (f1 > a.frames) f1 = a.frames;
+    // each team takes every TEAMS-th frame of the block's range
+    for (long long fbase = f0; fbase < f1; fbase += TEAMS) {
+        const long long f = fbase + tm;
+        const bool act = f < f1;
+        cpx v[PT];
+#pragma unroll
+        for (int r = 0; r < PT; ++r) {
+            cpx x = act ? row[f * a.fhop + j + r * NT] : cmake(0.f, 0.f);
+            v[r] = cmake(x.x * w[r], x.y * w[r]);
+        }
+        team_fft<N, PT>(v, lds + tm * N, a.tw, j);
+        if (act) {
+#pragma unroll
+            for (int r = 0; r < PT; ++r) {
+                const float pw = (v[r].x * v[r].x + v[r].y * v[r].y) * inv2;
+                if (first) acc[r] = pw;
+                else if (a.det == 0) { if (pw > acc[r]) acc[r] = pw; }
+                else if (a.det == 1) { if (pw < acc[r]) acc[r] = pw; }
+                else acc[r] += pw;
+            }
+            first = false;
+        }
+    }
+    // combine the teams in team order, then write the block's partial line in
+    // display order: FFT bin k = j + r*NT sits at display bin (k + N/2) mod N
+    float *out = a.partial + ((long long)s * a.nfb + fb) * N;
+    if constexpr (TEAMS == 1) {
+#pragma unroll
+        for (int r = 0; r < PT; ++r) out[j + ((r + PT / 2) % PT) * NT] = first ? 0.0f : acc[r];
+    } else {
+        // a team with no frame (only at the very end of a stream) contributes nothing:
+        // mark it with NaN, skipped below
+#pragma unroll
+        for (int r = 0; r < PT; ++r) red[tm * N + j + r * NT] = first ? __builtin_nanf("") : acc[r];
+        __syncthreads();
+        for (int k = threadIdx.x; k < N; k += 256) {
+            float v = red[k];
+            for (int t = 1; t < TEAMS; ++t) {
+                const float u = red[t * N + k];
+                if (u != u) continue;
+                if (v != v) { v = u; continue; }
+                if (a.det == 0) { if (u > v) v = u; }
+                else if (a.det == 1) { if (u < v) v = u; }
+                else v += u;
+            }
+            out[(k + N / 2) % N] = (v != v) ? 0.0f : v;
+        }
+    }
+}
+
+// per (stream, bin): combine the frame blocks in order, then dB (reduce's
+// 20 log10(sqrt(P) + 1e-12), src/fft.c:286-292)
+__global__ void k_disp_final(const float *__restrict__ partial, int nfb, int N, int det, long long frames,
+                             float *__restrict__ line_db) {
+#pragma clang fp contract(off)
+    const int s = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float *p = partial + (long long)s * nfb * N + i;
+    float v = p[0];
+    for (int b = 1; b < nfb; ++b) {
+        const float u = p[(long long)b * N];
+        if (det == 0) { if (u > v) v = u; }
+        else if (det == 1) { if (u < v) v = u; }
+        else v += u;
+    }
+    const float scale = det == 2 ? 1.0f / (float)frames : 1.0f;
+    line_db[(long long)s * N + i] = 20.0f * log10f(sqrtf(v * scale) + 1e-12f);
+}
+
+// per stream: decimate the dB line to `cols` columns, then stitch those into
+// the stream's trace (NaN = unmeasured)
+__global__ void __launch_bounds__(256)
+k_disp_post(const float *__restrict__ line_db, int N, int cols, float *__restrict__ dec, float *__restrict__ trace,
+            int nout, double seg_lo0, double seg_lo_step, double seg_step, double out_lo, double out_step) {
+#pragma clang fp contract(off)
+    const int s = blockIdx.x;
+    const float *in = line_db + (long long)s * N;
+    float *d = dec + (long long)s * cols;
+    for (int c = threadIdx.x; c < cols; c += 256) {
+        if (cols >= N) {
+            d[c] = in[c < N ? c : N - 1];
+            continue;
+        }
+        const int per = N / cols;
+        const int a0 = c * per, b0 = (c + 1 == cols) ? N : a0 + per;
+        float best = in[a0];
+        for (int i = a0 + 1; i < b0; ++i)
+            if (in[i] > best) best = in[i];
+        d[c] = best;
+    }
+    if (!trace || nout <= 0) return;
+    __syncthreads();
+    // stitch: segment bin i at seg_lo + (i + 0.5) seg_step, kept over the whole
+    // segment span; runs that land on one output bin are applied in order by
+    // their first bin's thread
+    const double seg_lo = seg_lo0 + seg_lo_step * s;
+    const double keep_lo = seg_lo, keep_hi = seg_lo + seg_step * cols;
+    const double out_hi = out_lo + out_step * (double)nout;
+    float *t = trace + (long long)s * nout;
+    auto kmap = [&](int i, long long *k) -> bool {
+        const double f = seg_lo + ((double)i + 0.5) * seg_step;
+        if (f < keep_lo || f > keep_hi || f < out_lo || f > out_hi) return false;
+        long long kk = (long long)((f - out_lo) / out_step);
+        if (kk < 0) kk = 0;
+        if (kk >= nout) kk = nout - 1;
+        *k = kk;
+        return true;
+    };
+    for (int i = threadIdx.x; i < cols; i += 256) {
+        long long k, kp;
+        if (!kmap(i, &k)) continue;
+        if (i > 0 && kmap(i - 1, &kp) && kp == k) continue;
+        float cur = t[k];
+        for (int q = i; q < cols; ++q) {
+            long long kq;
+            if (!kmap(q, &kq) || kq != k) break;
+            const float v = d[q];
+            if (cur != cur || v > cur) cur = v;
+        }
+        t[k] = cur;
+    }
+}
+
+template <int N>
+static int launch_disp(const DispArgs &a, unsigned S, hipStream_t s) {
+    hipLaunchKernelGGL(k_disp_reduce<N>, dim3((unsigned)a.nfb, S), dim3(256), 0, s, a);
+    return LAUNCH_OK();
+}
+
+extern "C" int atkgpu_chain_display(void *st, const void *row0, size_t spitch, unsigned S, size_t frames,
+                                    size_t fhop, unsigned N, int nfb, const float *win, const void *tw, int det,
+                                    float *partial, float *line_db) {
+    if (!frames || !S) return 0;
+    DispArgs a;
+    a.row0 = (const cpx *)row0;
+    a.spitch = (long long)spitch;
+    a.frames = (long long)frames;
+    a.fhop = (long long)fhop;
+    a.nfb = nfb;
+    a.fpb = (int)((frames + nfb - 1) / nfb);
+    a.nfb = (int)((frames + a.fpb - 1) / a.fpb);
+    a.win = win;
+    a.tw = (const cpx *)tw;
+    a.det = det;
+    a.partial = partial;
+    hipStream_t s = (hipStream_t)st;
+    int rc;
+    switch (N) {
+    case 1024: rc = launch_disp<1024>(a, S, s); break;
+    case 2048: rc = launch_disp<2048>(a, S, s); break;
+    case 4096: rc = launch_disp<4096>(a, S, s); break;
+    default: return -1;
+    }
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_disp_final, dim3((N + 255) / 256, S), dim3(256), 0, s, (const float *)partial, a.nfb,
+                       (int)N, det, (long long)frames, line_db);
+    return LAUNCH_OK();
+}
+
+extern "C" int atkgpu_chain_post(void *st, const float *line_db, unsigned S, unsigned N, unsigned cols, float *dec,
+                                 float *trace, unsigned nout, double seg_lo0, double seg_lo_step, double seg_step,
+                                 double out_lo, double out_step) {
+    if (!S) return 0;
+    hipLaunchKernelGGL(k_disp_post, dim3(S), dim3(256), 0, (hipStream_t)st, line_db, (int)N, (int)cols, dec, trace,
+                       (int)nout, seg_lo0, seg_lo_step, seg_step, out_lo, out_step);
+    return LAUNCH_OK();
+}
+
+// ---- keep the last `keep` samples of every stream row ------------------------------------
+__global__ void k_chain_keep(const cpx *__restrict__ src, long long pitch, cpx *__restrict__ dst, long long n) {
+    const int s = blockIdx.y;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        dst[(long long)s * pitch + i] = src[(long long)s * pitch + i];
+}
+
+// src/dst: stream 0's first sample to move / destination, rows spitch apart
+extern "C" int atkgpu_chain_keep(void *st, const void *src, size_t spitch, void *dst, size_t keep, unsigned S) {
+    if (!keep || !S) return 0;
+    const unsigned gx = grid_for(keep, 256, 64);
+    hipLaunchKernelGGL(k_chain_keep, dim3(gx, S), dim3(256), 0, (hipStream_t)st, (const cpx *)src,
+                       (long long)spitch, (cpx *)dst, (long long)keep);
+    return LAUNCH_OK();
+}

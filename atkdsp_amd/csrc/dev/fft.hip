@@ -18,6 +18,7 @@
 #include "atkgpu_dev.h"
 #include "dev_common.hpp"
 #include "fft_core.hpp"
+#include "fft_team.hpp"
 
 #define LDS_MAX_N 8192
 
@@ -54,29 +55,6 @@ __device__ __forceinline__ void io_store(const FftIO &io, long long t, long long
         y.y *= io.scale;
     }
     io.out[t * io.out_dist + k * io.out_stride] = y;
-}
-
-// ---- team FFT over the stages of TeamFftPlan<N, PT> --------------------------
-template <int N, int PT, int STAGE>
-__device__ __forceinline__ void team_fft_from(cpx *v, cpx *lds, const cpx *tw, int j, int Ns) {
-    using P = TeamFftPlan<N, PT>;
-    if constexpr (STAGE < P::S) {
-        constexpr bool last = (STAGE == P::NSTAGE - 1);
-        team_stage<N, PT, PT>(v, lds, tw, j, Ns, last);
-        if constexpr (!last) {
-            __syncthreads();
-            team_load<N, PT>(v, lds, j);
-            __syncthreads();
-            team_fft_from<N, PT, STAGE + 1>(v, lds, tw, j, Ns * PT);
-        }
-    } else if constexpr (P::RL > 1) {
-        team_stage<N, PT, P::RL>(v, lds, tw, j, Ns, true);
-    }
-}
-
-template <int N, int PT>
-__device__ __forceinline__ void team_fft(cpx *v, cpx *lds, const cpx *tw, int j) {
-    team_fft_from<N, PT, 0>(v, lds, tw, j, 1);
 }
 
 template <int N>

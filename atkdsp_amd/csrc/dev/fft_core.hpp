@@ -175,15 +175,29 @@ FFT_HD cpx generic_stage_output(const cpx *in, const cpx *tw, int n, int Ns, int
     //     = sum_r x[j + r nr] * W_n^{r*(step + m*nr)}   (W_R = W_n^{nr})
     const long long unit = (long long)step + (long long)m * nr;
     const long long nn = n;
-    float ax = 0.f, ay = 0.f;
     long long e = 0;
+    if (R <= 16) {
+        float ax = 0.f, ay = 0.f;
+        for (int r = 0; r < R; ++r) {
+            const cpx a = in[j + r * nr];
+            const cpx w = tw[(int)e];
+            ax += a.x * w.x - a.y * w.y;
+            ay += a.x * w.y + a.y * w.x;
+            e += unit;
+            if (e >= nn) e -= nn * (e / nn);
+        }
+        return cmake(ax, ay);
+    }
+    // a long prime radix sums many terms: accumulate in FP64 so the error
+    // stays at the twiddles' own rounding
+    double ax = 0.0, ay = 0.0;
     for (int r = 0; r < R; ++r) {
         const cpx a = in[j + r * nr];
         const cpx w = tw[(int)e];
-        ax += a.x * w.x - a.y * w.y;
-        ay += a.x * w.y + a.y * w.x;
+        ax += (double)a.x * w.x - (double)a.y * w.y;
+        ay += (double)a.x * w.y + (double)a.y * w.x;
         e += unit;
         if (e >= nn) e -= nn * (e / nn);
     }
-    return cmake(ax, ay);
+    return cmake((float)ax, (float)ay);
 }

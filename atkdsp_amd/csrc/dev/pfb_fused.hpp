@@ -1,0 +1,161 @@
+// The fused polyphase filterbank kernel for power-of-two channel counts
+// (reference algorithm: src/channelizer.c:218-303). One launch does, per
+// group of G consecutive hops of one element:
+//
+//   fold   thread = (column u of a hop-row, subset of hops). Hop i's window is
+//          rows i .. i+KOS-1 of hop-rows R_j[u] = s[j*HOP + u], and fold index
+//          q = r*HOP + u sums rows t = k*OS + r (k ascending) with tap
+//          h[t*HOP + u]. Each loaded sample feeds every open hop it belongs to,
+//          so a row is read once per group from L2, FMA accumulate in FP32.
+//   FFT    the G fold rows go to LDS; teams of M/PT threads run the M-point
+//          team FFT (fft_core.hpp) in registers with LDS exchanges.
+//   rotate exp(-2 pi i k shift/M), shift = ((b+1)*HOP) mod M (int64 hop
+//          counter; at 2x this is (-1)^k on odd hops), then the cal factor.
+//   store  bins go back to LDS and leave channel-major, G consecutive hops of
+//          a channel per lane group, out[(e*M + k)*opitch + i].
+//
+// Taps are zero-padded from K*OS to KOS (an instantiated size): a zero tap
+// adds an exact zero, so results equal the unpadded sum for finite input.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "dev_common.hpp"
+#include "fft_core.hpp"
+#include "fft_team.hpp"
+
+template <int M, int OS>
+struct PfbShape {
+    static constexpr int HOP = M / OS;
+    static constexpr int HOPC = HOP >= 256 ? 256 : HOP;     // columns per pass
+    static constexpr int SUB = 256 / HOPC;                   // hop subsets
+    static constexpr int COLS = HOP >= 256 ? HOP / 256 : 1;  // column passes
+    static constexpr int PT = M >= 256 ? 16 : 8;
+    static constexpr int NT = M / PT;                        // FFT team
+    static constexpr int TEAMS = 256 / NT;
+    static constexpr int GP0 = COLS >= 4 ? 4 : 8;
+    // hops a subset folds: at least enough that every FFT team has a hop
+    static constexpr int GP = SUB * GP0 >= TEAMS ? GP0 : TEAMS / SUB;
+    static constexpr int G = SUB * GP;                       // hops per group
+    static constexpr int ROW = M + 1;                        // padded LDS row
+    static constexpr int LPC = G >= 64 ? 64 : G;             // store lanes per channel
+    static_assert(NT <= 256 && G % TEAMS == 0, "shape");
+};
+
+struct PfbFusedArgs {
+    const cpx *s;          // stream rows, element e at s + e*spitch
+    long long spitch;
+    long long nvalid;      // valid samples in each element's stream
+    long long m;           // hops to produce
+    long long b0;          // global hop index of hop 0
+    const float *taps;     // KOS*HOP, zero-padded
+    const cpx *tw;         // M twiddles
+    const cpx *cal;        // [cal_nel][M] or null
+    int cal_nel;
+    cpx *out;              // element e, channel k, hop i at out[(e*M + k)*opitch + ocol + i]
+    long long opitch, ocol;
+    int groups_per_block;
+    int nblk_per_el;       // blocks per element
+    int nblocks;           // real blocks (grid may be padded to a multiple of 8)
+};
+
+template <int M, int OS, int KOS>
+__global__ void __launch_bounds__(256) k_pfb_fused(PfbFusedArgs a) {
+    using S = PfbShape<M, OS>;
+    constexpr int HOP = S::HOP, G = S::G, GP = S::GP, ROW = S::ROW;
+    __shared__ cpx lds[G * ROW];
+
+    // XCD-aware order: the hardware deals blocks round-robin to 8 XCDs, so
+    // give each XCD a contiguous run of tiles (neighbours share KOS-1 rows)
+    const int nb8 = (int)gridDim.x / 8;
+    const int lb = (int)(blockIdx.x % 8) * nb8 + (int)(blockIdx.x / 8);
+    if (lb >= a.nblocks) return;
+    const int e = lb / a.nblk_per_el;
+    const int tile = lb % a.nblk_per_el;
+    const cpx *s = a.s + (long long)e * a.spitch;
+    const int tid = threadIdx.x;
+    const int sub = tid / S::HOPC;
+    const int ucol = tid % S::HOPC;
+
+    // this thread's taps: h[t*HOP + u], t < KOS, for each column pass
+    float h[S::COLS][KOS];
+#pragma unroll
+    for (int c = 0; c < S::COLS; ++c)
+#pragma unroll
+        for (int t = 0; t < KOS; ++t) h[c][t] = a.taps[t * HOP + ucol + c * 256];
+
+    for (int grp = 0; grp < a.groups_per_block; ++grp) {
+        const long long i0 = ((long long)tile * a.groups_per_block + grp) * G;
+        if (i0 >= a.m) break;
+        // ---- fold ----
+#pragma unroll
+        for (int c = 0; c < S::COLS; ++c) {
+            const int u = ucol + c * 256;
+            float ar[GP][OS], ai[GP][OS];
+#pragma unroll
+            for (int p = 0; p < GP; ++p)
+#pragma unroll
+                for (int r = 0; r < OS; ++r) { ar[p][r] = 0.f; ai[p][r] = 0.f; }
+            const long long row0 = i0 + sub * GP;
+#pragma unroll
+            for (int jj = 0; jj < GP + KOS - 1; ++jj) {
+                const long long idx = (row0 + jj) * HOP + u;
+                cpx x = cmake(0.f, 0.f);
+                if (idx < a.nvalid) x = s[idx];
+#pragma unroll
+                for (int t = 0; t < KOS; ++t) {
+                    const int p = jj - t;
+                    if (p >= 0 && p < GP) {
+                        ar[p][t % OS] = fmaf(h[c][t], x.x, ar[p][t % OS]);
+                        ai[p][t % OS] = fmaf(h[c][t], x.y, ai[p][t % OS]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < GP; ++p)
+#pragma unroll
+                for (int r = 0; r < OS; ++r)
+                    lds[(sub * GP + p) * ROW + r * HOP + u] = cmake(ar[p][r], ai[p][r]);
+        }
+        __syncthreads();
+        // ---- FFT, rotation, cal: team tm transforms hops tm, tm+TEAMS, ... ----
+        const int tm = tid / S::NT, j = tid % S::NT;
+#pragma unroll
+        for (int rnd = 0; rnd < G / S::TEAMS; ++rnd) {
+            const int p = tm + rnd * S::TEAMS;
+            cpx *row = lds + p * ROW;
+            cpx v[S::PT];
+            team_load<M, S::PT>(v, row, j);
+            __syncthreads();
+            if constexpr (S::NT == 1) dft_pow2<S::PT>(v);
+            else team_fft_from<M, S::PT, 0>(v, row, a.tw, j, 1);
+            const long long bi = a.b0 + i0 + p + 1;
+            const bool flip = (OS == 2) && (bi & 1);
+            const unsigned shift = (OS == 1 || OS == 2) ? 0u : (unsigned)(((bi * HOP) % M + M) % M);
+            const cpx *cal = (a.cal && e < a.cal_nel) ? a.cal + (long long)e * M : nullptr;
+#pragma unroll
+            for (int r = 0; r < S::PT; ++r) {
+                const int k = j + r * S::NT;
+                cpx y = v[r];
+                if (flip && (k & 1)) y = cmake(-y.x, -y.y);
+                if (shift) y = cmul(y, a.tw[(unsigned)(((unsigned long long)k * shift) % M)]);
+                if (cal) y = cmul(y, cal[k]);
+                row[k] = y;
+            }
+            __syncthreads();
+        }
+        // ---- channel-major store: LPC lanes per channel, consecutive hops ----
+        {
+            constexpr int CH_PER_PASS = 256 / S::LPC;
+            const int lp = tid % S::LPC, lc = tid / S::LPC;
+#pragma unroll 4
+            for (int k0 = 0; k0 < M; k0 += CH_PER_PASS) {
+                const int k = k0 + lc;
+#pragma unroll
+                for (int p = lp; p < G; p += S::LPC) {
+                    const long long i = i0 + p;
+                    if (i < a.m) a.out[((long long)e * M + k) * a.opitch + a.ocol + i] = lds[p * ROW + k];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}

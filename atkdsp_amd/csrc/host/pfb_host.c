@@ -21,7 +21,8 @@ struct atkdsp_pfb {
     unsigned nchan, os, K, max_nel, hop, ov;
     size_t L;
     double *h;                 /* prototype, FP64 (host) */
-    float *d_taps;             /* (float)h, device       */
+    float *d_taps;             /* (float)h, device, zero-padded to kos*hop */
+    int kos;                   /* > 0: the fused kernel runs this shape */
     atkgpu_fftplan *plan;      /* nchan-point            */
     void *stream;
     size_t slab;               /* input samples per sub-call (multiple of hop) */
@@ -63,11 +64,13 @@ static double kaiser_beta(double a) {
 /* Kaiser-windowed sinc lowpass at fc = 1/(2 nchan), unit DC gain, times nchan
  * (src/channelizer.c:79-99). Same FP64 operations in the same order, so the
  * taps equal the reference's. */
-static void build_prototype(atkdsp_pfb *p) {
-    const size_t L = p->L;
-    const double fc = 1.0 / (2.0 * (double)p->nchan);
+int atkgpu_pfb_prototype(double *h, unsigned nchan, unsigned K, double atten_db) {
+    if (!h || nchan < 2 || K < 1) return ATKDSP_E_ARG;
+    if (atten_db < 20.0) atten_db = 20.0;
+    const size_t L = (size_t)nchan * K;
+    const double fc = 1.0 / (2.0 * (double)nchan);
     const double Mid = (double)(L - 1) * 0.5;
-    const double beta = kaiser_beta(p->atten);
+    const double beta = kaiser_beta(atten_db);
     const double i0b = pfb_i0(beta);
     double sum = 0.0;
     for (int pass = 0; pass < 2; ++pass) {
@@ -79,10 +82,11 @@ static void build_prototype(atkdsp_pfb *p) {
             const double w = pfb_i0(beta * sqrt(r * r < 1.0 ? 1.0 - r * r : 0.0)) / i0b;
             const double hv = 2.0 * fc * sinc * w;
             if (pass == 0) sum += hv;
-            else p->h[n] = (hv / sum) * (double)p->nchan;
+            else h[n] = (hv / sum) * (double)nchan;
         }
         if (sum == 0.0) sum = 1.0;
     }
+    return ATKDSP_OK;
 }
 
 static void pfb_free(atkdsp_pfb *p) {
@@ -133,7 +137,9 @@ atkdsp_pfb *atkdsp_pfb_create(double fs, unsigned nchan, unsigned oversample, un
     if (!p->h || !p->lo || atkgpu_stream_create(&p->stream)) { pfb_free(p); return NULL; }
     p->plan = atkgpu_fftplan_create(nchan);
     const size_t nm = (size_t)max_nel * p->hops_cap * nchan;
-    p->d_taps = (float *)atkgpu_dmalloc(p->L * sizeof(float));
+    p->kos = atkgpu_pfb_fused_kos(nchan, oversample, taps_per_poly);
+    const size_t ntaps = p->kos ? (size_t)p->kos * p->hop : p->L;
+    p->d_taps = (float *)atkgpu_dmalloc(ntaps * sizeof(float));
     p->d_s[0] = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)max_nel * p->spitch * sizeof(atkdsp_cf32));
     p->d_s[1] = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)max_nel * p->spitch * sizeof(atkdsp_cf32));
     p->d_fold = (atkdsp_cf32 *)atkgpu_dmalloc(nm * sizeof(atkdsp_cf32));
@@ -146,11 +152,11 @@ atkdsp_pfb *atkdsp_pfb_create(double fs, unsigned nchan, unsigned oversample, un
         pfb_free(p);
         return NULL;
     }
-    build_prototype(p);
-    float *hf = (float *)malloc(p->L * sizeof(float));
+    atkgpu_pfb_prototype(p->h, nchan, taps_per_poly, p->atten);
+    float *hf = (float *)calloc(ntaps, sizeof(float));
     if (!hf) { pfb_free(p); return NULL; }
     for (size_t q = 0; q < p->L; ++q) hf[q] = (float)p->h[q];
-    int rc = atkgpu_h2d(p->stream, p->d_taps, hf, p->L * sizeof(float));
+    int rc = atkgpu_h2d(p->stream, p->d_taps, hf, ntaps * sizeof(float));
     /* zero history (the reference callocs it) */
     if (!rc) rc = atkgpu_fill_u32(p->stream, p->d_s[0], 0, (size_t)max_nel * p->spitch * 2);
     if (!rc) rc = atkgpu_sync(p->stream);
@@ -205,9 +211,9 @@ int atkdsp_pfb_describe(const atkdsp_pfb *p, char *buf, size_t cap) {
     if (!p || !buf || cap == 0) return ATKDSP_E_ARG;
     return snprintf(buf, cap,
                     "PFB %.6g MSPS: %u chan x%u oversample, %.6g kHz spacing, %.6g kHz out, "
-                    "%zu-tap proto (%u/poly), %.0f dB, <=%u el; hip gfx950 fp32 fold, fp32 fft",
+                    "%zu-tap proto (%u/poly), %.0f dB, <=%u el; hip gfx950 %s, fp32 fold and fft",
                     p->fs / 1e6, p->nchan, p->os, p->fs / p->nchan / 1e3, atkdsp_pfb_out_rate(p) / 1e3, p->L,
-                    p->K, p->atten, p->max_nel);
+                    p->K, p->atten, p->max_nel, p->kos ? "fused kernel" : "general kernels");
 }
 
 /* One sub-call over ins[e][pos, pos+c): writes its m_s hops at column i_done
@@ -231,11 +237,18 @@ static int pfb_sub(atkdsp_pfb *p, const atkdsp_cf32 *const *ins, unsigned nel, s
         if (p->lc) rc = atkgpu_h2d(p->stream, se + ov, p->lo + (size_t)e * hop, p->lc * sizeof(atkdsp_cf32));
         if (!rc && c) rc = atkgpu_h2d(p->stream, se + ov + p->lc, ins[e] + pos, c * sizeof(atkdsp_cf32));
     }
-    if (!rc) rc = atkgpu_pfb_fold(p->stream, S, p->spitch, nel, ms, M, hop, p->K, p->d_taps, p->d_fold);
-    if (!rc) rc = atkgpu_fft_batch(p->stream, p->plan, p->d_fold, M, NULL, (size_t)nel * ms, 0, p->d_fold, p->d_scr);
-    if (!rc) rc = atkgpu_pfb_rotate_store(p->stream, p->d_fold, nel, ms, M, hop, (long long)p->b,
-                                          atkgpu_fftplan_twiddles(p->plan),
-                                          p->cal_nel ? (const void *)p->d_cal : NULL, p->cal_nel, p->d_out);
+    if (p->kos) {
+        if (!rc) rc = atkgpu_pfb_fused(p->stream, S, p->spitch, (size_t)ov + avail, nel, ms, (long long)p->b, M,
+                                       p->os, (unsigned)p->kos, p->d_taps, atkgpu_fftplan_twiddles(p->plan),
+                                       p->cal_nel ? (const void *)p->d_cal : NULL, p->cal_nel, p->d_out, ms, 0);
+    } else {
+        if (!rc) rc = atkgpu_pfb_fold(p->stream, S, p->spitch, nel, ms, M, hop, p->K, p->d_taps, p->d_fold);
+        if (!rc) rc = atkgpu_fft_batch(p->stream, p->plan, p->d_fold, M, NULL, (size_t)nel * ms, 0, p->d_fold,
+                                       p->d_scr);
+        if (!rc) rc = atkgpu_pfb_rotate_store(p->stream, p->d_fold, nel, ms, M, hop, (long long)p->b,
+                                              atkgpu_fftplan_twiddles(p->plan),
+                                              p->cal_nel ? (const void *)p->d_cal : NULL, p->cal_nel, p->d_out);
+    }
     if (!rc) rc = atkgpu_d2h_2d(p->stream, out + i_done, m * sizeof(atkdsp_cf32), p->d_out,
                                 ms * sizeof(atkdsp_cf32), ms * sizeof(atkdsp_cf32), (size_t)nel * M);
     /* next history: s[ms hop, ms hop + ov) into the other slab buffer */

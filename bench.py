@@ -1,0 +1,197 @@
+"""Throughput of the wideband front end on one MI355X.
+
+Workload (one step): one 50 ms block of the 40 MSPS ci16q11 receiver workload
+(2,000,000 samples: -30 dBFS noise, 20 tones, 4 bursts) for each of S
+independent streams (default 128, the >= 100 concurrent 40 MSPS streams the
+path is sized for), already resident in HBM, through
+    unpack with block DC removal -> polyphase filterbank (1024 channels,
+    16 taps/branch, 2x oversampled, history in HBM) -> 4096-point Hann
+    spectrum line at hop 2048 with max-hold (tail carried across blocks) ->
+    decimate to 1024 columns -> NaN-aware stitch into an 8192-bin trace.
+
+Prints one JSON line. `value` is aggregate input samples per second over all
+streams; `roofline` is the filterbank kernel (the dominant one) timed with
+HIP events on its own stream; `cpu_baseline` is the reference C library
+(built from its sources into oracle/_ref, OpenMP on the host cores) running
+the same chain on a bounded sample of the same blocks.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+FS = 40e6
+N_BLOCK = 2_000_000
+M, OS, K = 1024, 2, 16
+NFFT, FHOP, COLS, TRACE = 4096, 2048, 1024, 8192
+HBM_PEAK_GBS = 8000.0
+
+
+def make_streams(S: int):
+    from atkdsp_amd import synth
+    base = np.frombuffer(synth.c2_bytes(n=N_BLOCK), "<i2").reshape(-1, 2)
+    # stream s: the same block content at a different phase (circular shift)
+    return base, [np.roll(base, s * 7919, axis=0) for s in range(S)]
+
+
+def pfb_bytes_per_launch(S: int, n: int, m: int) -> float:
+    """Compulsory bytes of one filterbank launch: every input sample and the
+    history read once as cf32, every output written once as cf32."""
+    ov = M * K - M // OS
+    return S * ((n + ov) * 8 + m * M * 8)
+
+
+def cpu_baseline(seconds: float, base: np.ndarray):
+    """The reference C library on the host cores: the same chain per block,
+    streaming, blocks of the same content, until `seconds` elapse."""
+    try:
+        from oracle.ref import Ref, available, Cf
+    except Exception as e:                       # noqa: BLE001
+        return {"value": None, "unit": "Msamples/s", "cores": 0, "kind": "reference",
+                "sample": f"unavailable: {e}"}
+    if not available():
+        return {"value": None, "unit": "Msamples/s", "cores": 0, "kind": "reference",
+                "sample": "oracle/_ref/atkdsp_ref.so not built"}
+    r = Ref()
+    cores = min(16, os.cpu_count() or 1)
+    r.set_threads(cores)
+    L = r.L
+    raw = np.ascontiguousarray(base).tobytes()
+    pfb = L.atkdsp_pfb_create(FS, M, OS, K, 80.0, 1)
+    plan = L.atkdsp_fft_create(NFFT)
+    win = r.window(1, NFFT)
+    x = np.empty(N_BLOCK, np.complex64)
+    out = np.empty(M * (N_BLOCK // (M // OS) + 2), np.complex64)
+    line = np.empty(NFFT, np.float32)
+    cols = np.empty(COLS, np.float32)
+    trace = np.full(TRACE, np.nan, np.float32)
+    tail = np.zeros(0, np.complex64)
+    off = [0.0, 0.0]
+    b = np.frombuffer(raw, np.uint8)
+    cf = lambda a: a.ctypes.data_as(C.POINTER(Cf))
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    blocks = 0
+    t0 = time.perf_counter()
+    while True:
+        mr, mi = C.c_double(), C.c_double()
+        L.atkdsp_unpack_dc(b.ctypes.data, b.size, 3, cf(x), x.size, off[0], off[1], C.byref(mr), C.byref(mi))
+        off = [mr.value, mi.value]
+        ins = (C.POINTER(Cf) * 1)(cf(x))
+        L.atkdsp_pfb_process_n(pfb, ins, 1, x.size, cf(out), out.size)
+        d = np.concatenate([tail, x])
+        fr = L.atkdsp_spectrum_reduce(plan, cf(d), d.size, FHOP, fp(win), 0, fp(line))
+        tail = d[fr * FHOP:].copy()
+        L.atkdsp_decimate_max(fp(line), NFFT, fp(cols), COLS)
+        L.atkdsp_stitch_max(fp(cols), COLS, -FS / 2, FS / COLS, -FS / 2, FS / 2, -4 * FS, FS / COLS * 8 / 8,
+                            fp(trace), TRACE)
+        blocks += 1
+        el = time.perf_counter() - t0
+        if el >= seconds and blocks >= 2:
+            break
+    L.atkdsp_pfb_destroy(pfb)
+    L.atkdsp_fft_destroy(plan)
+    rate = blocks * N_BLOCK / el / 1e6
+    return {"value": round(rate, 2), "unit": "Msamples/s", "cores": cores, "kind": "reference",
+            "sample": f"{blocks} consecutive 2,000,000-sample ci16q11 blocks of one stream through "
+                      f"unpack_dc, pfb_process (1024 ch, 16 taps, 2x), spectrum_reduce (4096/2048 Hann max, "
+                      f"carried tail), decimate_max, stitch_max: {el:.1f} s wall on {cores} threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--streams", type=int, default=128)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(streams are independent: several GPUs would each take a share of them)", file=sys.stderr)
+        return 2
+    import atkdsp_amd
+    from atkdsp_amd.chain import Chain
+
+    S = args.streams
+    print(f"[bench] generating {S} streams of {N_BLOCK} samples", file=sys.stderr, flush=True)
+    base, streams = make_streams(S)
+    ch = Chain(fmt="ci16q11", fs=FS, nchan=M, oversample=OS, taps_per_poly=K, atten_db=80.0, nfft=NFFT,
+               fft_hop=FHOP, window="hann", detector="max", line_cols=COLS, trace_bins=TRACE,
+               fc0_hz=0.0, fc_step_hz=FS, trace_lo_hz=-FS / 2, trace_hz_per_bin=FS / 1024 * 8 / 8 * 4,
+               dc_mode=1, streams=S, block_max=N_BLOCK)
+    for s in range(S):
+        ch.upload(s, streams[s])
+    ch.sync()
+    print(f"[bench] uploaded; warmup {args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        ch.run(N_BLOCK)
+    ch.sync()
+    # the timed region: exactly `steps` blocks of every stream
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m = ch.run(N_BLOCK)
+    ch.sync()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    print(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
+    # per-kernel device times (HIP events on the chain's stream), one step at a time
+    runs, pfbs, disps, lat = [], [], [], []
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        ch.run(N_BLOCK)
+        a, b, d = ch.timing()           # waits for the step
+        lat.append((time.perf_counter() - ts) * 1e3)
+        runs.append(a)
+        pfbs.append(b)
+        disps.append(d)
+    ms_pfb = float(np.mean(pfbs))
+    pfb_b = pfb_bytes_per_launch(S, N_BLOCK, m)
+    achieved = pfb_b / (ms_pfb * 1e-3) / 1e9
+    value = S * N_BLOCK / (ms_step * 1e-3)
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, streams[0])
+    out = {
+        "metric": "channelized Msamples/s at M=1024 (1 GPU, %HBM peak); p99 chunk latency ms",
+        "value": round(value / 1e6, 1),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"40 MSPS ci16q11 2,000,000-sample blocks x {S} streams: unpack+DC, "
+                               f"PFB M=1024 K=16 2x, 4096/2048 Hann max line, 1024 cols, 8192-bin stitch",
+                   "streams": S, "block_samples": N_BLOCK, "nchan": M, "taps_per_poly": K, "oversample": OS,
+                   "nfft": NFFT, "fft_hop": FHOP, "parallelism": "single GPU"},
+        "p50_chunk_latency_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_chunk_latency_ms": round(float(np.percentile(lat, 99)), 3),
+        "latency_note": "submit-to-complete of one step: a 50 ms block of every stream",
+        "kernel_ms": {"run": round(float(np.mean(runs)), 4), "pfb_fused": round(ms_pfb, 4),
+                      "display": round(float(np.mean(disps)), 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_pfb_fused<1024,2,32>",
+                     "bytes_per_launch": pfb_b},
+        "cpu_baseline": cpu,
+        "build": atkdsp_amd.build_info(),
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
