@@ -188,7 +188,8 @@ __global__ void __launch_bounds__(256) k_disp_reduce(DispArgs a) {
 #pragma clang fp contract(off)
     using D = DispShape<N>;
     constexpr int PT = D::PT, NT = D::NT, TEAMS = D::TEAMS;
-    __shared__ cpx lds[TEAMS * N];
+    constexpr int TL = lpad_size<N>();
+    __shared__ cpx lds[TEAMS * TL];
     __shared__ float red[TEAMS > 1 ? TEAMS * N : 1];
     const int s = blockIdx.y, fb = blockIdx.x;
     const int tm = threadIdx.x / NT, j = threadIdx.x % NT;
@@ -212,7 +213,7 @@ __global__ void __launch_bounds__(256) k_disp_reduce(DispArgs a) {
             cpx x = act ? row[f * a.fhop + j + r * NT] : cmake(0.f, 0.f);
             v[r] = cmake(x.x * w[r], x.y * w[r]);
         }
-        team_fft<N, PT>(v, lds + tm * N, a.tw, j);
+        team_fft<N, PT>(v, lds + tm * TL, a.tw, j);
         if (act) {
 #pragma unroll
             for (int r = 0; r < PT; ++r) {

@@ -70,7 +70,8 @@ __global__ void __launch_bounds__(TeamShape<N>::BLOCK)
 k_fft_team(FftIO io, const cpx *__restrict__ tw) {
     using SH = TeamShape<N>;
     constexpr int PT = SH::PT, NT = SH::NT;
-    __shared__ cpx lds[SH::TEAMS * N];
+    constexpr int TL = lpad_size<N>();
+    __shared__ cpx lds[SH::TEAMS * TL];
     const int team = threadIdx.x / NT, j = threadIdx.x % NT;
     const long long t = (long long)blockIdx.x * SH::TEAMS + team;
     const bool act = t < io.batch;
@@ -81,7 +82,7 @@ k_fft_team(FftIO io, const cpx *__restrict__ tw) {
         // a single thread holds the whole transform
         if constexpr (N > 1) dft_pow2<N>(v);
     } else {
-        team_fft<N, PT>(v, lds + team * N, tw, j);
+        team_fft<N, PT>(v, lds + team * TL, tw, j);
     }
     if (act) {
 #pragma unroll

@@ -109,6 +109,13 @@ constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 // A team of NT = N/PT threads transforms N points. Thread j holds x[j + r*NT]
 // (r < PT) on entry and X[j + r*NT] on exit. Full stages have radix PT; a
 // final stage of radix RL = N / PT^S (RL < PT) runs PT/RL butterflies a thread.
+//
+// The team's LDS holds element i at lpad(i) = i + i/16 (N + N/16 slots): the
+// first stage writes at a stride of PT elements, which without the pad would
+// put every lane of a wavefront on the same LDS bank.
+FFT_HD int lpad(int i) { return i + (i >> 4); }
+template <int N>
+constexpr int lpad_size() { return N + N / 16 + 1; }
 template <int N, int PT>
 struct TeamFftPlan {
     static constexpr int NT = N / PT;
@@ -144,7 +151,7 @@ FFT_HD void team_stage(cpx *v, cpx *lds, const cpx *tw, int j, int Ns, bool last
         } else {
             const int base = (jj / Ns) * Ns * R + k;
 #pragma unroll
-            for (int m = 0; m < R; ++m) lds[base + m * Ns] = u[m];
+            for (int m = 0; m < R; ++m) lds[lpad(base + m * Ns)] = u[m];
         }
     }
     if (last) {
@@ -157,7 +164,7 @@ template <int N, int PT>
 FFT_HD void team_load(cpx *v, const cpx *lds, int j) {
     constexpr int NT = N / PT;
 #pragma unroll
-    for (int r = 0; r < PT; ++r) v[r] = lds[j + r * NT];
+    for (int r = 0; r < PT; ++r) v[r] = lds[lpad(j + r * NT)];
 }
 
 // ---- generic mixed-radix Stockham stage (runtime radix) ----------------------

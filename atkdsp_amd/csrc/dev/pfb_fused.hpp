@@ -35,7 +35,10 @@ struct PfbShape {
     // hops a subset folds: at least enough that every FFT team has a hop
     static constexpr int GP = SUB * GP0 >= TEAMS ? GP0 : TEAMS / SUB;
     static constexpr int G = SUB * GP;                       // hops per group
-    static constexpr int ROW = M + 1;                        // padded LDS row
+    // an LDS row holds one hop: the FFT's padded layout (lpad) during the
+    // transform, natural bin order after it; the odd stride spreads the
+    // store phase's reads of several hops over distinct banks
+    static constexpr int ROW = lpad_size<M>() | 1;
     static constexpr int LPC = G >= 64 ? 64 : G;             // store lanes per channel
     static_assert(NT <= 256 && G % TEAMS == 0, "shape");
 };
@@ -58,10 +61,11 @@ struct PfbFusedArgs {
 };
 
 template <int M, int OS, int KOS>
-__global__ void __launch_bounds__(256) k_pfb_fused(PfbFusedArgs a) {
+__global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
     using S = PfbShape<M, OS>;
     constexpr int HOP = S::HOP, G = S::G, GP = S::GP, ROW = S::ROW;
     __shared__ cpx lds[G * ROW];
+    __shared__ cpx twl[M];
 
     // XCD-aware order: the hardware deals blocks round-robin to 8 XCDs, so
     // give each XCD a contiguous run of tiles (neighbours share KOS-1 rows)
@@ -70,23 +74,26 @@ __global__ void __launch_bounds__(256) k_pfb_fused(PfbFusedArgs a) {
     if (lb >= a.nblocks) return;
     const int e = lb / a.nblk_per_el;
     const int tile = lb % a.nblk_per_el;
+    // per-element bases in 64-bit once; every offset below fits 32 bits
     const cpx *s = a.s + (long long)e * a.spitch;
+    cpx *oute = a.out + (long long)e * M * a.opitch + a.ocol;
+    const int nlast = (int)a.nvalid - 1;
+    const int m = (int)a.m;
+    const int opitch = (int)a.opitch;
     const int tid = threadIdx.x;
     const int sub = tid / S::HOPC;
     const int ucol = tid % S::HOPC;
+    for (int i = tid; i < M; i += 256) twl[i] = a.tw[i];
+    const cpx *cal = (a.cal && e < a.cal_nel) ? a.cal + (long long)e * M : nullptr;
+    const bool flip_base = (a.b0 & 1) != 0;     // parity of the hop counter at hop 0
 
-    // this thread's taps: h[t*HOP + u], t < KOS, for each column pass
-    float h[S::COLS][KOS];
-#pragma unroll
-    for (int c = 0; c < S::COLS; ++c)
-#pragma unroll
-        for (int t = 0; t < KOS; ++t) h[c][t] = a.taps[t * HOP + ucol + c * 256];
-
+    // The loops over column passes and FFT rounds stay rolled: the unrolled
+    // fold is long, and the kernel must fit the instruction cache.
     for (int grp = 0; grp < a.groups_per_block; ++grp) {
-        const long long i0 = ((long long)tile * a.groups_per_block + grp) * G;
-        if (i0 >= a.m) break;
+        const int i0 = (tile * a.groups_per_block + grp) * G;
+        if (i0 >= m) break;
         // ---- fold ----
-#pragma unroll
+#pragma unroll 1
         for (int c = 0; c < S::COLS; ++c) {
             const int u = ucol + c * 256;
             float ar[GP][OS], ai[GP][OS];
@@ -94,49 +101,57 @@ __global__ void __launch_bounds__(256) k_pfb_fused(PfbFusedArgs a) {
             for (int p = 0; p < GP; ++p)
 #pragma unroll
                 for (int r = 0; r < OS; ++r) { ar[p][r] = 0.f; ai[p][r] = 0.f; }
-            const long long row0 = i0 + sub * GP;
+            // rows past the valid samples only reach hops >= m (not stored) or
+            // zero padding taps, so the address is clamped, not the value
+            const int base = (i0 + sub * GP) * HOP + u;
+            // taps in NCH chunks, each over a register window of GP+TH-1 rows;
+            // every loop has a constant trip count and no condition, so all
+            // register indices are compile-time. Per hop the taps still run
+            // in ascending order (k ascending for each branch offset r).
+            constexpr int NCH = KOS >= 24 ? 4 : 2;
+            constexpr int TH = KOS / NCH;
 #pragma unroll
-            for (int jj = 0; jj < GP + KOS - 1; ++jj) {
-                const long long idx = (row0 + jj) * HOP + u;
-                cpx x = cmake(0.f, 0.f);
-                if (idx < a.nvalid) x = s[idx];
+            for (int half = 0; half < NCH; ++half) {
+                const int t0 = half * TH;
+                cpx x[GP + TH - 1];
+                float h[TH];
 #pragma unroll
-                for (int t = 0; t < KOS; ++t) {
-                    const int p = jj - t;
-                    if (p >= 0 && p < GP) {
-                        ar[p][t % OS] = fmaf(h[c][t], x.x, ar[p][t % OS]);
-                        ai[p][t % OS] = fmaf(h[c][t], x.y, ai[p][t % OS]);
+                for (int jj = 0; jj < GP + TH - 1; ++jj) x[jj] = s[min(base + (t0 + jj) * HOP, nlast)];
+#pragma unroll
+                for (int t = 0; t < TH; ++t) h[t] = a.taps[(t0 + t) * HOP + u];
+#pragma unroll
+                for (int p = 0; p < GP; ++p)
+#pragma unroll
+                    for (int t = 0; t < TH; ++t) {
+                        ar[p][(t0 + t) % OS] = fmaf(h[t], x[p + t].x, ar[p][(t0 + t) % OS]);
+                        ai[p][(t0 + t) % OS] = fmaf(h[t], x[p + t].y, ai[p][(t0 + t) % OS]);
                     }
-                }
             }
 #pragma unroll
             for (int p = 0; p < GP; ++p)
 #pragma unroll
                 for (int r = 0; r < OS; ++r)
-                    lds[(sub * GP + p) * ROW + r * HOP + u] = cmake(ar[p][r], ai[p][r]);
+                    lds[(sub * GP + p) * ROW + lpad(r * HOP + u)] = cmake(ar[p][r], ai[p][r]);
         }
         __syncthreads();
         // ---- FFT, rotation, cal: team tm transforms hops tm, tm+TEAMS, ... ----
         const int tm = tid / S::NT, j = tid % S::NT;
-#pragma unroll
+#pragma unroll 1
         for (int rnd = 0; rnd < G / S::TEAMS; ++rnd) {
             const int p = tm + rnd * S::TEAMS;
             cpx *row = lds + p * ROW;
             cpx v[S::PT];
             team_load<M, S::PT>(v, row, j);
             __syncthreads();
-            if constexpr (S::NT == 1) dft_pow2<S::PT>(v);
-            else team_fft_from<M, S::PT, 0>(v, row, a.tw, j, 1);
-            const long long bi = a.b0 + i0 + p + 1;
-            const bool flip = (OS == 2) && (bi & 1);
-            const unsigned shift = (OS == 1 || OS == 2) ? 0u : (unsigned)(((bi * HOP) % M + M) % M);
-            const cpx *cal = (a.cal && e < a.cal_nel) ? a.cal + (long long)e * M : nullptr;
+            team_fft_from<M, S::PT, 0>(v, row, twl, j, 1);
+            // at 2x the rotation is (-1)^k on hops whose b+1 is odd
+            // (reference src/channelizer.c:275-276 with shift = M/2)
+            const bool flip = (OS == 2) && (((i0 + p) & 1) == (flip_base ? 1 : 0));
 #pragma unroll
             for (int r = 0; r < S::PT; ++r) {
                 const int k = j + r * S::NT;
                 cpx y = v[r];
                 if (flip && (k & 1)) y = cmake(-y.x, -y.y);
-                if (shift) y = cmul(y, a.tw[(unsigned)(((unsigned long long)k * shift) % M)]);
                 if (cal) y = cmul(y, cal[k]);
                 row[k] = y;
             }
@@ -149,11 +164,10 @@ __global__ void __launch_bounds__(256) k_pfb_fused(PfbFusedArgs a) {
 #pragma unroll 4
             for (int k0 = 0; k0 < M; k0 += CH_PER_PASS) {
                 const int k = k0 + lc;
+                cpx *ok = oute + k * opitch + i0;
 #pragma unroll
-                for (int p = lp; p < G; p += S::LPC) {
-                    const long long i = i0 + p;
-                    if (i < a.m) a.out[((long long)e * M + k) * a.opitch + a.ocol + i] = lds[p * ROW + k];
-                }
+                for (int p = lp; p < G; p += S::LPC)
+                    if (i0 + p < m) ok[p] = lds[p * ROW + k];
             }
         }
         __syncthreads();
