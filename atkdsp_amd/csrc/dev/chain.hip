@@ -183,7 +183,26 @@ struct DispShape {
     static constexpr int TEAMS = NT >= 256 ? 1 : 256 / NT;
 };
 
-template <int N>
+// the reference's detector step (src/fft.c:184-189) as a select, so the
+// accumulation has no branches: the first frame takes pw, MAX keeps the first
+// of equals (pw > acc), MIN likewise (pw < acc), AVG sums
+template <int DET>
+__device__ __forceinline__ float det_step(float acc, float pw, bool first) {
+    float nv;
+    if constexpr (DET == 0) nv = pw > acc ? pw : acc;
+    else if constexpr (DET == 1) nv = pw < acc ? pw : acc;
+    else nv = acc + pw;
+    return first ? pw : nv;
+}
+
+template <int DET>
+__device__ __forceinline__ float det_merge(float v, float u) {
+    if constexpr (DET == 0) return u > v ? u : v;
+    else if constexpr (DET == 1) return u < v ? u : v;
+    else return v + u;
+}
+
+template <int N, int DET>
 __global__ void __launch_bounds__(256) k_disp_reduce(DispArgs a) {
 #pragma clang fp contract(off)
     using D = DispShape<N>;
@@ -191,40 +210,43 @@ __global__ void __launch_bounds__(256) k_disp_reduce(DispArgs a) {
     constexpr int TL = lpad_size<N>();
     __shared__ cpx lds[TEAMS * TL];
     __shared__ float red[TEAMS > 1 ? TEAMS * N : 1];
+    __shared__ cpx twl[N];                 // twiddles: every stage reads them from LDS
     const int s = blockIdx.y, fb = blockIdx.x;
     const int tm = threadIdx.x / NT, j = threadIdx.x % NT;
     const cpx *row = a.row0 + (long long)s * a.spitch;
+    for (int i = threadIdx.x; i < N; i += 256) twl[i] = a.tw[i];
+    __syncthreads();
     float w[PT];
 #pragma unroll
     for (int r = 0; r < PT; ++r) w[r] = a.win ? a.win[j + r * NT] : 1.0f;
     const float inv2 = 1.0f / ((float)N * (float)N);
     float acc[PT];
+#pragma unroll
+    for (int r = 0; r < PT; ++r) acc[r] = 0.0f;
     bool first = true;
-    const long long f0 = (long long)fb * a.fpb;
-    long long f1 = f0 + a.fpb;
-    if (f1 > a.frames) f1 = a.frames;
+    // frame offsets within a stream row fit 32 bits
+    const int fhop = (int)a.fhop;
+    const int f0 = fb * a.fpb;
+    const int f1 = min(f0 + a.fpb, (int)a.frames);
     // each team takes every TEAMS-th frame of the block's range
-    for (long long fbase = f0; fbase < f1; fbase += TEAMS) {
-        const long long f = fbase + tm;
+    for (int fbase = f0; fbase < f1; fbase += TEAMS) {
+        const int f = fbase + tm;
         const bool act = f < f1;
+        const cpx *fr = row + (act ? f : f0) * fhop + j;
         cpx v[PT];
 #pragma unroll
         for (int r = 0; r < PT; ++r) {
-            cpx x = act ? row[f * a.fhop + j + r * NT] : cmake(0.f, 0.f);
-            v[r] = cmake(x.x * w[r], x.y * w[r]);
+            const cpx x = fr[r * NT];
+            v[r] = act ? cmake(x.x * w[r], x.y * w[r]) : cmake(0.f, 0.f);
         }
-        team_fft<N, PT>(v, lds + tm * TL, a.tw, j);
-        if (act) {
+        team_fft<N, PT>(v, lds + tm * TL, twl, j);
 #pragma unroll
-            for (int r = 0; r < PT; ++r) {
-                const float pw = (v[r].x * v[r].x + v[r].y * v[r].y) * inv2;
-                if (first) acc[r] = pw;
-                else if (a.det == 0) { if (pw > acc[r]) acc[r] = pw; }
-                else if (a.det == 1) { if (pw < acc[r]) acc[r] = pw; }
-                else acc[r] += pw;
-            }
-            first = false;
+        for (int r = 0; r < PT; ++r) {
+            const float pw = (v[r].x * v[r].x + v[r].y * v[r].y) * inv2;
+            const float nv = det_step<DET>(acc[r], pw, first);
+            acc[r] = act ? nv : acc[r];
         }
+        first = first && !act;
     }
     // combine the teams in team order, then write the block's partial line in
     // display order: FFT bin k = j + r*NT sits at display bin (k + N/2) mod N
@@ -244,9 +266,7 @@ __global__ void __launch_bounds__(256) k_disp_reduce(DispArgs a) {
                 const float u = red[t * N + k];
                 if (u != u) continue;
                 if (v != v) { v = u; continue; }
-                if (a.det == 0) { if (u > v) v = u; }
-                else if (a.det == 1) { if (u < v) v = u; }
-                else v += u;
+                v = det_merge<DET>(v, u);
             }
             out[(k + N / 2) % N] = (v != v) ? 0.0f : v;
         }
@@ -329,7 +349,12 @@ k_disp_post(const float *__restrict__ line_db, int N, int cols, float *__restric
 
 template <int N>
 static int launch_disp(const DispArgs &a, unsigned S, hipStream_t s) {
-    hipLaunchKernelGGL(k_disp_reduce<N>, dim3((unsigned)a.nfb, S), dim3(256), 0, s, a);
+    const dim3 g((unsigned)a.nfb, S);
+    switch (a.det) {
+    case 0: hipLaunchKernelGGL((k_disp_reduce<N, 0>), g, dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_disp_reduce<N, 1>), g, dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_disp_reduce<N, 2>), g, dim3(256), 0, s, a); break;
+    }
     return LAUNCH_OK();
 }
 

@@ -1,21 +1,37 @@
-// The power-of-two team FFT as a device routine (LDS exchanges between
-// stages, block-wide barriers: every team of a block must call it together).
+// The power-of-two team FFT as a device routine. Stages exchange data through
+// LDS; between a stage's writes and the next stage's reads the team must
+// synchronise:
+//   WAVE = false  block-wide barrier (every team of the block calls together);
+//   WAVE = true   the team lies inside one wavefront (N/PT <= 64): LDS
+//                 operations of a wavefront are performed in order, so only
+//                 the compiler's reordering has to be stopped.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "fft_core.hpp"
 
+template <bool WAVE>
+__device__ __forceinline__ void team_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        __syncthreads();
+    }
+}
+
 // ---- team FFT over the stages of TeamFftPlan<N, PT> --------------------------
-template <int N, int PT, int STAGE>
+template <int N, int PT, int STAGE, bool WAVE = false>
 __device__ __forceinline__ void team_fft_from(cpx *v, cpx *lds, const cpx *tw, int j, int Ns) {
     using P = TeamFftPlan<N, PT>;
+    static_assert(!WAVE || N / PT <= 64, "a wave-synchronised team must fit one wavefront");
     if constexpr (STAGE < P::S) {
         constexpr bool last = (STAGE == P::NSTAGE - 1);
         team_stage<N, PT, PT>(v, lds, tw, j, Ns, last);
         if constexpr (!last) {
-            __syncthreads();
+            team_sync<WAVE>();
             team_load<N, PT>(v, lds, j);
-            __syncthreads();
-            team_fft_from<N, PT, STAGE + 1>(v, lds, tw, j, Ns * PT);
+            team_sync<WAVE>();
+            team_fft_from<N, PT, STAGE + 1, WAVE>(v, lds, tw, j, Ns * PT);
         }
     } else if constexpr (P::RL > 1) {
         team_stage<N, PT, P::RL>(v, lds, tw, j, Ns, true);
@@ -24,6 +40,5 @@ __device__ __forceinline__ void team_fft_from(cpx *v, cpx *lds, const cpx *tw, i
 
 template <int N, int PT>
 __device__ __forceinline__ void team_fft(cpx *v, cpx *lds, const cpx *tw, int j) {
-    team_fft_from<N, PT, 0>(v, lds, tw, j, 1);
+    team_fft_from<N, PT, 0, false>(v, lds, tw, j, 1);
 }
-

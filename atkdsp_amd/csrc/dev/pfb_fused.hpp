@@ -136,14 +136,17 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
         __syncthreads();
         // ---- FFT, rotation, cal: team tm transforms hops tm, tm+TEAMS, ... ----
         const int tm = tid / S::NT, j = tid % S::NT;
+        // a team inside one wavefront synchronises its stages wave-locally;
+        // the rows it reads were published by the block barrier above
+        constexpr bool WV = S::NT <= 64;
 #pragma unroll 1
         for (int rnd = 0; rnd < G / S::TEAMS; ++rnd) {
             const int p = tm + rnd * S::TEAMS;
             cpx *row = lds + p * ROW;
             cpx v[S::PT];
             team_load<M, S::PT>(v, row, j);
-            __syncthreads();
-            team_fft_from<M, S::PT, 0>(v, row, twl, j, 1);
+            team_sync<WV>();
+            team_fft_from<M, S::PT, 0, WV>(v, row, twl, j, 1);
             // at 2x the rotation is (-1)^k on hops whose b+1 is odd
             // (reference src/channelizer.c:275-276 with shift = M/2)
             const bool flip = (OS == 2) && (((i0 + p) & 1) == (flip_base ? 1 : 0));
@@ -155,8 +158,9 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
                 if (cal) y = cmul(y, cal[k]);
                 row[k] = y;
             }
-            __syncthreads();
+            if constexpr (!WV) __syncthreads();
         }
+        if constexpr (WV) __syncthreads();       // every row done before the store phase
         // ---- channel-major store: LPC lanes per channel, consecutive hops ----
         {
             constexpr int CH_PER_PASS = 256 / S::LPC;
