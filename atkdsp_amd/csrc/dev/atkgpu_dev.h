@@ -37,6 +37,8 @@ int         atkgpu_event_create(void **e);
 void        atkgpu_event_destroy(void *e);
 int         atkgpu_event_record(void *e, void *s);
 int         atkgpu_event_elapsed_ms(void *e0, void *e1, float *ms);
+int         atkgpu_event_sync(void *e);
+int         atkgpu_d2h_sync(void *dst, const void *src, size_t bytes);
 
 /* ---- unpack --------------------------------------------------------------- */
 /* number of double pairs in the mean partial buffer */
@@ -128,6 +130,36 @@ int atkgpu_chain_post(void *s, const float *line_db, unsigned S, unsigned N, uns
                       double out_lo, double out_step);
 /* copy keep samples from src + s*spitch to dst + s*spitch for every stream */
 int atkgpu_chain_keep(void *s, const void *src, size_t spitch, void *dst, size_t keep, unsigned S);
+
+/* ---- ragged batches of independent streams (serve.hip) ----
+ * Job arrays live in device-visible memory (mapped pinned host memory is
+ * fine); job j owns blocks [blk0, blk0 + nblk) of the launch. */
+typedef struct {               /* layout of PfbJob in pfb_fused.hpp */
+    const void *s;             /* stream rows (cf32), hop i's window at s + i*hop */
+    void *out;                 /* channel k, hop i at out[k*opitch + i] */
+    const float *taps;         /* kos*hop floats, zero-padded */
+    long long b0;              /* global hop index of hop 0 */
+    int nvalid, m, opitch, blk0, nblk, pad_;
+} atkgpu_pfb_job;
+typedef struct {
+    const void *raw;           /* device-visible raw bytes */
+    void *dst;                 /* cf32 out */
+    float off_re, off_im;      /* offset folded into the constants */
+    int n, blk0, nblk, pad_;
+} atkgpu_unpack_job;
+typedef struct {
+    const void *src;
+    void *dst;
+    int n, blk0, nblk, pad_;
+} atkgpu_copy_job;
+#define ATKGPU_RAGGED_SPB 2048     /* samples a block of the ragged unpack/copy handles */
+int atkgpu_pfb_group(unsigned M, unsigned os);      /* hops per block of the fused kernel */
+int atkgpu_pfb_fused_jobs(void *s, const void *jobs, int njobs, int nblocks, unsigned M, unsigned os,
+                          unsigned kos, const void *tw);
+int atkgpu_unpack_jobs(void *s, int fmt, const void *jobs, int njobs, int nblocks);
+int atkgpu_copy_jobs(void *s, const void *jobs, int njobs, int nblocks);
+/* pinned host memory the device can read (and its device address) */
+void *atkgpu_hmalloc_mapped(size_t bytes, void **dev_ptr);
 
 /* W1 / W2 over a device [nel][nchan][m] block */
 int atkgpu_pfb_cov(void *s, const void *y, unsigned nel, unsigned nchan, size_t m,

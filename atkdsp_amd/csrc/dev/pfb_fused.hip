@@ -39,6 +39,8 @@ extern "C" int atkgpu_pfb_fused(void *s, const void *sbuf, size_t spitch, size_t
     const int G = group_size(M, os);
     if (!G) return -1;
     PfbFusedArgs a;
+    a.jobs = nullptr;
+    a.njobs = 0;
     a.s = (const cpx *)sbuf;
     a.spitch = (long long)spitch;
     a.nvalid = (long long)nvalid;
@@ -68,4 +70,34 @@ extern "C" int atkgpu_pfb_fused(void *s, const void *sbuf, size_t spitch, size_t
     case 2048: return pfb_fused_m2048(a, (int)os, (int)kos, grid, st);
     }
     return -1;
+}
+
+static int dispatch_m(const PfbFusedArgs &a, unsigned M, int os, int kos, unsigned grid, hipStream_t st) {
+    switch (M) {
+    case 64: return pfb_fused_m64(a, os, kos, grid, st);
+    case 128: return pfb_fused_m128(a, os, kos, grid, st);
+    case 256: return pfb_fused_m256(a, os, kos, grid, st);
+    case 512: return pfb_fused_m512(a, os, kos, grid, st);
+    case 1024: return pfb_fused_m1024(a, os, kos, grid, st);
+    case 2048: return pfb_fused_m2048(a, os, kos, grid, st);
+    }
+    return -1;
+}
+
+extern "C" int atkgpu_pfb_group(unsigned M, unsigned os) { return group_size(M, os); }
+
+// A ragged batch: jobs (device-visible array, job j owns blocks
+// [blk0, blk0 + nblk), one group of G hops each) of one (M, os, kos) class.
+extern "C" int atkgpu_pfb_fused_jobs(void *s, const void *jobs, int njobs, int nblocks, unsigned M, unsigned os,
+                                     unsigned kos, const void *tw) {
+    if (!njobs || !nblocks) return 0;
+    if (!group_size(M, os)) return -1;
+    PfbFusedArgs a = {};
+    a.jobs = (const PfbJob *)jobs;
+    a.njobs = njobs;
+    a.tw = (const cpx *)tw;
+    a.nblocks = nblocks;
+    a.groups_per_block = 1;
+    const unsigned grid = (unsigned)((nblocks + 7) / 8 * 8);
+    return dispatch_m(a, M, (int)os, (int)kos, grid, (hipStream_t)s);
 }

@@ -43,7 +43,20 @@ struct PfbShape {
     static_assert(NT <= 256 && G % TEAMS == 0, "shape");
 };
 
+// One element of a ragged batch: its own stream, output, taps and counters.
+// Blocks [blk0, blk0 + nblk) of the launch work on it, one group each.
+struct PfbJob {
+    const cpx *s;
+    cpx *out;              // channel k, hop i at out[k*opitch + i]
+    const float *taps;
+    long long b0;
+    int nvalid, m, opitch, blk0, nblk;
+    int pad_;
+};
+
 struct PfbFusedArgs {
+    const PfbJob *jobs;    // non-null: a ragged batch of njobs elements
+    int njobs;
     const cpx *s;          // stream rows, element e at s + e*spitch
     long long spitch;
     long long nvalid;      // valid samples in each element's stream
@@ -72,25 +85,53 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
     const int nb8 = (int)gridDim.x / 8;
     const int lb = (int)(blockIdx.x % 8) * nb8 + (int)(blockIdx.x / 8);
     if (lb >= a.nblocks) return;
-    const int e = lb / a.nblk_per_el;
-    const int tile = lb % a.nblk_per_el;
-    // per-element bases in 64-bit once; every offset below fits 32 bits
-    const cpx *s = a.s + (long long)e * a.spitch;
-    cpx *oute = a.out + (long long)e * M * a.opitch + a.ocol;
-    const int nlast = (int)a.nvalid - 1;
-    const int m = (int)a.m;
-    const int opitch = (int)a.opitch;
+    int e, tile, nlast, m, opitch, gpb;
+    const cpx *s;
+    cpx *oute;
+    const float *taps;
+    long long b0;
+    if (a.jobs) {
+        // ragged batch: find the job whose block range holds lb
+        int lo = 0, hi = a.njobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.jobs[mid].blk0 <= lb) lo = mid; else hi = mid - 1;
+        }
+        const PfbJob &jb = a.jobs[lo];
+        e = 0;
+        tile = lb - jb.blk0;
+        s = jb.s;
+        oute = jb.out;
+        taps = jb.taps;
+        b0 = jb.b0;
+        nlast = jb.nvalid - 1;
+        m = jb.m;
+        opitch = jb.opitch;
+        gpb = 1;
+    } else {
+        e = lb / a.nblk_per_el;
+        tile = lb % a.nblk_per_el;
+        // per-element bases in 64-bit once; every offset below fits 32 bits
+        s = a.s + (long long)e * a.spitch;
+        oute = a.out + (long long)e * M * a.opitch + a.ocol;
+        taps = a.taps;
+        b0 = a.b0;
+        nlast = (int)a.nvalid - 1;
+        m = (int)a.m;
+        opitch = (int)a.opitch;
+        gpb = a.groups_per_block;
+    }
     const int tid = threadIdx.x;
     const int sub = tid / S::HOPC;
     const int ucol = tid % S::HOPC;
     for (int i = tid; i < M; i += 256) twl[i] = a.tw[i];
     const cpx *cal = (a.cal && e < a.cal_nel) ? a.cal + (long long)e * M : nullptr;
-    const bool flip_base = (a.b0 & 1) != 0;     // parity of the hop counter at hop 0
+    const bool flip_base = (b0 & 1) != 0;     // parity of the hop counter at hop 0
 
     // The loops over column passes and FFT rounds stay rolled: the unrolled
     // fold is long, and the kernel must fit the instruction cache.
-    for (int grp = 0; grp < a.groups_per_block; ++grp) {
-        const int i0 = (tile * a.groups_per_block + grp) * G;
+    for (int grp = 0; grp < gpb; ++grp) {
+        const int i0 = (tile * gpb + grp) * G;
         if (i0 >= m) break;
         // ---- fold ----
 #pragma unroll 1
@@ -118,7 +159,7 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
 #pragma unroll
                 for (int jj = 0; jj < GP + TH - 1; ++jj) x[jj] = s[min(base + (t0 + jj) * HOP, nlast)];
 #pragma unroll
-                for (int t = 0; t < TH; ++t) h[t] = a.taps[(t0 + t) * HOP + u];
+                for (int t = 0; t < TH; ++t) h[t] = taps[(t0 + t) * HOP + u];
 #pragma unroll
                 for (int p = 0; p < GP; ++p)
 #pragma unroll

@@ -116,3 +116,14 @@ extern "C" int atkgpu_event_elapsed_ms(void *e0, void *e1, float *ms) {
     if (hipEventSynchronize((hipEvent_t)e1) != hipSuccess) return -4;
     return hipEventElapsedTime(ms, (hipEvent_t)e0, (hipEvent_t)e1) == hipSuccess ? 0 : -4;
 }
+
+extern "C" int atkgpu_event_sync(void *e) {
+    return hipEventSynchronize((hipEvent_t)e) == hipSuccess ? 0 : -4;
+}
+
+// a synchronous device-to-host copy on the null stream (reads of results
+// already complete on their own stream)
+extern "C" int atkgpu_d2h_sync(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return 0;
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -4;
+}

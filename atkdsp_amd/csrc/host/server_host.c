@@ -1,0 +1,607 @@
+/* The serving loop (include/atkgpu.h): many independent receiver streams,
+ * each with its own format and filterbank shape, sending chunks of any size.
+ *
+ *   admission   a stream's device state (two history rows, taps, an output
+ *               ring) is allocated when it is added and charged against a
+ *               fixed HBM budget; a stream that does not fit is refused.
+ *   submit      copies the chunk's bytes into a pinned staging ring that the
+ *               device reads directly (mapped), and returns a ticket. It
+ *               allocates nothing and makes no HIP call.
+ *   dispatcher  a thread forms batches continuously: at most one chunk per
+ *               stream (each stream's chunks stay in order), taken round-robin
+ *               over the streams that have work, while earlier batches still
+ *               run. A batch is one ragged unpack launch per format, one fused
+ *               filterbank launch per (M, os, taps) class and one history
+ *               launch, all on one HIP stream, so a stream's chunks execute in
+ *               order.
+ *   completion  a thread waits on each batch's event in order, stamps the
+ *               host monotonic clock and records each chunk's latency
+ *               (completion minus submission), then frees staging space.
+ *
+ * Every chunk of a stream is computed by the same kernels with the same
+ * per-hop arithmetic whatever else is in its batch, so a stream's output is
+ * identical alone and in the mix. The offset is fixed per stream (no
+ * per-chunk re-estimation), which keeps the output independent of chunking.
+ */
+#define _GNU_SOURCE
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "internal.h"
+#include "../../../include/atkgpu.h"
+
+int atkgpu_pfb_prototype(double *h, unsigned nchan, unsigned K, double atten_db);
+
+#define NINFLIGHT 4
+#define QCAP 256               /* pending chunks per stream */
+#define LAT_CAP (1u << 22)
+
+typedef struct {
+    long long ticket;
+    size_t off;                /* staging byte offset (monotonic) */
+    size_t nbytes, n;
+    double t_submit;
+} chunk_t;
+
+typedef struct {
+    int used, removing;
+    atkgpu_stream_cfg cfg;
+    unsigned M, os, K, hop, ov, keep;
+    int kos, bps;
+    size_t m_max, row_len, bytes;
+    atkdsp_cf32 *rows[2];
+    int cur;
+    float *taps;
+    const void *tw;
+    atkdsp_cf32 *out;
+    unsigned lc;
+    long long b;
+    long long submitted, dispatched, completed;
+    size_t *slot_m;            /* hops of the result held in each output slot */
+    chunk_t q[QCAP];
+    unsigned qh, qn;           /* queue head, count */
+} sstream;
+
+/* one chunk in a batch, with everything the launch needs, taken from the
+ * stream's state when the dispatcher admitted the chunk */
+typedef struct {
+    int sid, fmt;
+    chunk_t c;
+    unsigned M, os, kos, ov, keep, lc;
+    size_t m;
+    long long b0;
+    atkdsp_cf32 *row, *next, *out;
+    const float *taps;
+    const void *tw;
+    float off_re, off_im;
+} bitem;
+
+typedef struct {
+    void *event;
+    int n;
+    bitem *items;              /* max_batch */
+    void *desc_h, *desc_d;     /* mapped descriptor buffer */
+} batch_t;
+
+typedef struct {
+    size_t off, nbytes;
+    int released;
+} stage_rec;
+
+struct atkgpu_server {
+    size_t budget, used_bytes;
+    unsigned max_streams, max_batch;
+    sstream *st;
+    unsigned char *stage_h, *stage_d;
+    size_t stage_cap, stage_head, stage_tail;    /* monotonic byte counters */
+    stage_rec *recs;
+    size_t rec_cap, rec_head, rec_tail;
+    void *stream;
+    atkgpu_fftplan *plans[6];                    /* twiddles for M = 64..2048 */
+    batch_t batch[NINFLIGHT];
+    unsigned b_submit, b_done;                   /* batches launched / completed */
+    unsigned rr;                                 /* round-robin start */
+    pthread_mutex_t mu;
+    pthread_cond_t cv_work, cv_launch, cv_done, cv_space;   /* dispatcher, completion, waiters, submitters */
+    pthread_t th_disp, th_comp;
+    int running, failed;
+    double *lat, *lat_sort;
+    size_t nlat;
+    long long chunks, samples, nbatches;
+    double t_first, t_last;
+};
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static int plan_index(unsigned M) {
+    for (int i = 0; i < 6; ++i)
+        if (M == (64u << i)) return i;
+    return -1;
+}
+
+/* descriptor buffer layout for one batch: unpack jobs, pfb jobs, copy jobs */
+static size_t desc_bytes(unsigned max_batch) {
+    return (size_t)max_batch * (sizeof(atkgpu_unpack_job) + sizeof(atkgpu_pfb_job) + sizeof(atkgpu_copy_job)) + 256;
+}
+
+static void *disp_main(void *arg);
+static void *comp_main(void *arg);
+
+atkgpu_server *atkgpu_server_create(size_t hbm_budget, unsigned max_streams, unsigned max_batch, size_t staging) {
+    if (max_streams < 1 || max_batch < 1 || staging < ((size_t)1 << 20)) return NULL;
+    if (atkgpu_init() != 0) return NULL;
+    atkgpu_server *v = (atkgpu_server *)calloc(1, sizeof *v);
+    if (!v) return NULL;
+    v->budget = hbm_budget;
+    v->max_streams = max_streams;
+    v->max_batch = max_batch;
+    v->st = (sstream *)calloc(max_streams, sizeof(sstream));
+    v->stage_cap = staging & ~(size_t)63;
+    v->stage_h = (unsigned char *)atkgpu_hmalloc_mapped(staging, (void **)&v->stage_d);
+    v->rec_cap = (size_t)max_streams * QCAP + 16;
+    v->recs = (stage_rec *)calloc(v->rec_cap, sizeof(stage_rec));
+    v->lat = (double *)malloc(LAT_CAP * sizeof(double));
+    v->lat_sort = (double *)malloc(LAT_CAP * sizeof(double));
+    int ok = v->st && v->stage_h && v->recs && v->lat && v->lat_sort && !atkgpu_stream_create(&v->stream);
+    for (int i = 0; ok && i < 6; ++i) ok = (v->plans[i] = atkgpu_fftplan_create(64u << i)) != NULL;
+    for (int i = 0; ok && i < NINFLIGHT; ++i) {
+        batch_t *b = &v->batch[i];
+        ok = !atkgpu_event_create(&b->event) && (b->items = (bitem *)calloc(max_batch, sizeof(bitem))) &&
+             (b->desc_h = atkgpu_hmalloc_mapped(desc_bytes(max_batch), &b->desc_d));
+    }
+    if (!ok) { atkgpu_server_destroy(v); return NULL; }
+    pthread_mutex_init(&v->mu, NULL);
+    pthread_cond_init(&v->cv_work, NULL);
+    pthread_cond_init(&v->cv_launch, NULL);
+    pthread_cond_init(&v->cv_done, NULL);
+    pthread_cond_init(&v->cv_space, NULL);
+    v->running = 1;
+    pthread_create(&v->th_disp, NULL, disp_main, v);
+    pthread_create(&v->th_comp, NULL, comp_main, v);
+    return v;
+}
+
+static void stream_free(sstream *s) {
+    atkgpu_dfree(s->rows[0]);
+    atkgpu_dfree(s->rows[1]);
+    atkgpu_dfree(s->taps);
+    atkgpu_dfree(s->out);
+    free(s->slot_m);
+    memset(s, 0, sizeof *s);
+}
+
+void atkgpu_server_destroy(atkgpu_server *v) {
+    if (!v) return;
+    if (v->running) {
+        atkgpu_server_drain(v);
+        pthread_mutex_lock(&v->mu);
+        v->running = 0;
+        pthread_cond_broadcast(&v->cv_work);
+        pthread_cond_broadcast(&v->cv_launch);
+        pthread_cond_broadcast(&v->cv_done);
+        pthread_cond_broadcast(&v->cv_space);
+        pthread_mutex_unlock(&v->mu);
+        pthread_join(v->th_disp, NULL);
+        pthread_join(v->th_comp, NULL);
+    }
+    if (v->stream) atkgpu_sync(v->stream);
+    for (unsigned i = 0; v->st && i < v->max_streams; ++i)
+        if (v->st[i].used) stream_free(&v->st[i]);
+    for (int i = 0; i < NINFLIGHT; ++i) {
+        atkgpu_event_destroy(v->batch[i].event);
+        free(v->batch[i].items);
+        atkgpu_hfree(v->batch[i].desc_h);
+    }
+    for (int i = 0; i < 6; ++i) atkgpu_fftplan_destroy(v->plans[i]);
+    if (v->stream) atkgpu_stream_destroy(v->stream);
+    atkgpu_hfree(v->stage_h);
+    free(v->recs);
+    free(v->lat);
+    free(v->lat_sort);
+    free(v->st);
+    free(v);
+}
+
+int atkgpu_server_add_stream(atkgpu_server *v, const atkgpu_stream_cfg *cfg) {
+    if (!v || !cfg) return ATKDSP_E_ARG;
+    const int bps = atkdsp_bytes_per_sample(cfg->fmt);
+    const int kos = atkgpu_pfb_fused_kos(cfg->nchan, cfg->oversample, cfg->taps_per_poly);
+    if (!bps || !kos || cfg->taps_per_poly < 2 || cfg->chunk_max < 1 || cfg->out_slots < 1 || !(cfg->fs > 0))
+        return ATKDSP_E_ARG;
+    const unsigned M = cfg->nchan, os = cfg->oversample, K = cfg->taps_per_poly;
+    const unsigned hop = M / os, ov = M * K - hop;
+    const unsigned keep = ov + hop;
+    const size_t m_max = (cfg->chunk_max + hop) / hop + 1;
+    const size_t row_len = (size_t)keep + cfg->chunk_max;
+    const size_t bytes = 2 * row_len * sizeof(atkdsp_cf32) + (size_t)kos * hop * sizeof(float) +
+                         (size_t)cfg->out_slots * M * m_max * sizeof(atkdsp_cf32);
+    pthread_mutex_lock(&v->mu);
+    int sid = -1;
+    for (unsigned i = 0; i < v->max_streams; ++i)
+        if (!v->st[i].used) { sid = (int)i; break; }
+    if (sid < 0 || v->used_bytes + bytes > v->budget) {
+        pthread_mutex_unlock(&v->mu);
+        return ATKDSP_E_NOMEM;                      /* refused: no slot or over the HBM budget */
+    }
+    sstream *s = &v->st[sid];
+    memset(s, 0, sizeof *s);
+    s->used = 1;                                    /* reserve the slot while allocating */
+    v->used_bytes += bytes;
+    pthread_mutex_unlock(&v->mu);
+
+    s->cfg = *cfg;
+    s->M = M; s->os = os; s->K = K; s->hop = hop; s->ov = ov; s->keep = keep;
+    s->kos = kos; s->bps = bps; s->m_max = m_max; s->row_len = row_len; s->bytes = bytes;
+    s->tw = atkgpu_fftplan_twiddles(v->plans[plan_index(M)]);
+    s->rows[0] = (atkdsp_cf32 *)atkgpu_dmalloc(row_len * sizeof(atkdsp_cf32));
+    s->rows[1] = (atkdsp_cf32 *)atkgpu_dmalloc(row_len * sizeof(atkdsp_cf32));
+    s->taps = (float *)atkgpu_dmalloc((size_t)kos * hop * sizeof(float));
+    s->out = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)cfg->out_slots * M * m_max * sizeof(atkdsp_cf32));
+    s->slot_m = (size_t *)calloc(cfg->out_slots, sizeof(size_t));
+    double *h = (double *)malloc((size_t)M * K * sizeof(double));
+    float *hf = (float *)calloc((size_t)kos * hop, sizeof(float));
+    int rc = (!s->rows[0] || !s->rows[1] || !s->taps || !s->out || !s->slot_m || !h || !hf) ? ATKDSP_E_NOMEM : 0;
+    if (!rc) rc = atkgpu_pfb_prototype(h, M, K, cfg->atten_db);
+    if (!rc) {
+        for (size_t q = 0; q < (size_t)M * K; ++q) hf[q] = (float)h[q];
+        /* history rows start at zero (the reference callocs its history) */
+        rc = atkgpu_h2d(v->stream, s->taps, hf, (size_t)kos * hop * sizeof(float));
+        if (!rc) rc = atkgpu_fill_u32(v->stream, s->rows[0], 0, row_len * 2);
+        if (!rc) rc = atkgpu_sync(v->stream);
+    }
+    free(h);
+    free(hf);
+    if (rc) {
+        pthread_mutex_lock(&v->mu);
+        v->used_bytes -= bytes;
+        stream_free(s);
+        pthread_mutex_unlock(&v->mu);
+        return rc < 0 ? rc : ATKDSP_E_NOMEM;
+    }
+    return sid;
+}
+
+int atkgpu_server_remove_stream(atkgpu_server *v, int sid) {
+    if (!v || sid < 0 || (unsigned)sid >= v->max_streams) return ATKDSP_E_ARG;
+    pthread_mutex_lock(&v->mu);
+    sstream *s = &v->st[sid];
+    if (!s->used) { pthread_mutex_unlock(&v->mu); return ATKDSP_E_ARG; }
+    s->removing = 1;
+    while (s->completed < s->submitted && !v->failed) pthread_cond_wait(&v->cv_done, &v->mu);
+    v->used_bytes -= s->bytes;
+    stream_free(s);
+    pthread_mutex_unlock(&v->mu);
+    return ATKDSP_OK;
+}
+
+/* staging space for nbytes at the ring's head (contiguous; a wrap skips the tail end) */
+static int stage_alloc(atkgpu_server *v, size_t nbytes, size_t *off) {
+    /* every chunk starts 64-byte aligned, so the device's vector loads of any
+     * format are naturally aligned */
+    nbytes = (nbytes + 63) & ~(size_t)63;
+    size_t head = (v->stage_head + 63) & ~(size_t)63;
+    const size_t pos = head % v->stage_cap;
+    if (pos + nbytes > v->stage_cap) head += v->stage_cap - pos;       /* skip to the ring start */
+    if (head + nbytes - v->stage_tail > v->stage_cap) return 0;
+    if (v->rec_head - v->rec_tail >= v->rec_cap) return 0;
+    *off = head;
+    v->stage_head = head + nbytes;
+    stage_rec *r = &v->recs[v->rec_head % v->rec_cap];
+    r->off = head;
+    r->nbytes = nbytes;
+    r->released = 0;
+    v->rec_head++;
+    return 1;
+}
+
+static void stage_release(atkgpu_server *v, size_t off) {
+    for (size_t i = v->rec_tail; i < v->rec_head; ++i) {
+        stage_rec *r = &v->recs[i % v->rec_cap];
+        if (r->off == off && !r->released) { r->released = 1; break; }
+    }
+    while (v->rec_tail < v->rec_head && v->recs[v->rec_tail % v->rec_cap].released) {
+        const stage_rec *r = &v->recs[v->rec_tail % v->rec_cap];
+        v->stage_tail = r->off + r->nbytes;
+        v->rec_tail++;
+    }
+    if (v->rec_tail == v->rec_head) v->stage_tail = v->stage_head;
+}
+
+long long atkgpu_server_submit(atkgpu_server *v, int sid, const void *raw, size_t nbytes) {
+    if (!v || !raw || sid < 0 || (unsigned)sid >= v->max_streams) return ATKDSP_E_ARG;
+    pthread_mutex_lock(&v->mu);
+    sstream *s = &v->st[sid];
+    if (!s->used || s->removing || v->failed) { pthread_mutex_unlock(&v->mu); return v->failed ? ATKDSP_E_FFT : ATKDSP_E_ARG; }
+    const size_t n = nbytes / (size_t)s->bps;
+    if (n > s->cfg.chunk_max || n * (size_t)s->bps + 64 > v->stage_cap) {
+        pthread_mutex_unlock(&v->mu);
+        return ATKDSP_E_ARG;
+    }
+    const size_t nb = n * (size_t)s->bps;
+    size_t off = 0;
+    /* backpressure: wait for a queue entry and staging space */
+    while (v->running && !v->failed && (s->qn >= QCAP || !stage_alloc(v, nb ? nb : 1, &off)))
+        pthread_cond_wait(&v->cv_space, &v->mu);
+    if (!v->running || v->failed) { pthread_mutex_unlock(&v->mu); return ATKDSP_E_FFT; }
+    const double t = now_s();
+    /* the bytes go in before the chunk is published to the dispatcher */
+    if (nb) memcpy(v->stage_h + (off % v->stage_cap), raw, nb);
+    const long long ticket = s->submitted++;
+    chunk_t *c = &s->q[(s->qh + s->qn) % QCAP];
+    c->ticket = ticket;
+    c->off = off;
+    c->nbytes = nb;
+    c->n = n;
+    c->t_submit = t;
+    s->qn++;
+    if (!v->chunks && !v->nlat) v->t_first = t;
+    pthread_cond_signal(&v->cv_work);
+    pthread_mutex_unlock(&v->mu);
+    return ticket;
+}
+
+/* Launch one batch (the mutex is not held): ragged unpack per format, fused
+ * filterbank per (M, os, kos) class, then every stream's kept prefix. */
+static int launch_batch(atkgpu_server *v, batch_t *bt) {
+    const int n = bt->n;
+    atkgpu_unpack_job *uj = (atkgpu_unpack_job *)bt->desc_h;
+    atkgpu_pfb_job *pj = (atkgpu_pfb_job *)(uj + v->max_batch);
+    atkgpu_copy_job *cj = (atkgpu_copy_job *)(pj + v->max_batch);
+    char *dbase = (char *)bt->desc_d, *hbase = (char *)bt->desc_h;
+#define DEV(p) ((void *)(dbase + ((char *)(p) - hbase)))
+    int rc = 0;
+    /* unpack, one launch per format present */
+    int nu = 0;
+    for (int fmt = 0; fmt <= 4 && !rc; ++fmt) {
+        const int u0 = nu;
+        int blk = 0;
+        for (int i = 0; i < n; ++i) {
+            const bitem *it = &bt->items[i];
+            if (it->fmt != fmt || it->c.n == 0) continue;
+            atkgpu_unpack_job *j = &uj[nu++];
+            j->raw = v->stage_d + (it->c.off % v->stage_cap);
+            j->dst = it->row + it->keep;
+            j->off_re = it->off_re;
+            j->off_im = it->off_im;
+            j->n = (int)it->c.n;
+            j->blk0 = blk;
+            j->nblk = (j->n + ATKGPU_RAGGED_SPB - 1) / ATKGPU_RAGGED_SPB;
+            blk += j->nblk;
+        }
+        if (nu > u0) rc = atkgpu_unpack_jobs(v->stream, fmt, DEV(&uj[u0]), nu - u0, blk);
+    }
+    /* the filterbank, one launch per shape class present */
+    int np = 0;
+    unsigned char done[4096];
+    memset(done, 0, sizeof done);
+    for (int i = 0; i < n && !rc; ++i) {
+        if (done[i]) continue;
+        const bitem *a = &bt->items[i];
+        const int p0 = np;
+        int blk = 0;
+        const int G = atkgpu_pfb_group(a->M, a->os);
+        for (int k = i; k < n; ++k) {
+            const bitem *it = &bt->items[k];
+            if (done[k] || it->M != a->M || it->os != a->os || it->kos != a->kos) continue;
+            done[k] = 1;
+            if (!it->m) continue;
+            atkgpu_pfb_job *j = &pj[np++];
+            j->s = it->row + (it->keep - it->ov - it->lc);
+            j->out = it->out;
+            j->taps = it->taps;
+            j->b0 = it->b0;
+            j->nvalid = (int)(it->ov + it->lc + it->c.n);
+            j->m = (int)it->m;
+            j->opitch = (int)it->m;
+            j->blk0 = blk;
+            j->nblk = (int)((it->m + G - 1) / G);
+            j->pad_ = 0;
+            blk += j->nblk;
+        }
+        if (np > p0) rc = atkgpu_pfb_fused_jobs(v->stream, DEV(&pj[p0]), np - p0, blk, a->M, a->os, a->kos, a->tw);
+    }
+    /* kept prefix: the last keep samples of [row, row + keep + n) to the next row */
+    int nc = 0, blk = 0;
+    for (int i = 0; i < n; ++i) {
+        const bitem *it = &bt->items[i];
+        atkgpu_copy_job *j = &cj[nc++];
+        j->src = it->row + it->c.n;
+        j->dst = it->next;
+        j->n = (int)it->keep;
+        j->blk0 = blk;
+        j->nblk = (j->n + ATKGPU_RAGGED_SPB - 1) / ATKGPU_RAGGED_SPB;
+        blk += j->nblk;
+    }
+    if (!rc && nc) rc = atkgpu_copy_jobs(v->stream, DEV(cj), nc, blk);
+#undef DEV
+    if (!rc) rc = atkgpu_event_record(bt->event, v->stream);
+    return rc;
+}
+
+/* take up to max_batch chunks, one per stream, round-robin; mutex held */
+static int form_batch(atkgpu_server *v, batch_t *bt) {
+    int n = 0;
+    const unsigned S = v->max_streams;
+    for (unsigned t = 0; t < S && n < (int)v->max_batch && n < 4096; ++t) {
+        const unsigned sid = (v->rr + t) % S;
+        sstream *s = &v->st[sid];
+        if (!s->used || !s->qn) continue;
+        const chunk_t c = s->q[s->qh];
+        s->qh = (s->qh + 1) % QCAP;
+        s->qn--;
+        bitem *it = &bt->items[n++];
+        it->sid = (int)sid;
+        it->fmt = s->cfg.fmt;
+        it->c = c;
+        it->M = s->M;
+        it->os = s->os;
+        it->kos = (unsigned)s->kos;
+        it->ov = s->ov;
+        it->keep = s->keep;
+        it->lc = s->lc;
+        it->b0 = s->b;
+        it->m = ((size_t)s->lc + c.n) / s->hop;
+        it->row = s->rows[s->cur];
+        it->next = s->rows[s->cur ^ 1];
+        const unsigned slot = (unsigned)(c.ticket % s->cfg.out_slots);
+        it->out = s->out + (size_t)slot * s->M * s->m_max;
+        it->taps = s->taps;
+        it->tw = s->tw;
+        it->off_re = s->cfg.off_re;
+        it->off_im = s->cfg.off_im;
+        /* advance the stream */
+        s->slot_m[slot] = it->m;
+        s->lc = (unsigned)(((size_t)s->lc + c.n) - it->m * s->hop);
+        s->b += (long long)it->m;
+        s->cur ^= 1;
+        s->dispatched++;
+    }
+    v->rr = (v->rr + 1) % S;
+    bt->n = n;
+    return n;
+}
+
+static void *disp_main(void *arg) {
+    atkgpu_server *v = (atkgpu_server *)arg;
+    pthread_mutex_lock(&v->mu);
+    for (;;) {
+        /* a free batch slot (at most NINFLIGHT batches in flight) */
+        while (v->running && v->b_submit - v->b_done >= NINFLIGHT) pthread_cond_wait(&v->cv_done, &v->mu);
+        if (!v->running) break;
+        batch_t *bt = &v->batch[v->b_submit % NINFLIGHT];
+        while (v->running && !form_batch(v, bt)) pthread_cond_wait(&v->cv_work, &v->mu);
+        if (!v->running) break;
+        pthread_mutex_unlock(&v->mu);
+        const int rc = launch_batch(v, bt);
+        pthread_mutex_lock(&v->mu);
+        if (rc) {
+            v->failed = 1;
+            pthread_cond_broadcast(&v->cv_done);
+            pthread_cond_broadcast(&v->cv_launch);
+            pthread_cond_broadcast(&v->cv_space);
+            break;
+        }
+        v->b_submit++;
+        v->nbatches++;
+        pthread_cond_signal(&v->cv_launch);        /* wake the completion thread */
+    }
+    pthread_mutex_unlock(&v->mu);
+    return NULL;
+}
+
+static void *comp_main(void *arg) {
+    atkgpu_server *v = (atkgpu_server *)arg;
+    pthread_mutex_lock(&v->mu);
+    for (;;) {
+        while (v->running && v->b_done == v->b_submit && !v->failed) pthread_cond_wait(&v->cv_launch, &v->mu);
+        if (v->b_done == v->b_submit && (!v->running || v->failed)) break;
+        batch_t *bt = &v->batch[v->b_done % NINFLIGHT];
+        pthread_mutex_unlock(&v->mu);
+        const int rc = atkgpu_event_sync(bt->event);
+        const double t = now_s();
+        pthread_mutex_lock(&v->mu);
+        if (rc) { v->failed = 1; pthread_cond_broadcast(&v->cv_done); pthread_cond_broadcast(&v->cv_space); break; }
+        for (int i = 0; i < bt->n; ++i) {
+            const bitem *it = &bt->items[i];
+            sstream *s = &v->st[it->sid];
+            s->completed++;
+            if (v->nlat < LAT_CAP) v->lat[v->nlat++] = (t - it->c.t_submit) * 1e3;
+            v->chunks++;
+            v->samples += (long long)it->c.n;
+            stage_release(v, it->c.off);
+        }
+        v->t_last = t;
+        v->b_done++;
+        pthread_cond_broadcast(&v->cv_done);
+        pthread_cond_broadcast(&v->cv_space);
+    }
+    pthread_mutex_unlock(&v->mu);
+    return NULL;
+}
+
+int atkgpu_server_wait(atkgpu_server *v, int sid, long long ticket) {
+    if (!v || sid < 0 || (unsigned)sid >= v->max_streams) return ATKDSP_E_ARG;
+    pthread_mutex_lock(&v->mu);
+    sstream *s = &v->st[sid];
+    if (!s->used || ticket < 0 || ticket >= s->submitted) { pthread_mutex_unlock(&v->mu); return ATKDSP_E_ARG; }
+    while (s->completed <= ticket && !v->failed) pthread_cond_wait(&v->cv_done, &v->mu);
+    const int rc = v->failed ? ATKDSP_E_FFT : ATKDSP_OK;
+    pthread_mutex_unlock(&v->mu);
+    return rc;
+}
+
+ptrdiff_t atkgpu_server_read(atkgpu_server *v, int sid, long long ticket, atkdsp_cf32 *out, size_t cap) {
+    if (!v || !out || sid < 0 || (unsigned)sid >= v->max_streams) return ATKDSP_E_ARG;
+    int rc = atkgpu_server_wait(v, sid, ticket);
+    if (rc) return rc;
+    pthread_mutex_lock(&v->mu);
+    sstream *s = &v->st[sid];
+    /* the slot still holds this ticket's result until out_slots later chunks run */
+    if (ticket + (long long)s->cfg.out_slots < s->dispatched) { pthread_mutex_unlock(&v->mu); return ATKDSP_E_ARG; }
+    const unsigned slot = (unsigned)(ticket % s->cfg.out_slots);
+    const size_t m = s->slot_m[slot];
+    const atkdsp_cf32 *src = s->out + (size_t)slot * s->M * s->m_max;
+    const unsigned M = s->M;
+    pthread_mutex_unlock(&v->mu);
+    if (cap < (size_t)M * m) return ATKDSP_E_CAP;
+    rc = atkgpu_d2h_sync(out, src, (size_t)M * m * sizeof(atkdsp_cf32));
+    return rc ? ATKDSP_E_FFT : (ptrdiff_t)m;
+}
+
+int atkgpu_server_drain(atkgpu_server *v) {
+    if (!v) return ATKDSP_E_ARG;
+    pthread_mutex_lock(&v->mu);
+    for (;;) {
+        int pending = 0;
+        for (unsigned i = 0; i < v->max_streams; ++i)
+            if (v->st[i].used && v->st[i].completed < v->st[i].submitted) { pending = 1; break; }
+        if (!pending || v->failed || !v->running) break;
+        pthread_cond_wait(&v->cv_done, &v->mu);
+    }
+    const int rc = v->failed ? ATKDSP_E_FFT : ATKDSP_OK;
+    pthread_mutex_unlock(&v->mu);
+    return rc;
+}
+
+static int cmp_d(const void *a, const void *b) {
+    const double x = *(const double *)a, y = *(const double *)b;
+    return x < y ? -1 : x > y;
+}
+
+int atkgpu_server_stats(atkgpu_server *v, atkgpu_serve_stats *out, int reset) {
+    if (!v || !out) return ATKDSP_E_ARG;
+    pthread_mutex_lock(&v->mu);
+    memset(out, 0, sizeof *out);
+    out->chunks = v->chunks;
+    out->samples = v->samples;
+    out->batches = v->nbatches;
+    out->hbm_used = v->used_bytes;
+    out->hbm_budget = v->budget;
+    for (unsigned i = 0; i < v->max_streams; ++i) out->streams += v->st[i].used ? 1u : 0u;
+    out->busy_s = v->nlat ? v->t_last - v->t_first : 0.0;
+    const size_t n = v->nlat;
+    if (n) {
+        memcpy(v->lat_sort, v->lat, n * sizeof(double));
+        qsort(v->lat_sort, n, sizeof(double), cmp_d);
+        double sum = 0;
+        for (size_t i = 0; i < n; ++i) sum += v->lat_sort[i];
+        out->mean_ms = sum / (double)n;
+        out->p50_ms = v->lat_sort[(size_t)(0.50 * (double)(n - 1))];
+        out->p99_ms = v->lat_sort[(size_t)(0.99 * (double)(n - 1))];
+        out->p999_ms = v->lat_sort[(size_t)(0.999 * (double)(n - 1))];
+        out->max_ms = v->lat_sort[n - 1];
+    }
+    if (reset) {
+        v->nlat = 0;
+        v->chunks = 0;
+        v->samples = 0;
+        v->nbatches = 0;
+    }
+    pthread_mutex_unlock(&v->mu);
+    return ATKDSP_OK;
+}

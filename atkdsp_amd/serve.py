@@ -1,0 +1,112 @@
+"""The serving loop (include/atkgpu.h atkgpu_server_*) from Python.
+
+    srv = Server(hbm_budget=32 << 30, max_streams=128)
+    sid = srv.add_stream(fmt="ci16", fs=40e6, nchan=1024, taps_per_poly=16, chunk_max=65536)
+    t = srv.submit(sid, raw_bytes)        # returns a ticket at once
+    y = srv.read(sid, t)                  # [nchan][m] once the chunk is done
+    srv.stats()                           # chunks, samples, p50/p99/p99.9 latency
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import AtkDspError, _Cf32, _check, _fmt, load
+
+
+class _StreamCfg(C.Structure):
+    _fields_ = [("fmt", C.c_int), ("fs", C.c_double), ("nchan", C.c_uint), ("oversample", C.c_uint),
+                ("taps_per_poly", C.c_uint), ("atten_db", C.c_double), ("off_re", C.c_float),
+                ("off_im", C.c_float), ("chunk_max", C.c_size_t), ("out_slots", C.c_uint)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("chunks", C.c_longlong), ("samples", C.c_longlong), ("batches", C.c_longlong),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("p999_ms", C.c_double),
+                ("max_ms", C.c_double), ("mean_ms", C.c_double), ("busy_s", C.c_double),
+                ("hbm_used", C.c_size_t), ("hbm_budget", C.c_size_t), ("streams", C.c_uint)]
+
+
+def _bind(lib) -> None:
+    if getattr(lib, "_serve_bound", False):
+        return
+    P, vp, sz = C.POINTER, C.c_void_p, C.c_size_t
+    sig = {
+        "atkgpu_server_create": (vp, [sz, C.c_uint, C.c_uint, sz]),
+        "atkgpu_server_destroy": (None, [vp]),
+        "atkgpu_server_add_stream": (C.c_int, [vp, P(_StreamCfg)]),
+        "atkgpu_server_remove_stream": (C.c_int, [vp, C.c_int]),
+        "atkgpu_server_submit": (C.c_longlong, [vp, C.c_int, vp, sz]),
+        "atkgpu_server_wait": (C.c_int, [vp, C.c_int, C.c_longlong]),
+        "atkgpu_server_read": (C.c_ssize_t, [vp, C.c_int, C.c_longlong, P(_Cf32), sz]),
+        "atkgpu_server_drain": (C.c_int, [vp]),
+        "atkgpu_server_stats": (C.c_int, [vp, P(_Stats), C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype, f.argtypes = res, args
+    lib._serve_bound = True
+
+
+class Server:
+    def __init__(self, hbm_budget: int = 32 << 30, max_streams: int = 128, max_batch: int = 256,
+                 staging_bytes: int = 256 << 20):
+        self._lib = load()
+        _bind(self._lib)
+        self._h = self._lib.atkgpu_server_create(int(hbm_budget), int(max_streams), int(max_batch),
+                                                  int(staging_bytes))
+        if not self._h:
+            raise AtkDspError("atkgpu_server_create failed (no GPU or out of memory)")
+        self._nchan = {}
+
+    def close(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._lib.atkgpu_server_destroy(h)
+
+    def __del__(self):
+        self.close()
+
+    def add_stream(self, fmt="ci16", fs=40e6, nchan=1024, oversample=2, taps_per_poly=16, atten_db=80.0,
+                   offset: complex = 0j, chunk_max=65536, out_slots=4) -> int:
+        cfg = _StreamCfg(_fmt(fmt), float(fs), int(nchan), int(oversample), int(taps_per_poly), float(atten_db),
+                         float(offset.real), float(offset.imag), int(chunk_max), int(out_slots))
+        sid = self._lib.atkgpu_server_add_stream(self._h, C.byref(cfg))
+        _check(sid, "server_add_stream")
+        self._nchan[sid] = int(nchan)
+        return sid
+
+    def try_add_stream(self, **kw):
+        """add_stream, returning None when admission is refused."""
+        try:
+            return self.add_stream(**kw)
+        except AtkDspError:
+            return None
+
+    def remove_stream(self, sid: int) -> None:
+        _check(self._lib.atkgpu_server_remove_stream(self._h, int(sid)), "server_remove_stream")
+        self._nchan.pop(sid, None)
+
+    def submit(self, sid: int, raw) -> int:
+        b = np.frombuffer(raw, np.uint8) if not isinstance(raw, np.ndarray) else raw.view(np.uint8).ravel()
+        t = self._lib.atkgpu_server_submit(self._h, int(sid), b.ctypes.data, b.size)
+        return _check(int(t), "server_submit")
+
+    def wait(self, sid: int, ticket: int) -> None:
+        _check(self._lib.atkgpu_server_wait(self._h, int(sid), int(ticket)), "server_wait")
+
+    def read(self, sid: int, ticket: int, max_hops: int = 1 << 17) -> np.ndarray:
+        M = self._nchan[sid]
+        out = np.empty(M * max_hops, np.complex64)
+        m = _check(self._lib.atkgpu_server_read(self._h, int(sid), int(ticket),
+                                                out.ctypes.data_as(C.POINTER(_Cf32)), out.size), "server_read")
+        return out[: M * m].reshape(M, m)
+
+    def drain(self) -> None:
+        _check(self._lib.atkgpu_server_drain(self._h), "server_drain")
+
+    def stats(self, reset: bool = False) -> dict:
+        st = _Stats()
+        _check(self._lib.atkgpu_server_stats(self._h, C.byref(st), int(reset)), "server_stats")
+        return {k: getattr(st, k) for k, _ in _Stats._fields_}

@@ -75,6 +75,54 @@ ATKGPU_API int           atkgpu_chain_read_line(atkgpu_chain *c, unsigned s, flo
 ATKGPU_API int           atkgpu_chain_read_mean(atkgpu_chain *c, unsigned s, double *mean2);
 ATKGPU_API long long     atkgpu_chain_frames(const atkgpu_chain *c);   /* frames of the last run */
 
+/* ---- the serving loop: many independent streams, chunks of any size ----------
+ * A stream is admitted with its own format and filterbank shape; its device
+ * state (history, taps, an output ring of out_slots chunk results) is
+ * allocated at admission and charged against the server's HBM budget, and a
+ * stream that does not fit is refused with ATKDSP_E_NOMEM. Chunks are taken
+ * by a dispatcher thread in batches (one chunk per stream per batch, streams
+ * round-robin) while earlier batches run; every stream's chunks execute in
+ * order. The offset (off_re, off_im) is fixed per stream, so the output does
+ * not depend on how the stream is chunked or what else is in a batch.
+ * Latency = completion of the chunk's batch minus its submit call, both on
+ * the host monotonic clock. */
+typedef struct {
+    int fmt;                    /* enum atkdsp_format */
+    double fs;
+    unsigned nchan, oversample, taps_per_poly;  /* nchan 64..2048 (power of 2), os 1 or 2, K*os <= 32 */
+    double atten_db;
+    float off_re, off_im;       /* offset removed at unpack */
+    size_t chunk_max;           /* largest chunk, samples */
+    unsigned out_slots;         /* chunk results kept per stream */
+} atkgpu_stream_cfg;
+
+typedef struct {
+    long long chunks, samples, batches;
+    double p50_ms, p99_ms, p999_ms, max_ms, mean_ms;
+    double busy_s;              /* first submit to last completion */
+    size_t hbm_used, hbm_budget;
+    unsigned streams;
+} atkgpu_serve_stats;
+
+typedef struct atkgpu_server atkgpu_server;
+/* staging_bytes: pinned host ring for chunks in flight (>= 1 MiB) */
+ATKGPU_API atkgpu_server *atkgpu_server_create(size_t hbm_budget_bytes, unsigned max_streams, unsigned max_batch,
+                                               size_t staging_bytes);
+ATKGPU_API void      atkgpu_server_destroy(atkgpu_server *v);
+/* returns the stream id (>= 0), ATKDSP_E_NOMEM (over budget / no slot) or ATKDSP_E_ARG */
+ATKGPU_API int       atkgpu_server_add_stream(atkgpu_server *v, const atkgpu_stream_cfg *cfg);
+/* waits for the stream's chunks, then returns its memory to the budget */
+ATKGPU_API int       atkgpu_server_remove_stream(atkgpu_server *v, int sid);
+/* copies the chunk (whole samples of nbytes) and returns its ticket (>= 0);
+ * blocks while the stream's queue or the staging ring is full */
+ATKGPU_API long long atkgpu_server_submit(atkgpu_server *v, int sid, const void *raw, size_t nbytes);
+ATKGPU_API int       atkgpu_server_wait(atkgpu_server *v, int sid, long long ticket);
+/* the chunk's filterbank output, [nchan][m] channel-major; returns m. Valid
+ * until out_slots later chunks of the stream have been dispatched. */
+ATKGPU_API ptrdiff_t atkgpu_server_read(atkgpu_server *v, int sid, long long ticket, atkdsp_cf32 *out, size_t cap);
+ATKGPU_API int       atkgpu_server_drain(atkgpu_server *v);
+ATKGPU_API int       atkgpu_server_stats(atkgpu_server *v, atkgpu_serve_stats *st, int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,103 @@
+"""The serving loop: independent streams of mixed formats and filterbank
+shapes, chunks of uneven size, continuous batching. Each stream's output
+equals the drop-in filterbank fed the same samples (bit-identical: same
+kernel, same per-hop arithmetic), equals its own output when served alone,
+and admission refuses streams beyond the HBM budget."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FS = 40e6
+SHAPES = [("ci16", 1024, 16, 0j), ("cu8", 256, 12, 0j), ("ci8", 64, 8, 0.01 - 0.02j), ("cf32", 1024, 10, 0j),
+          ("ci16q11", 256, 16, 0.005j), ("cu8", 64, 14, 0j), ("ci16", 2048, 12, 0j), ("cf32", 128, 9, 0j)]
+
+
+def _streams(seed=11, nsamp=150_000):
+    from atkdsp_amd import synth
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, (fmt, M, K, off) in enumerate(SHAPES):
+        f = synth.FMT[fmt] if hasattr(synth, "FMT") else {"cu8": 0, "ci8": 1, "ci16": 2, "ci16q11": 3, "cf32": 4}[fmt]
+        raw = synth.quantize(synth.wideband_iq(50 + i, nsamp, FS), f)
+        cuts = [0]
+        while cuts[-1] < nsamp:
+            cuts.append(min(nsamp, cuts[-1] + int(rng.integers(1, 20_000))))
+        out.append((fmt, f, M, K, off, raw, cuts))
+    return out
+
+
+def _serve(gpu, streams, which):
+    from atkdsp_amd.serve import Server
+    srv = Server(hbm_budget=4 << 30, max_streams=16, max_batch=64, staging_bytes=64 << 20)
+    sids = {i: srv.add_stream(fmt=streams[i][0], fs=FS, nchan=streams[i][2], oversample=2,
+                              taps_per_poly=streams[i][3], offset=streams[i][4], chunk_max=20_000,
+                              out_slots=64) for i in which}
+    tickets = {i: [] for i in which}
+    # interleave the streams' chunks as they would arrive
+    pos = {i: 0 for i in which}
+    active = list(which)
+    while active:
+        for i in list(active):
+            fmt, f, M, K, off, raw, cuts = streams[i]
+            bps = O.BPS[f]
+            c = pos[i]
+            if c + 1 >= len(cuts):
+                active.remove(i)
+                continue
+            a, b = cuts[c], cuts[c + 1]
+            tickets[i].append(srv.submit(sids[i], raw[a * bps:b * bps]))
+            pos[i] = c + 1
+            if len(tickets[i]) % 32 == 0:       # read before the output ring wraps
+                pass
+    outs = {}
+    for i in which:
+        parts = [srv.read(sids[i], t) for t in tickets[i]]
+        outs[i] = np.concatenate([p for p in parts if p.shape[1]], axis=1)
+    st = srv.stats()
+    srv.close()
+    return outs, st
+
+
+def test_serving_matches_dropin_and_alone(gpu):
+    streams = _streams()
+    which = list(range(len(SHAPES)))
+    mix, st = _serve(gpu, streams, which)
+    assert st["chunks"] == sum(len(s[6]) - 1 for s in streams)
+    assert st["p50_ms"] > 0 and st["p99_ms"] >= st["p50_ms"]
+    for i in which:
+        fmt, f, M, K, off, raw, cuts = streams[i]
+        x, _ = O.unpack_dc(raw, f, off)
+        ref = gpu.Pfb(FS, M, 2, K, 80.0).process(x)
+        assert mix[i].shape == ref.shape, i
+        assert np.array_equal(mix[i], ref), f"stream {i} ({fmt}, M={M}, K={K}) differs from the drop-in"
+    alone, _ = _serve(gpu, streams, [0])
+    assert np.array_equal(alone[0], mix[0])
+    alone, _ = _serve(gpu, streams, [6])
+    assert np.array_equal(alone[6], mix[6])
+    # and the FP64 oracle: within the filterbank's 5e-7 of full scale
+    x, _ = O.unpack_dc(streams[1][5], streams[1][1], streams[1][4])
+    yo = O.Pfb(FS, 256, 2, 12, 80.0).process(x)
+    assert O.rel_err(mix[1], yo) < 5e-7
+
+
+def test_admission_against_the_hbm_budget(gpu):
+    from atkdsp_amd.serve import Server
+    srv = Server(hbm_budget=40 << 20, max_streams=8, max_batch=8, staging_bytes=16 << 20)
+    a = srv.try_add_stream(fmt="ci16", nchan=1024, taps_per_poly=16, chunk_max=65536, out_slots=2)
+    assert a is not None
+    used = srv.stats()["hbm_used"]
+    assert 0 < used <= 40 << 20
+    refused = srv.try_add_stream(fmt="ci16", nchan=1024, taps_per_poly=16, chunk_max=65536, out_slots=2)
+    assert refused is None                                   # over the budget
+    srv.remove_stream(a)
+    assert srv.stats()["hbm_used"] == 0
+    b = srv.try_add_stream(fmt="ci16", nchan=1024, taps_per_poly=16, chunk_max=65536, out_slots=2)
+    assert b is not None                                     # memory back, admitted
+    with pytest.raises(gpu.AtkDspError):
+        srv.add_stream(fmt="ci16", nchan=1000)               # not a supported shape
+    with pytest.raises(gpu.AtkDspError):
+        srv.submit(b, np.zeros(65537 * 4, np.uint8))         # chunk over chunk_max
+    srv.close()
