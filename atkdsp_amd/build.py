@@ -113,6 +113,10 @@ def build(verbose: bool = False) -> Path:
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", str(tmp), *objs,
           "-lpthread", "-lm"])
     os.replace(tmp, LIB)
+    # the serving-loop load generator, linked against the library beside it
+    _run(["gcc", "-std=c11", "-O2", "-Wall", "-I", str(ROOT / "include"), "-o", str(PKG / "serve_load"),
+          str(CSRC / "load" / "serve_load.c"), "-L", str(PKG), "-l:atkdsp.so", "-Wl,-rpath,$ORIGIN",
+          "-lpthread", "-lm"])
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -38,6 +38,7 @@ void        atkgpu_event_destroy(void *e);
 int         atkgpu_event_record(void *e, void *s);
 int         atkgpu_event_elapsed_ms(void *e0, void *e1, float *ms);
 int         atkgpu_event_sync(void *e);
+int         atkgpu_event_query(void *e);           /* 1 done, 0 pending, <0 error */
 int         atkgpu_d2h_sync(void *dst, const void *src, size_t bytes);
 
 /* ---- unpack --------------------------------------------------------------- */
@@ -142,10 +143,10 @@ typedef struct {               /* layout of PfbJob in pfb_fused.hpp */
     int nvalid, m, opitch, blk0, nblk, pad_;
 } atkgpu_pfb_job;
 typedef struct {
-    const void *raw;           /* device-visible raw bytes */
-    void *dst;                 /* cf32 out */
+    const void *raw;           /* device-visible raw bytes, 16-byte aligned */
+    void *dst;                 /* cf32 out, 16-byte aligned */
     float off_re, off_im;      /* offset folded into the constants */
-    int n, blk0, nblk, pad_;
+    int n, blk0, nblk, fmt;    /* fmt: ATKDSP_FMT_* of this job's bytes */
 } atkgpu_unpack_job;
 typedef struct {
     const void *src;
@@ -156,7 +157,7 @@ typedef struct {
 int atkgpu_pfb_group(unsigned M, unsigned os);      /* hops per block of the fused kernel */
 int atkgpu_pfb_fused_jobs(void *s, const void *jobs, int njobs, int nblocks, unsigned M, unsigned os,
                           unsigned kos, const void *tw);
-int atkgpu_unpack_jobs(void *s, int fmt, const void *jobs, int njobs, int nblocks);
+int atkgpu_unpack_jobs(void *s, const void *jobs, int njobs, int nblocks);   /* any mix of formats */
 int atkgpu_copy_jobs(void *s, const void *jobs, int njobs, int nblocks);
 /* pinned host memory the device can read (and its device address) */
 void *atkgpu_hmalloc_mapped(size_t bytes, void **dev_ptr);

@@ -117,6 +117,13 @@ extern "C" int atkgpu_event_elapsed_ms(void *e0, void *e1, float *ms) {
     return hipEventElapsedTime(ms, (hipEvent_t)e0, (hipEvent_t)e1) == hipSuccess ? 0 : -4;
 }
 
+extern "C" int atkgpu_event_query(void *e) {
+    const hipError_t r = hipEventQuery((hipEvent_t)e);
+    if (r == hipSuccess) return 1;
+    if (r == hipErrorNotReady) return 0;
+    return -4;
+}
+
 extern "C" int atkgpu_event_sync(void *e) {
     return hipEventSynchronize((hipEvent_t)e) == hipSuccess ? 0 : -4;
 }

@@ -22,11 +22,15 @@ __device__ __forceinline__ int find_job(const J *jobs, int njobs, int b) {
 
 struct f2 { float x, y; };
 
+// One format's share of a ragged unpack block: samples [i0, i1) of job jb.
+// Whole 16-byte words of raw bytes are loaded at once (a PCIe read of mapped
+// host memory moves 1 KiB per wavefront load), the ragged end sample by
+// sample. The arithmetic is the reference's per sample, contraction off.
 template <int FMT>
-__global__ void __launch_bounds__(256) k_srv_unpack(const atkgpu_unpack_job *__restrict__ jobs, int njobs) {
+__device__ __forceinline__ void srv_unpack_range(const atkgpu_unpack_job &jb, int i0, int i1) {
 #pragma clang fp contract(off)
-    const int b = blockIdx.x;
-    const atkgpu_unpack_job jb = jobs[find_job(jobs, njobs, b)];
+    constexpr int BPS = (FMT == FMT_CU8 || FMT == FMT_CI8) ? 2 : (FMT == FMT_CF32 ? 8 : 4);
+    constexpr int SPW = 16 / BPS;                 // samples per 16-byte word
     float c0, kk;
     if constexpr (FMT == FMT_CU8) { c0 = 127.5f; kk = 1.0f / 127.5f; }
     else if constexpr (FMT == FMT_CI8) { c0 = 0.0f; kk = 1.0f / 128.0f; }
@@ -35,29 +39,69 @@ __global__ void __launch_bounds__(256) k_srv_unpack(const atkgpu_unpack_job *__r
     else { c0 = 0.0f; kk = 1.0f; }
     // the folded constants as src/convert.c:47, :57, :68 form them
     const float cre = c0 + jb.off_re / kk, cim = c0 + jb.off_im / kk;
+    const bool plain = jb.off_re == 0.0f && jb.off_im == 0.0f;
+    const unsigned char *raw = (const unsigned char *)jb.raw;
     f2 *dst = (f2 *)jb.dst;
-    const int i0 = (b - jb.blk0) * ATKGPU_RAGGED_SPB;
-    const int i1 = min(i0 + ATKGPU_RAGGED_SPB, jb.n);
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
+    auto conv = [&](unsigned re_bits, unsigned im_bits) -> f2 {
         f2 o;
         if constexpr (FMT == FMT_CU8) {
-            const uchar2 v = ((const uchar2 *)jb.raw)[i];
-            o.x = ((float)v.x - cre) * kk;
-            o.y = ((float)v.y - cim) * kk;
+            o.x = ((float)(re_bits & 0xffu) - cre) * kk;
+            o.y = ((float)(im_bits & 0xffu) - cim) * kk;
         } else if constexpr (FMT == FMT_CI8) {
-            const char2 v = ((const char2 *)jb.raw)[i];
-            o.x = ((float)(signed char)v.x - cre) * kk;
-            o.y = ((float)(signed char)v.y - cim) * kk;
+            o.x = ((float)(signed char)(re_bits & 0xffu) - cre) * kk;
+            o.y = ((float)(signed char)(im_bits & 0xffu) - cim) * kk;
         } else if constexpr (FMT == FMT_CI16 || FMT == FMT_CI16Q11) {
-            const short2 v = ((const short2 *)jb.raw)[i];
-            o.x = ((float)v.x - cre) * kk;
-            o.y = ((float)v.y - cim) * kk;
+            o.x = ((float)(short)(re_bits & 0xffffu) - cre) * kk;
+            o.y = ((float)(short)(im_bits & 0xffffu) - cim) * kk;
         } else {
-            const float2 v = ((const float2 *)jb.raw)[i];
-            if (jb.off_re == 0.0f && jb.off_im == 0.0f) { o.x = v.x; o.y = v.y; }
-            else { o.x = v.x - jb.off_re; o.y = v.y - jb.off_im; }
+            const float a = __uint_as_float(re_bits), b = __uint_as_float(im_bits);
+            if (plain) { o.x = a; o.y = b; }
+            else { o.x = a - jb.off_re; o.y = b - jb.off_im; }
         }
-        dst[i] = o;
+        return o;
+    };
+    const int nw = (i1 - i0) / SPW;
+    for (int w = (int)threadIdx.x; w < nw; w += 256) {
+        const int i = i0 + w * SPW;
+        const uint4 v = *(const uint4 *)(raw + (size_t)i * BPS);
+        const unsigned u[4] = {v.x, v.y, v.z, v.w};
+        f2 o[SPW];
+#pragma unroll
+        for (int k = 0; k < SPW; ++k) {
+            if constexpr (BPS == 2) {           // two samples per 32-bit word
+                const unsigned x = u[k >> 1] >> (16 * (k & 1));
+                o[k] = conv(x, x >> 8);
+            } else if constexpr (BPS == 4) {
+                o[k] = conv(u[k], u[k] >> 16);
+            } else {
+                o[k] = conv(u[2 * k], u[2 * k + 1]);
+            }
+        }
+        float4 *d4 = (float4 *)(dst + i);
+#pragma unroll
+        for (int k = 0; k < SPW / 2; ++k) d4[k] = make_float4(o[2 * k].x, o[2 * k].y, o[2 * k + 1].x, o[2 * k + 1].y);
+    }
+    for (int i = i0 + nw * SPW + (int)threadIdx.x; i < i1; i += 256) {
+        const unsigned char *p = raw + (size_t)i * BPS;
+        if constexpr (BPS == 2) dst[i] = conv(p[0], p[1]);
+        else if constexpr (BPS == 4) dst[i] = conv(*(const unsigned short *)p, *(const unsigned short *)(p + 2));
+        else dst[i] = conv(*(const unsigned *)p, *(const unsigned *)(p + 4));
+    }
+}
+
+// Every format in one launch: a block belongs to one job, so the switch is
+// uniform over the block.
+__global__ void __launch_bounds__(256) k_srv_unpack(const atkgpu_unpack_job *__restrict__ jobs, int njobs) {
+    const int b = blockIdx.x;
+    const atkgpu_unpack_job jb = jobs[find_job(jobs, njobs, b)];
+    const int i0 = (b - jb.blk0) * ATKGPU_RAGGED_SPB;
+    const int i1 = min(i0 + ATKGPU_RAGGED_SPB, jb.n);
+    switch (jb.fmt) {
+    case FMT_CU8: srv_unpack_range<FMT_CU8>(jb, i0, i1); break;
+    case FMT_CI8: srv_unpack_range<FMT_CI8>(jb, i0, i1); break;
+    case FMT_CI16: srv_unpack_range<FMT_CI16>(jb, i0, i1); break;
+    case FMT_CI16Q11: srv_unpack_range<FMT_CI16Q11>(jb, i0, i1); break;
+    default: srv_unpack_range<FMT_CF32>(jb, i0, i1); break;
     }
 }
 
@@ -71,20 +115,10 @@ __global__ void __launch_bounds__(256) k_srv_copy(const atkgpu_copy_job *__restr
     for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) dst[i] = src[i];
 }
 
-extern "C" int atkgpu_unpack_jobs(void *s, int fmt, const void *jobs, int njobs, int nblocks) {
+extern "C" int atkgpu_unpack_jobs(void *s, const void *jobs, int njobs, int nblocks) {
     if (!njobs || !nblocks) return 0;
-    const atkgpu_unpack_job *j = (const atkgpu_unpack_job *)jobs;
-    hipStream_t st = (hipStream_t)s;
-    switch (fmt) {
-    case FMT_CU8: hipLaunchKernelGGL(k_srv_unpack<FMT_CU8>, dim3(nblocks), dim3(256), 0, st, j, njobs); break;
-    case FMT_CI8: hipLaunchKernelGGL(k_srv_unpack<FMT_CI8>, dim3(nblocks), dim3(256), 0, st, j, njobs); break;
-    case FMT_CI16: hipLaunchKernelGGL(k_srv_unpack<FMT_CI16>, dim3(nblocks), dim3(256), 0, st, j, njobs); break;
-    case FMT_CI16Q11:
-        hipLaunchKernelGGL(k_srv_unpack<FMT_CI16Q11>, dim3(nblocks), dim3(256), 0, st, j, njobs);
-        break;
-    case FMT_CF32: hipLaunchKernelGGL(k_srv_unpack<FMT_CF32>, dim3(nblocks), dim3(256), 0, st, j, njobs); break;
-    default: return -1;
-    }
+    hipLaunchKernelGGL(k_srv_unpack, dim3(nblocks), dim3(256), 0, (hipStream_t)s,
+                       (const atkgpu_unpack_job *)jobs, njobs);
     return LAUNCH_OK();
 }
 

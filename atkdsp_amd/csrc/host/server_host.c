@@ -4,19 +4,22 @@
  *   admission   a stream's device state (two history rows, taps, an output
  *               ring) is allocated when it is added and charged against a
  *               fixed HBM budget; a stream that does not fit is refused.
- *   submit      copies the chunk's bytes into a pinned staging ring that the
- *               device reads directly (mapped), and returns a ticket. It
+ *   submit      reserves a ticket and staging space under the lock, copies the
+ *               chunk's bytes into the pinned staging ring (which the device
+ *               reads directly, mapped) outside it, then publishes the chunk.
+ *               Submitters of different streams copy in parallel. It
  *               allocates nothing and makes no HIP call.
  *   dispatcher  a thread forms batches continuously: at most one chunk per
  *               stream (each stream's chunks stay in order), taken round-robin
  *               over the streams that have work, while earlier batches still
- *               run. A batch is one ragged unpack launch per format, one fused
- *               filterbank launch per (M, os, taps) class and one history
+ *               run. A batch is one ragged unpack launch (every format), one
+ *               fused filterbank launch per (M, os, taps) class and one history
  *               launch, all on one HIP stream, so a stream's chunks execute in
  *               order.
- *   completion  a thread waits on each batch's event in order, stamps the
- *               host monotonic clock and records each chunk's latency
- *               (completion minus submission), then frees staging space.
+ *   completion  a thread polls each batch's event in order (sleeping a few
+ *               microseconds between polls, so no core is held spinning),
+ *               stamps the host monotonic clock and records each chunk's
+ *               latency (completion minus submission), then frees staging.
  *
  * Every chunk of a stream is computed by the same kernels with the same
  * per-hop arithmetic whatever else is in its batch, so a stream's output is
@@ -25,6 +28,7 @@
  */
 #define _GNU_SOURCE
 #include <pthread.h>
+#include <sys/prctl.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -33,7 +37,8 @@
 
 int atkgpu_pfb_prototype(double *h, unsigned nchan, unsigned K, double atten_db);
 
-#define NINFLIGHT 4
+#define NINFLIGHT 3
+#define POLL_NS 10000          /* completion poll interval */
 #define QCAP 256               /* pending chunks per stream */
 #define LAT_CAP (1u << 22)
 
@@ -42,6 +47,7 @@ typedef struct {
     size_t off;                /* staging byte offset (monotonic) */
     size_t nbytes, n;
     double t_submit;
+    int ready;                 /* bytes are in staging: the dispatcher may take it */
 } chunk_t;
 
 typedef struct {
@@ -329,8 +335,6 @@ long long atkgpu_server_submit(atkgpu_server *v, int sid, const void *raw, size_
         pthread_cond_wait(&v->cv_space, &v->mu);
     if (!v->running || v->failed) { pthread_mutex_unlock(&v->mu); return ATKDSP_E_FFT; }
     const double t = now_s();
-    /* the bytes go in before the chunk is published to the dispatcher */
-    if (nb) memcpy(v->stage_h + (off % v->stage_cap), raw, nb);
     const long long ticket = s->submitted++;
     chunk_t *c = &s->q[(s->qh + s->qn) % QCAP];
     c->ticket = ticket;
@@ -338,8 +342,15 @@ long long atkgpu_server_submit(atkgpu_server *v, int sid, const void *raw, size_
     c->nbytes = nb;
     c->n = n;
     c->t_submit = t;
+    c->ready = 0;              /* queued in ticket order, not yet takeable */
     s->qn++;
     if (!v->chunks && !v->nlat) v->t_first = t;
+    pthread_mutex_unlock(&v->mu);
+    /* the copy runs unlocked; the queue entry and staging span are this
+     * call's alone until it is marked ready */
+    if (nb) memcpy(v->stage_h + (off % v->stage_cap), raw, nb);
+    pthread_mutex_lock(&v->mu);
+    c->ready = 1;
     pthread_cond_signal(&v->cv_work);
     pthread_mutex_unlock(&v->mu);
     return ticket;
@@ -355,26 +366,26 @@ static int launch_batch(atkgpu_server *v, batch_t *bt) {
     char *dbase = (char *)bt->desc_d, *hbase = (char *)bt->desc_h;
 #define DEV(p) ((void *)(dbase + ((char *)(p) - hbase)))
     int rc = 0;
-    /* unpack, one launch per format present */
-    int nu = 0;
-    for (int fmt = 0; fmt <= 4 && !rc; ++fmt) {
-        const int u0 = nu;
-        int blk = 0;
-        for (int i = 0; i < n; ++i) {
-            const bitem *it = &bt->items[i];
-            if (it->fmt != fmt || it->c.n == 0) continue;
-            atkgpu_unpack_job *j = &uj[nu++];
-            j->raw = v->stage_d + (it->c.off % v->stage_cap);
-            j->dst = it->row + it->keep;
-            j->off_re = it->off_re;
-            j->off_im = it->off_im;
-            j->n = (int)it->c.n;
-            j->blk0 = blk;
-            j->nblk = (j->n + ATKGPU_RAGGED_SPB - 1) / ATKGPU_RAGGED_SPB;
-            blk += j->nblk;
-        }
-        if (nu > u0) rc = atkgpu_unpack_jobs(v->stream, fmt, DEV(&uj[u0]), nu - u0, blk);
+    /* unpack, every format in one launch (staging offsets are 64-byte
+     * aligned and keep is a multiple of M >= 64, so both ends are 16-byte
+     * aligned for the kernel's word loads and stores) */
+    int nu = 0, ublk = 0;
+    for (int i = 0; i < n; ++i) {
+        const bitem *it = &bt->items[i];
+        if (it->c.n == 0) continue;
+        atkgpu_unpack_job *j = &uj[nu++];
+        j->raw = v->stage_d + (it->c.off % v->stage_cap);
+        j->dst = it->row + it->keep;
+        j->off_re = it->off_re;
+        j->off_im = it->off_im;
+        j->n = (int)it->c.n;
+        j->blk0 = ublk;
+        j->nblk = (j->n + ATKGPU_RAGGED_SPB - 1) / ATKGPU_RAGGED_SPB;
+        j->fmt = it->fmt;
+        ublk += j->nblk;
+        if ((((size_t)j->raw) | ((size_t)j->dst)) & 15) rc = ATKDSP_E_ARG;
     }
+    if (!rc && nu) rc = atkgpu_unpack_jobs(v->stream, DEV(uj), nu, ublk);
     /* the filterbank, one launch per shape class present */
     int np = 0;
     unsigned char done[4096];
@@ -430,7 +441,7 @@ static int form_batch(atkgpu_server *v, batch_t *bt) {
     for (unsigned t = 0; t < S && n < (int)v->max_batch && n < 4096; ++t) {
         const unsigned sid = (v->rr + t) % S;
         sstream *s = &v->st[sid];
-        if (!s->used || !s->qn) continue;
+        if (!s->used || !s->qn || !s->q[s->qh].ready) continue;
         const chunk_t c = s->q[s->qh];
         s->qh = (s->qh + 1) % QCAP;
         s->qn--;
@@ -494,15 +505,27 @@ static void *disp_main(void *arg) {
     return NULL;
 }
 
+/* wait for an event by polling: a blocking wait either spins a core (which
+ * the submitters and the dispatcher need) or sleeps on an interrupt */
+static int poll_event(void *ev) {
+    const struct timespec ts = {0, POLL_NS};
+    for (;;) {
+        const int q = atkgpu_event_query(ev);
+        if (q) return q < 0 ? q : 0;
+        nanosleep(&ts, NULL);
+    }
+}
+
 static void *comp_main(void *arg) {
     atkgpu_server *v = (atkgpu_server *)arg;
+    prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);      /* wake within ~1 us of the poll interval */
     pthread_mutex_lock(&v->mu);
     for (;;) {
         while (v->running && v->b_done == v->b_submit && !v->failed) pthread_cond_wait(&v->cv_launch, &v->mu);
         if (v->b_done == v->b_submit && (!v->running || v->failed)) break;
         batch_t *bt = &v->batch[v->b_done % NINFLIGHT];
         pthread_mutex_unlock(&v->mu);
-        const int rc = atkgpu_event_sync(bt->event);
+        const int rc = poll_event(bt->event);
         const double t = now_s();
         pthread_mutex_lock(&v->mu);
         if (rc) { v->failed = 1; pthread_cond_broadcast(&v->cv_done); pthread_cond_broadcast(&v->cv_space); break; }

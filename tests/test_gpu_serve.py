@@ -85,11 +85,13 @@ def test_serving_matches_dropin_and_alone(gpu):
 
 def test_admission_against_the_hbm_budget(gpu):
     from atkdsp_amd.serve import Server
-    srv = Server(hbm_budget=40 << 20, max_streams=8, max_batch=8, staging_bytes=16 << 20)
+    # one 1024-channel, 16-tap stream with 64K-sample chunks needs about 3.5 MiB
+    # (two history rows, taps, two output slots): the budget holds one, not two
+    srv = Server(hbm_budget=6 << 20, max_streams=8, max_batch=8, staging_bytes=16 << 20)
     a = srv.try_add_stream(fmt="ci16", nchan=1024, taps_per_poly=16, chunk_max=65536, out_slots=2)
     assert a is not None
     used = srv.stats()["hbm_used"]
-    assert 0 < used <= 40 << 20
+    assert 0 < used <= 6 << 20
     refused = srv.try_add_stream(fmt="ci16", nchan=1024, taps_per_poly=16, chunk_max=65536, out_slots=2)
     assert refused is None                                   # over the budget
     srv.remove_stream(a)
