@@ -40,6 +40,8 @@ int         atkgpu_event_elapsed_ms(void *e0, void *e1, float *ms);
 int         atkgpu_event_sync(void *e);
 int         atkgpu_event_query(void *e);           /* 1 done, 0 pending, <0 error */
 int         atkgpu_d2h_sync(void *dst, const void *src, size_t bytes);
+int         atkgpu_d2h_2d_sync(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
+                               size_t height);
 
 /* ---- unpack --------------------------------------------------------------- */
 /* number of double pairs in the mean partial buffer */

@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
     const int nb8 = (int)gridDim.x / 8;
     const int lb = (int)(blockIdx.x % 8) * nb8 + (int)(blockIdx.x / 8);
     if (lb >= a.nblocks) return;
-    int e, tile, nlast, m, opitch, gpb;
+    int e, tile, nlast, m, opitch, gpb, ntile;
     const cpx *s;
     cpx *oute;
     const float *taps;
@@ -108,6 +108,7 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
         m = jb.m;
         opitch = jb.opitch;
         gpb = 1;
+        ntile = jb.nblk;
     } else {
         e = lb / a.nblk_per_el;
         tile = lb % a.nblk_per_el;
@@ -120,6 +121,7 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
         m = (int)a.m;
         opitch = (int)a.opitch;
         gpb = a.groups_per_block;
+        ntile = a.nblk_per_el;
     }
     const int tid = threadIdx.x;
     const int sub = tid / S::HOPC;
@@ -130,8 +132,13 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
 
     // The loops over column passes and FFT rounds stay rolled: the unrolled
     // fold is long, and the kernel must fit the instruction cache.
+    // Groups are dealt to an element's blocks in turn (group tile, then
+    // tile + ntile, ...): blocks resident together on an XCD fold adjacent
+    // groups, whose windows share KOS-1 rows, so those rows are read from HBM
+    // once and from L2 after; the neighbours' output segments also fill the
+    // same cache lines.
     for (int grp = 0; grp < gpb; ++grp) {
-        const int i0 = (tile * gpb + grp) * G;
+        const int i0 = (tile + grp * ntile) * G;
         if (i0 >= m) break;
         // ---- fold ----
 #pragma unroll 1

@@ -134,3 +134,9 @@ extern "C" int atkgpu_d2h_sync(void *dst, const void *src, size_t bytes) {
     if (!bytes) return 0;
     return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -4;
 }
+
+extern "C" int atkgpu_d2h_2d_sync(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
+                                  size_t height) {
+    if (!width || !height) return 0;
+    return hipMemcpy2D(dst, dpitch, src, spitch, width, height, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -4;
+}

@@ -80,7 +80,9 @@ atkgpu_chain *atkgpu_chain_create(const atkgpu_chain_cfg *cfg, unsigned S) {
     if (keep < cfg->nfft) keep = cfg->nfft;
     c->keep = (keep + 1) & ~(size_t)1;
     c->spitch = (c->keep + cfg->block_max + 63) & ~(size_t)63;
-    c->m_cap = (cfg->block_max + c->hop) / c->hop + 1;
+    /* output rows are padded to whole 128-byte lines: the filterbank's
+     * per-channel segments then fill whole 32-byte sectors */
+    c->m_cap = (((cfg->block_max + c->hop) / c->hop + 1) + 15) & ~(size_t)15;
     c->uparts = 256;
     c->nfb_max = 64;
     if (atkgpu_stream_create(&c->stream)) { chain_free(c); return NULL; }
@@ -171,7 +173,7 @@ ptrdiff_t atkgpu_chain_run(atkgpu_chain *c, size_t n) {
     if (!rc && m)
         rc = atkgpu_pfb_fused(c->stream, R + (keep - c->ov - c->lc), c->spitch, (size_t)c->ov + avail, c->S, m,
                               c->b, c->M, c->os, (unsigned)c->kos, c->d_taps, atkgpu_fftplan_twiddles(c->plan_m),
-                              NULL, 0, c->d_out, m, 0);
+                              NULL, 0, c->d_out, (m + 15) & ~(size_t)15, 0);
     if (!rc) rc = atkgpu_event_record(c->ev[2], c->stream);
     if (!rc && frames) {
         int nfb = (int)((2048 + c->S - 1) / c->S);
@@ -217,9 +219,11 @@ int atkgpu_chain_timing(atkgpu_chain *c, float *ms_run, float *ms_pfb, float *ms
 
 ptrdiff_t atkgpu_chain_read_pfb(atkgpu_chain *c, unsigned s, atkdsp_cf32 *out, size_t cap) {
     if (!c || s >= c->S || !out) return ATKDSP_E_ARG;
-    const size_t cnt = (size_t)c->M * c->last_m;
+    const size_t cnt = (size_t)c->M * c->last_m, mp = (c->last_m + 15) & ~(size_t)15;
     if (cap < cnt) return ATKDSP_E_CAP;
-    int rc = atkgpu_d2h(c->stream, out, c->d_out + (size_t)s * cnt, cnt * sizeof(atkdsp_cf32));
+    int rc = cnt ? atkgpu_d2h_2d(c->stream, out, c->last_m * sizeof(atkdsp_cf32), c->d_out + (size_t)s * c->M * mp,
+                                 mp * sizeof(atkdsp_cf32), c->last_m * sizeof(atkdsp_cf32), c->M)
+                 : 0;
     if (!rc) rc = atkgpu_sync(c->stream);
     return rc ? ATKDSP_E_FFT : (ptrdiff_t)c->last_m;
 }

@@ -131,7 +131,8 @@ atkdsp_pfb *atkdsp_pfb_create(double fs, unsigned nchan, unsigned oversample, un
     slab = (slab + p->hop - 1) / p->hop * p->hop;
     p->slab = slab;
     p->spitch = p->ov + p->hop + slab;
-    p->hops_cap = (slab + p->hop) / p->hop + 1;
+    /* padded to 16 hops: the fused kernel's output rows start on 128-byte lines */
+    p->hops_cap = (((slab + p->hop) / p->hop + 1) + 15) & ~(size_t)15;
     p->h = (double *)malloc(p->L * sizeof(double));
     p->lo = (atkdsp_cf32 *)calloc((size_t)max_nel * p->hop, sizeof(atkdsp_cf32));
     if (!p->h || !p->lo || atkgpu_stream_create(&p->stream)) { pfb_free(p); return NULL; }
@@ -231,6 +232,8 @@ static int pfb_sub(atkdsp_pfb *p, const atkdsp_cf32 *const *ins, unsigned nel, s
         return 0;
     }
     atkdsp_cf32 *S = p->d_s[p->cur];
+    /* device output pitch: whole 128-byte lines for the fused kernel */
+    const size_t mp = p->kos ? (ms + 15) & ~(size_t)15 : ms;
     int rc = 0;
     for (unsigned e = 0; e < nel && !rc; ++e) {
         atkdsp_cf32 *se = S + (size_t)e * p->spitch;
@@ -240,7 +243,7 @@ static int pfb_sub(atkdsp_pfb *p, const atkdsp_cf32 *const *ins, unsigned nel, s
     if (p->kos) {
         if (!rc) rc = atkgpu_pfb_fused(p->stream, S, p->spitch, (size_t)ov + avail, nel, ms, (long long)p->b, M,
                                        p->os, (unsigned)p->kos, p->d_taps, atkgpu_fftplan_twiddles(p->plan),
-                                       p->cal_nel ? (const void *)p->d_cal : NULL, p->cal_nel, p->d_out, ms, 0);
+                                       p->cal_nel ? (const void *)p->d_cal : NULL, p->cal_nel, p->d_out, mp, 0);
     } else {
         if (!rc) rc = atkgpu_pfb_fold(p->stream, S, p->spitch, nel, ms, M, hop, p->K, p->d_taps, p->d_fold);
         if (!rc) rc = atkgpu_fft_batch(p->stream, p->plan, p->d_fold, M, NULL, (size_t)nel * ms, 0, p->d_fold,
@@ -250,7 +253,7 @@ static int pfb_sub(atkdsp_pfb *p, const atkdsp_cf32 *const *ins, unsigned nel, s
                                               p->cal_nel ? (const void *)p->d_cal : NULL, p->cal_nel, p->d_out);
     }
     if (!rc) rc = atkgpu_d2h_2d(p->stream, out + i_done, m * sizeof(atkdsp_cf32), p->d_out,
-                                ms * sizeof(atkdsp_cf32), ms * sizeof(atkdsp_cf32), (size_t)nel * M);
+                                mp * sizeof(atkdsp_cf32), ms * sizeof(atkdsp_cf32), (size_t)nel * M);
     /* next history: s[ms hop, ms hop + ov) into the other slab buffer */
     atkdsp_cf32 *T = p->d_s[p->cur ^ 1];
     for (unsigned e = 0; e < nel && !rc; ++e)

@@ -222,7 +222,8 @@ int atkgpu_server_add_stream(atkgpu_server *v, const atkgpu_stream_cfg *cfg) {
     const unsigned M = cfg->nchan, os = cfg->oversample, K = cfg->taps_per_poly;
     const unsigned hop = M / os, ov = M * K - hop;
     const unsigned keep = ov + hop;
-    const size_t m_max = (cfg->chunk_max + hop) / hop + 1;
+    /* output rows padded to 16 hops (128 bytes): whole sectors per segment */
+    const size_t m_max = (((cfg->chunk_max + hop) / hop + 1) + 15) & ~(size_t)15;
     const size_t row_len = (size_t)keep + cfg->chunk_max;
     const size_t bytes = 2 * row_len * sizeof(atkdsp_cf32) + (size_t)kos * hop * sizeof(float) +
                          (size_t)cfg->out_slots * M * m_max * sizeof(atkdsp_cf32);
@@ -408,7 +409,7 @@ static int launch_batch(atkgpu_server *v, batch_t *bt) {
             j->b0 = it->b0;
             j->nvalid = (int)(it->ov + it->lc + it->c.n);
             j->m = (int)it->m;
-            j->opitch = (int)it->m;
+            j->opitch = (int)((it->m + 15) & ~(size_t)15);
             j->blk0 = blk;
             j->nblk = (int)((it->m + G - 1) / G);
             j->pad_ = 0;
@@ -572,7 +573,8 @@ ptrdiff_t atkgpu_server_read(atkgpu_server *v, int sid, long long ticket, atkdsp
     const unsigned M = s->M;
     pthread_mutex_unlock(&v->mu);
     if (cap < (size_t)M * m) return ATKDSP_E_CAP;
-    rc = atkgpu_d2h_sync(out, src, (size_t)M * m * sizeof(atkdsp_cf32));
+    rc = atkgpu_d2h_2d_sync(out, m * sizeof(atkdsp_cf32), src, ((m + 15) & ~(size_t)15) * sizeof(atkdsp_cf32),
+                            m * sizeof(atkdsp_cf32), M);
     return rc ? ATKDSP_E_FFT : (ptrdiff_t)m;
 }
 
