@@ -51,6 +51,18 @@ def pfb_bytes_per_launch(S: int, n: int, m: int) -> float:
     return S * ((n + ov) * 8 + m * M * 8)
 
 
+def pfb_traffic():
+    """HBM bytes per launch of the filterbank kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each scaled by what the counter
+    reads for a plain copy kernel of known size)."""
+    f = ROOT / "profiles" / "k_pfb_fused_c2_traffic.json"
+    if not f.exists():
+        return None, None
+    d = json.loads(f.read_text())
+    b = (d["fetch_kib"] * d["fetch_scale"] + d["write_kib"] * d["write_scale"]) * 1024
+    return round(b), f"profiles/{f.name}: {d['source']}"
+
+
 def cpu_baseline(seconds: float, base: np.ndarray):
     """The reference C library on the host cores: the same chain per block,
     streaming, blocks of the same content, until `seconds` elapse."""
@@ -185,6 +197,7 @@ def main() -> int:
         pfbs.append(b)
         disps.append(d)
     ms_pfb = float(np.mean(pfbs))
+    traffic, traffic_src = pfb_traffic()
     pfb_b = pfb_bytes_per_launch(S, N_BLOCK, m)
     achieved = pfb_b / (ms_pfb * 1e-3) / 1e9
     value = S * N_BLOCK / (ms_step * 1e-3)
@@ -224,7 +237,8 @@ def main() -> int:
         "kernel_ms": {"run": round(float(np.mean(runs)), 4), "pfb_fused": round(ms_pfb, 4),
                       "display": round(float(np.mean(disps)), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "k_pfb_fused<1024,2,32>",
                      "bytes_per_launch": pfb_b},
         "cpu_baseline": cpu,
