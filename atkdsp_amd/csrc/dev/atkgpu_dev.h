@@ -161,6 +161,21 @@ int atkgpu_pfb_fused_jobs(void *s, const void *jobs, int njobs, int nblocks, uns
                           unsigned kos, const void *tw);
 int atkgpu_unpack_jobs(void *s, const void *jobs, int njobs, int nblocks);   /* any mix of formats */
 int atkgpu_copy_jobs(void *s, const void *jobs, int njobs, int nblocks);
+int atkgpu_copy_bytes_jobs(void *s, const void *jobs, int njobs, int nblocks);   /* n counts bytes, 4096 per block */
+
+/* ---- the filterbank's matrix-core form (pfb_mx.hip) ----
+ * raw rows of 8/12-bit samples -> fold on MFMA (exact BF16/FP16 samples,
+ * hi+lo or single taps, FP32 accumulation) -> the fused FFT/rotation/store.
+ * form: 1 BF16 hi+lo, 2 BF16, 3 FP16 hi+lo, 4 FP16 */
+int    atkgpu_pfb_mx_supported(unsigned M, unsigned os, unsigned kos, int fmt, int form);
+size_t atkgpu_pfb_mx_tpad_count(unsigned M, unsigned os, unsigned kos, int form);   /* 16-bit entries */
+int    atkgpu_pfb_mx_tpad(void *s, const float *taps, unsigned M, unsigned os, unsigned kos, int form,
+                          float tscale, void *tpad);
+int    atkgpu_pfb_mx(void *s, const void *raw, size_t spitch, size_t nvalid, unsigned nel, size_t m, long long b0,
+                     unsigned M, unsigned os, unsigned kos, int fmt, int form, const void *tpad,
+                     const float *tapsum, const float *taps, int zlo, float scale, const void *offs,
+                     const void *tw, const void *cal, unsigned cal_nel, void *out, size_t opitch);
+
 /* pinned host memory the device can read (and its device address) */
 void *atkgpu_hmalloc_mapped(size_t bytes, void **dev_ptr);
 

@@ -123,6 +123,43 @@ ATKGPU_API ptrdiff_t atkgpu_server_read(atkgpu_server *v, int sid, long long tic
 ATKGPU_API int       atkgpu_server_drain(atkgpu_server *v);
 ATKGPU_API int       atkgpu_server_stats(atkgpu_server *v, atkgpu_serve_stats *st, int reset);
 
+/* ---- the filterbank in matrix-core form: S streams of one shape ----------------
+ * The polyphase fold as a matrix product on the MFMA units, from raw 8-bit
+ * (cu8, ci8) or 12-bit (ci16q11) samples held exactly in BF16 or FP16, taps
+ * split into hi + lo terms (or one term), FP32 accumulation; the FFT,
+ * rotation and channel-major store are the FP32 path's. The offset is fixed
+ * per stream (unpack_dc's offset argument). form ATKGPU_FORM_FP32 runs the
+ * plain FP32 path (unpack, then the fused FP32 filterbank) on the same
+ * plumbing, the form the others are measured against.
+ * Supported shapes: 64 channels 1x with up to 8 taps/branch (every form and
+ * format), 64 channels 1x up to 16 or 32 taps, 64 channels 2x up to 8 taps,
+ * 128 channels 2x up to 8 taps (BF16 hi+lo for 8-bit, FP16 hi+lo for 12-bit). */
+enum { ATKGPU_FORM_FP32 = 0, ATKGPU_FORM_BF16X2 = 1, ATKGPU_FORM_BF16 = 2, ATKGPU_FORM_FP16X2 = 3,
+       ATKGPU_FORM_FP16 = 4 };
+typedef struct {
+    int fmt;                   /* ATKDSP_FMT_CU8, _CI8 or _CI16Q11 (FP32 form: any) */
+    double fs;
+    unsigned nchan, oversample, taps_per_poly;
+    double atten_db;
+    int form;                  /* ATKGPU_FORM_* */
+    size_t block_max;          /* samples per stream per run, a multiple of 8 */
+    float off_re, off_im;      /* the fixed offset subtracted by the unpack */
+} atkgpu_pfbx_cfg;
+typedef struct atkgpu_pfbx atkgpu_pfbx;
+ATKGPU_API atkgpu_pfbx *atkgpu_pfbx_create(const atkgpu_pfbx_cfg *cfg, unsigned nstreams);
+ATKGPU_API void         atkgpu_pfbx_destroy(atkgpu_pfbx *v);
+ATKGPU_API int          atkgpu_pfbx_reset(atkgpu_pfbx *v);
+/* device address of stream s's next raw block (valid until the next run) */
+ATKGPU_API void        *atkgpu_pfbx_input(atkgpu_pfbx *v, unsigned s);
+ATKGPU_API int          atkgpu_pfbx_upload(atkgpu_pfbx *v, unsigned s, const void *raw, size_t nbytes);
+/* one block of n samples for every stream; returns the hops produced */
+ATKGPU_API ptrdiff_t    atkgpu_pfbx_run(atkgpu_pfbx *v, size_t n);
+ATKGPU_API int          atkgpu_pfbx_sync(atkgpu_pfbx *v);
+/* device times of the last run: whole run, the unpack (FP32 form only) and the filterbank kernel */
+ATKGPU_API int          atkgpu_pfbx_timing(atkgpu_pfbx *v, float *ms_run, float *ms_unpack, float *ms_pfb);
+/* stream s's last output, [nchan][m] */
+ATKGPU_API ptrdiff_t    atkgpu_pfbx_read(atkgpu_pfbx *v, unsigned s, atkdsp_cf32 *out, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif
