@@ -40,6 +40,10 @@ class _Cf32(C.Structure):
     _fields_ = [("re", C.c_float), ("im", C.c_float)]
 
 
+class _Nco(C.Structure):
+    _fields_ = [("phase", C.c_double), ("step", C.c_double)]
+
+
 _ERRORS = {-1: "bad argument", -2: "out of memory", -3: "output buffer too small",
            -4: "FFT backend or device failure"}
 
@@ -93,6 +97,9 @@ def _bind(lib) -> None:
         "atkdsp_pfb_cov_accumulate": (None, [cf, u, u, sz, P(C.c_uint8), cf, P(C.c_uint32)]),
         "atkdsp_pfb_cov_reset": (None, [cf, P(C.c_uint32), u, u, P(C.c_uint8)]),
         "atkdsp_pfb_channel_power": (None, [cf, u, u, sz, fp, fp]),
+        "atkdsp_nco_init": (None, [P(_Nco), d, d]),
+        "atkdsp_nco_set_freq": (None, [P(_Nco), d, d]),
+        "atkdsp_nco_mix": (None, [P(_Nco), cf, cf, sz]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -382,6 +389,32 @@ def stitch_max(seg, seg_lo_hz: float, seg_hz_per_bin: float, keep_lo_hz: float,
 
 
 # ---- polyphase filterbank --------------------------------------------------------------
+class Nco:
+    """Phase-continuous complex mixer (reference python/atkdsp/__init__.py
+    Nco): ``mix`` shifts a channel at +freq to DC; the FP64 phase is carried
+    in the struct across calls, the rotation runs on the GPU."""
+
+    def __init__(self, freq_hz: float, sample_rate: float):
+        self._lib = load()
+        self._s = _Nco()
+        self._lib.atkdsp_nco_init(C.byref(self._s), float(freq_hz), float(sample_rate))
+
+    @property
+    def phase(self) -> float:
+        return float(self._s.phase)
+
+    def set_freq(self, freq_hz: float, sample_rate: float) -> None:
+        self._lib.atkdsp_nco_set_freq(C.byref(self._s), float(freq_hz), float(sample_rate))
+
+    def mix(self, x, out: np.ndarray | None = None) -> np.ndarray:
+        x = _cf32(x, "x")
+        if out is None:
+            out = np.empty_like(x)
+        self._lib.atkdsp_nco_mix(C.byref(self._s), x.ctypes.data_as(C.POINTER(_Cf32)),
+                                 out.ctypes.data_as(C.POINTER(_Cf32)), x.size)
+        return out
+
+
 class Pfb:
     """Oversampled polyphase filterbank: nchan channels spaced fs/nchan,
     `oversample` x; process_n runs N element streams on one block clock."""

@@ -48,6 +48,7 @@ IMPLEMENTED = {
     "atkdsp_pfb_block_origin", "atkdsp_pfb_group_delay", "atkdsp_pfb_set_cal",
     "atkdsp_pfb_describe", "atkdsp_pfb_cov_accumulate", "atkdsp_pfb_cov_reset",
     "atkdsp_pfb_channel_power",
+    "atkdsp_nco_init", "atkdsp_nco_set_freq", "atkdsp_nco_mix",
 }
 
 _STUB_BODY = {

@@ -176,6 +176,14 @@ int    atkgpu_pfb_mx(void *s, const void *raw, size_t spitch, size_t nvalid, uns
                      const float *tapsum, const float *taps, int zlo, float scale, const void *offs,
                      const void *tw, const void *cal, unsigned cal_nel, void *out, size_t opitch);
 
+/* ---- the NCO (nco.hip): FP64 phase, FP32 rotation ----
+ * mix: samples base .. base+n-1 of a call whose phase is ph0 (any chunking
+ * gives the same values); run: the phase lives in device memory,
+ * st_in = {phase, step} -> st_out (and *logp when set) = the wrapped next phase */
+int atkgpu_nco_mix(void *s, const void *in, void *out, size_t n, size_t base, double ph0, double step);
+int atkgpu_nco_run_dev(void *s, const void *in, void *out, size_t n, const double *st_in, double *st_out,
+                   double *logp);
+
 /* pinned host memory the device can read (and its device address) */
 void *atkgpu_hmalloc_mapped(size_t bytes, void **dev_ptr);
 

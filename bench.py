@@ -51,6 +51,76 @@ def pfb_bytes_per_launch(S: int, n: int, m: int) -> float:
     return S * ((n + ov) * 8 + m * M * 8)
 
 
+def precision_forms(S: int = 256, n: int = 1_200_000, steps: int = 10):
+    """The filterbank's matrix-core forms against the plain FP32 form on the
+    C1 shape (2.4 MSPS cu8, 64 channels, 8 taps/branch, critically sampled):
+    S streams of n samples each, resident in HBM. The FP32 form is unpack +
+    the fused FP32 filterbank; the matrix-core forms read the raw bytes."""
+    from atkdsp_amd import synth
+    from atkdsp_amd.pfbx import PfbX
+    base = np.frombuffer(synth.c1_bytes(n=n), np.uint8).reshape(-1, 2)
+    raws = [np.roll(base, s * 7919, axis=0) for s in range(S)]
+    out = {}
+    for form in ["fp32", "bf16x2", "fp16x2", "bf16"]:
+        px = PfbX(form=form, fmt="cu8", fs=2.4e6, nchan=64, oversample=1, taps_per_poly=8, atten_db=81.2953,
+                  streams=S, block_max=n, offset=0.001 + 0.002j)
+        for s in range(S):
+            px.upload(s, raws[s])
+        px.sync()
+        for _ in range(2):
+            px.run(n)
+        px.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            px.run(n)
+        px.sync()
+        wall = (time.perf_counter() - t0) * 1e3 / steps
+        ev = []
+        for _ in range(3):
+            px.run(n)
+            ev.append(px.timing())
+        run_ms, unpack_ms, pfb_ms = (float(np.mean([e[i] for e in ev])) for i in range(3))
+        out[form] = {"ms_per_run": round(wall, 4), "device_ms": round(run_ms, 4), "unpack_ms": round(unpack_ms, 4),
+                     "filterbank_ms": round(pfb_ms, 4), "Msamples_per_s": round(S * n / (wall * 1e-3) / 1e6, 1)}
+        px.close()
+        print(f"[bench] precision form {form}: {out[form]}", file=sys.stderr, flush=True)
+    fp = out["fp32"]["device_ms"]
+    return {"workload": f"{S} streams x {n} cu8 samples (2.4 MSPS), PFB 64 ch, 8 taps/branch, 1x, 81.3 dB",
+            "forms": out,
+            "speedup_vs_fp32": {f: round(fp / out[f]["device_ms"], 3) for f in out if f != "fp32"},
+            "note": "device_ms: HIP events around one whole run (unpack where there is one, filterbank, "
+                    "history copy); errors against the FP64 oracle are in tests/test_gpu_precision.py"}
+
+
+def nco_phase_run(seconds: float = 10.0, fs: float = 40e6, p: int = 1234, q: int = 10007, chunk: int = 65536):
+    """10 s of 40 MSPS through the device-resident NCO in 65,536-sample runs,
+    f = fs * p / q: the FP64 phase after every run against the exact phase
+    -2 pi ((n p) mod q) / q (integer arithmetic, no accumulation)."""
+    import math
+    from atkdsp_amd.nco import NcoRun
+
+    def wrap(ph):
+        ph = math.fmod(ph, 2 * math.pi)
+        return ph - 2 * math.pi if ph >= math.pi else (ph + 2 * math.pi if ph < -math.pi else ph)
+
+    nrun = int(math.ceil(seconds * fs / chunk))
+    run = NcoRun(fs * p / q, fs, chunk_max=chunk, log_max=nrun)
+    run.upload(np.ones(chunk, np.complex64))
+    run.sync()
+    t0 = time.perf_counter()
+    for _ in range(nrun):
+        run.run(chunk)
+    run.sync()
+    ms = (time.perf_counter() - t0) * 1e3
+    log = run.log()
+    err = np.array([abs(wrap(log[k] - wrap(-2 * math.pi * (((k + 1) * chunk * p) % q) / q)))
+                    for k in range(nrun)])
+    run.close()
+    return {"samples": nrun * chunk, "runs": nrun, "chunk": chunk, "f_over_fs": f"{p}/{q}", "ms": round(ms, 2),
+            "Msamples_per_s": round(nrun * chunk / (ms * 1e-3) / 1e6, 1),
+            "final_phase_err_rad": float(err[-1]), "max_phase_err_rad": float(err.max())}
+
+
 def pfb_traffic():
     """HBM bytes per launch of the filterbank kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each scaled by what the counter
@@ -143,6 +213,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="0 skips the serving runs")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the precision-form comparison and the NCO phase run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB -> 1 GiB copy kernel 5 times (the reference point for "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -206,6 +278,10 @@ def main() -> int:
     if args.serve_seconds > 0:
         serve["uniform"] = serving("uniform", S, args.serve_seconds)
         serve["c4"] = serving("c4", 128, args.serve_seconds)
+    extras = {}
+    if not args.no_extras:
+        extras["precision_form"] = precision_forms()
+        extras["nco_phase_run"] = nco_phase_run()
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, streams[0])
     uni = serve.get("uniform", {})
     out = {
@@ -242,6 +318,7 @@ def main() -> int:
                      "kernel": "k_pfb_fused<1024,2,32>",
                      "bytes_per_launch": pfb_b},
         "cpu_baseline": cpu,
+        **extras,
         "build": atkdsp_amd.build_info(),
     }
     print(json.dumps(out))

@@ -117,6 +117,15 @@ ATKDSP_API void atkdsp_stitch_max(const float *seg, size_t nseg,
                                   double out_lo_hz, double out_hz_per_bin,
                                   float *out, size_t nout);
 
+/* ---- NCO: reference include/atkdsp.h:309-319, src/nco.c:27-98.
+ * out[n] = in[n] * exp(j*(phase + n*step)); the phase advances by n*step,
+ * wrapped, in FP64 in the caller's struct; step = -2*pi*f/fs. in and out may
+ * alias. The rotation runs on the GPU. */
+typedef struct { double phase; double step; } atkdsp_nco;
+ATKDSP_API void atkdsp_nco_init(atkdsp_nco *nco, double freq_hz, double sample_rate);
+ATKDSP_API void atkdsp_nco_set_freq(atkdsp_nco *nco, double freq_hz, double sample_rate);
+ATKDSP_API void atkdsp_nco_mix(atkdsp_nco *nco, const atkdsp_cf32 *in, atkdsp_cf32 *out, size_t n);
+
 /* ---- polyphase filterbank: reference include/atkdsp.h:3191-3250,
  * src/channelizer.c:79-471. Output layouts, bookkeeping, cal and W1/W2 are
  * the reference's; the streaming history (ov samples per element) stays in

@@ -160,6 +160,24 @@ ATKGPU_API int          atkgpu_pfbx_timing(atkgpu_pfbx *v, float *ms_run, float 
 /* stream s's last output, [nchan][m] */
 ATKGPU_API ptrdiff_t    atkgpu_pfbx_read(atkgpu_pfbx *v, unsigned s, atkdsp_cf32 *out, size_t cap);
 
+/* ---- the device-resident NCO: a long run fed from device buffers ---------------
+ * The FP64 phase lives in device memory across runs (two slots used in turn)
+ * and is logged after each run; the rotation is the drop-in's (FP64 phase
+ * per sample, FP32 sincos of the reduced angle). Nothing in a run allocates. */
+typedef struct atkgpu_nco atkgpu_nco;
+ATKGPU_API atkgpu_nco *atkgpu_nco_create(double freq_hz, double sample_rate, size_t chunk_max, size_t log_max);
+ATKGPU_API void        atkgpu_nco_destroy(atkgpu_nco *v);
+ATKGPU_API void       *atkgpu_nco_input(atkgpu_nco *v);     /* chunk_max cf32, device */
+ATKGPU_API void       *atkgpu_nco_output(atkgpu_nco *v);    /* chunk_max cf32, device */
+ATKGPU_API int         atkgpu_nco_upload(atkgpu_nco *v, const atkdsp_cf32 *in, size_t n);
+/* mix input[0, n) into output[0, n) and advance the phase by n samples */
+ATKGPU_API int         atkgpu_nco_run(atkgpu_nco *v, size_t n);
+ATKGPU_API int         atkgpu_nco_sync(atkgpu_nco *v);
+ATKGPU_API ptrdiff_t   atkgpu_nco_read(atkgpu_nco *v, atkdsp_cf32 *out, size_t n);
+ATKGPU_API int         atkgpu_nco_phase(atkgpu_nco *v, double *phase, long long *samples);
+/* the phase after each of the first log_max runs */
+ATKGPU_API ptrdiff_t   atkgpu_nco_log(atkgpu_nco *v, double *phases, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

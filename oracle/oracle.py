@@ -410,6 +410,42 @@ def pfb_channel_power(out, nel, nchan):
     return p, p.mean(axis=0)
 
 
+# ---- NCO: src/nco.c:27-98, src/internal.h:36-41 (twin python/atkdsp/reference.py:95-108) ----
+def wrap(ph: float) -> float:
+    """atk_wrap: fmod by 2 pi, then into [-pi, pi)."""
+    ph = math.fmod(ph, 2.0 * math.pi)
+    if ph >= math.pi:
+        ph -= 2.0 * math.pi
+    if ph < -math.pi:
+        ph += 2.0 * math.pi
+    return ph
+
+
+class Nco:
+    """out[n] = in[n] * exp(j (phase + n step)), step = -2 pi f / fs, the
+    phase carried wrapped in FP64 (the reference's float multiply is done
+    here in FP64, one rounding to complex64 at the end)."""
+
+    def __init__(self, freq_hz: float, fs: float):
+        self.phase = 0.0
+        self.step = -2.0 * math.pi * freq_hz / fs if fs > 0 else 0.0
+
+    def mix(self, x: np.ndarray) -> np.ndarray:
+        x = np.asarray(x, np.complex64)
+        if self.step == 0.0:
+            return x.copy()
+        n = np.arange(x.size, dtype=np.float64)
+        y = (x.astype(np.complex128) * np.exp(1j * (self.phase + self.step * n))).astype(np.complex64)
+        self.phase = wrap(self.phase + self.step * float(x.size))
+        return y
+
+
+def nco_exact_phase(n: int, p: int, q: int) -> float:
+    """The exact NCO phase after n samples at f = fs * p / q (step -2 pi p/q):
+    -2 pi ((n p) mod q) / q, wrapped: integer arithmetic, no accumulation."""
+    return wrap(-2.0 * math.pi * ((n * p) % q) / q)
+
+
 def rel_err(a, b) -> float:
     """max |a - b| / max |b| over the whole output (the reference's measure,
     tests/test_channelizer.py:88)."""

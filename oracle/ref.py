@@ -18,6 +18,10 @@ class Cf(C.Structure):
     _fields_ = [("re", C.c_float), ("im", C.c_float)]
 
 
+class NcoState(C.Structure):
+    _fields_ = [("phase", C.c_double), ("step", C.c_double)]
+
+
 def available() -> bool:
     return REF_SO.exists()
 
@@ -49,11 +53,29 @@ class Ref:
             "atkdsp_pfb_set_cal": (C.c_int, [vp, cf, u]),
             "atkdsp_pfb_cov_accumulate": (None, [cf, u, u, sz, P(C.c_uint8), cf, P(C.c_uint32)]),
             "atkdsp_pfb_channel_power": (None, [cf, u, u, sz, fp, fp]),
+            "atkdsp_nco_init": (None, [P(NcoState), d, d]),
+            "atkdsp_nco_mix": (None, [P(NcoState), cf, cf, sz]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
             f.restype, f.argtypes = res, args
         self.L = L
+
+    def nco_blocks(self, x: np.ndarray, f: float, fs: float, blocks):
+        """The reference NCO over x in consecutive blocks (sizes cycled);
+        returns (output, final phase)."""
+        st = NcoState()
+        self.L.atkdsp_nco_init(C.byref(st), f, fs)
+        out = np.empty_like(x)
+        a, k = 0, 0
+        while a < x.size:
+            b = min(x.size, a + blocks[k % len(blocks)])
+            xi = np.ascontiguousarray(x[a:b])
+            yi = np.empty_like(xi)
+            self.L.atkdsp_nco_mix(C.byref(st), self._cf(xi), self._cf(yi), xi.size)
+            out[a:b] = yi
+            a, k = b, k + 1
+        return out, st.phase
 
     @staticmethod
     def _cf(a):

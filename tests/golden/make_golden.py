@@ -195,5 +195,26 @@ def main():
     save("pfb", **d)
 
 
+def nco_golden():
+    """The reference NCO on noise in the block pattern of its own continuity
+    test (tests/test_cross_check.py:120-130), and a long tone run."""
+    r = Ref()
+    rng = np.random.default_rng(20261204)
+    x = noise(rng, 30000)
+    y, ph = r.nco_blocks(x, 173_125.0, 2.4e6, [1, 999, 4096, 7, 13000])
+    y1, ph1 = r.nco_blocks(x, 173_125.0, 2.4e6, [30000])
+    # 1,000,000 samples of a rational frequency (p/q of fs) in 65,536-sample calls
+    fs, p, q = 40e6, 1234, 10007
+    t = np.ones(1_000_000, np.complex64)
+    yt, pht = r.nco_blocks(t, fs * p / q, fs, [65536])
+    save("nco", x=x, y=y, phase=np.array(ph), y1=y1, phase1=np.array(ph1), tone_y_tail=yt[-4096:],
+         tone_phase=np.array(pht), pq=np.array([p, q]), fs=np.array(fs))
+
+
 if __name__ == "__main__":
-    main()
+    import sys
+    if sys.argv[1:] == ["nco"]:
+        nco_golden()
+    else:
+        main()
+        nco_golden()

@@ -61,7 +61,7 @@ def test_stubs_return_failure_values():
     f = lib.atkdsp_fir_create
     f.restype = C.c_void_p
     assert f() is None                                  # a *_create stub: NULL
-    g = lib.atkdsp_nco_mix
+    g = lib.atkdsp_arb_resampler_reset
     g.restype = None
     g()                                                 # void: nothing
     h = lib.atkdsp_median

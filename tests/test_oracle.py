@@ -278,3 +278,56 @@ def test_oracle_detects_planted_mistakes():
     p.h = p.h.copy()
     p.h[-192:] = 0
     assert O.rel_err(p.process(x)[:, :m], yref[:, :m]) > 1e-6
+
+
+# ---- NCO ---------------------------------------------------------------------------
+def _nco_blocks(nco, x, blocks):
+    out, a, k = [], 0, 0
+    while a < x.size:
+        b = min(x.size, a + blocks[k % len(blocks)])
+        out.append(nco.mix(x[a:b]))
+        a, k = b, k + 1
+    return np.concatenate(out)
+
+
+def test_nco_oracle_matches_the_reference():
+    """The reference's own continuity pattern (tests/test_cross_check.py:120-130)
+    against golden vectors written by the compiled reference."""
+    g = golden("nco")
+    o = O.Nco(173_125.0, 2.4e6)
+    y = _nco_blocks(o, g["x"], [1, 999, 4096, 7, 13000])
+    assert np.max(np.abs(y - g["y"])) < 1e-6
+    assert abs(o.phase - float(g["phase"])) < 1e-12
+    o1 = O.Nco(173_125.0, 2.4e6)
+    assert np.max(np.abs(o1.mix(g["x"]) - g["y1"])) < 1e-6
+    # and a long rational-frequency run: the reference's phasors are on the exact phase
+    p, q = (int(v) for v in g["pq"])
+    fs = float(g["fs"])
+    n_end = 1_000_000
+    exact = np.array([O.nco_exact_phase(n, p, q) for n in range(n_end - 4096, n_end)])
+    assert np.max(np.abs(g["tone_y_tail"] - np.exp(1j * exact))) < 1e-6
+    assert abs(O.wrap(float(g["tone_phase"]) - O.nco_exact_phase(n_end, p, q))) < 1e-8
+    ot = O.Nco(fs * p / q, fs)
+    _nco_blocks(ot, np.ones(n_end, np.complex64), [65536])
+    assert abs(O.wrap(ot.phase - O.nco_exact_phase(n_end, p, q))) < 1e-8
+
+
+def test_nco_known_answers():
+    """A tone at f0 mixed by an NCO at f0 is DC, continuously across blocks
+    (tests/test_smoke.c:60-76); zero frequency is the identity; the exact
+    phase closed form is periodic in q and linear in n."""
+    fs, f0, N = 1e6, 12345.0, 4096
+    n = np.arange(N)
+    x = (np.cos(2 * np.pi * f0 * n / fs) + 1j * np.sin(2 * np.pi * f0 * n / fs)).astype(np.complex64)
+    o = O.Nco(f0, fs)
+    y = np.concatenate([o.mix(x[: N // 2]), o.mix(x[N // 2:])])
+    assert np.max(np.abs(y.real - 1.0) + np.abs(y.imag)) < 2e-4
+    z = (np.arange(10) + 1j).astype(np.complex64)
+    assert np.array_equal(O.Nco(0.0, 1e6).mix(z), z)
+    p, q = 1234, 10007
+    assert O.nco_exact_phase(q, p, q) == 0.0
+    assert abs(O.wrap(O.nco_exact_phase(1, p, q) + 2 * np.pi * p / q)) < 1e-15
+    # a planted mistake (a conjugated mixer) fails the golden comparison
+    g = golden("nco")
+    o = O.Nco(-173_125.0, 2.4e6)
+    assert np.max(np.abs(o.mix(g["x"]) - g["y1"])) > 0.1
