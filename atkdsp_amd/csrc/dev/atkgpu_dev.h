@@ -188,11 +188,11 @@ int atkgpu_nco_run_dev(void *s, const void *in, void *out, size_t n, const doubl
 void *atkgpu_hmalloc_mapped(size_t bytes, void **dev_ptr);
 
 /* W1 / W2 over a device [nel][nchan][m] block */
-int atkgpu_pfb_cov(void *s, const void *y, unsigned nel, unsigned nchan, size_t m,
+int atkgpu_pfb_cov(void *s, const void *y, unsigned nel, unsigned nchan, size_t m, size_t ypitch,
                    const uint8_t *gate, void *R, uint32_t *counts);
 int atkgpu_pfb_cov_reset(void *s, void *R, uint32_t *counts, unsigned nel, unsigned nchan,
                          const uint8_t *gate);
-int atkgpu_pfb_power(void *s, const void *y, unsigned nel, unsigned nchan, size_t m,
+int atkgpu_pfb_power(void *s, const void *y, unsigned nel, unsigned nchan, size_t m, size_t ypitch,
                      float *pwr, float *mean_pwr);
 /* the plan's twiddle table (n complex floats, device) */
 const void *atkgpu_fftplan_twiddles(const atkgpu_fftplan *p);

@@ -105,8 +105,8 @@ extern "C" int atkgpu_pfb_rotate_store(void *s, const void *X, unsigned nel, siz
 // (src/channelizer.c:425-437), so R matches it to the bit.
 #define COV_TILE 64
 __global__ void __launch_bounds__(256)
-k_pfb_cov(const c2 *__restrict__ y, unsigned nel, unsigned nchan, size_t m, const uint8_t *__restrict__ gate,
-          c2 *__restrict__ R, uint32_t *__restrict__ counts) {
+k_pfb_cov(const c2 *__restrict__ y, unsigned nel, unsigned nchan, size_t m, size_t ypitch,
+          const uint8_t *__restrict__ gate, c2 *__restrict__ R, uint32_t *__restrict__ counts) {
     const unsigned k = blockIdx.x;
     if (gate && !gate[k]) return;
     __shared__ c2 ys[32][COV_TILE];
@@ -123,7 +123,7 @@ k_pfb_cov(const c2 *__restrict__ y, unsigned nel, unsigned nchan, size_t m, cons
             __syncthreads();
             for (unsigned idx = threadIdx.x; idx < nel * COV_TILE && nel <= 32; idx += 256) {
                 const unsigned el = idx / COV_TILE, ii = idx % COV_TILE;
-                if (ii < tn) ys[el][ii] = y[((size_t)el * nchan + k) * m + t0 + ii];
+                if (ii < tn) ys[el][ii] = y[((size_t)el * nchan + k) * ypitch + t0 + ii];
             }
             __syncthreads();
             if (pr < pairs && nel <= 32) {
@@ -134,8 +134,8 @@ k_pfb_cov(const c2 *__restrict__ y, unsigned nel, unsigned nchan, size_t m, cons
                 }
             } else if (pr < pairs) {
                 for (unsigned ii = 0; ii < tn; ++ii) {
-                    const c2 ya = y[((size_t)a * nchan + k) * m + t0 + ii];
-                    const c2 yb = y[((size_t)b * nchan + k) * m + t0 + ii];
+                    const c2 ya = y[((size_t)a * nchan + k) * ypitch + t0 + ii];
+                    const c2 yb = y[((size_t)b * nchan + k) * ypitch + t0 + ii];
                     rr += ya.x * yb.x + ya.y * yb.y;
                     ri += ya.y * yb.x - ya.x * yb.y;
                 }
@@ -155,33 +155,49 @@ __global__ void k_pfb_cov_reset(c2 *__restrict__ R, uint32_t *__restrict__ count
     if (threadIdx.x == 0) counts[k] = 0;
 }
 
-// ---- W2: per (element, channel) mean |y|^2 summed in FP64, hops ascending ------
-// one thread a channel, elements and hops in the reference's order
-// (src/channelizer.c:453-471)
-__global__ void k_pfb_power(const c2 *__restrict__ y, unsigned nel, unsigned nchan, size_t m,
-                            float *__restrict__ pwr, float *__restrict__ mean_pwr) {
-    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nchan) return;
+// ---- W2: per (element, channel) mean |y|^2 summed in FP64 (src/channelizer.c:453-471)
+// One block a channel, one wavefront an element: lanes sum hops i = lane,
+// lane + 64, ... in FP64, a wavefront reduction in FP64 finishes the sum,
+// then the element means are averaged in element order as the reference
+// does. Only the order of the FP64 hop sum differs from the reference's
+// ascending loop (a change of about 1e-16 relative, before rounding to float).
+__global__ void __launch_bounds__(1024) k_pfb_power(const c2 *__restrict__ y, unsigned nel, unsigned nchan,
+                                                    size_t m, size_t ypitch, float *__restrict__ pwr,
+                                                    float *__restrict__ mean_pwr) {
+    const unsigned k = blockIdx.x;
+    const unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    __shared__ double es[16];
     const double inv = m ? 1.0 / (double)m : 0.0;
-    double msum = 0.0;
-    for (unsigned a = 0; a < nel; ++a) {
-        const c2 *ya = y + ((size_t)a * nchan + k) * m;
-        double s = 0.0;
-        for (size_t i = 0; i < m; ++i) {
-            const c2 v = ya[i];
-            s += (double)v.x * v.x + (double)v.y * v.y;
+    double msum = 0.0;                         // thread 0's, in element order
+    for (unsigned a0 = 0; a0 < nel; a0 += nw) {
+        const unsigned a = a0 + w;
+        if (a < nel) {
+            const c2 *ya = y + ((size_t)a * nchan + k) * ypitch;
+            double s = 0.0;
+            for (size_t i = lane; i < m; i += 64) {
+                const c2 v = ya[i];
+                s += (double)v.x * v.x + (double)v.y * v.y;
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+            if (lane == 0) {
+                es[w] = s * inv;
+                if (pwr) pwr[(size_t)a * nchan + k] = (float)(s * inv);
+            }
         }
-        if (pwr) pwr[(size_t)a * nchan + k] = (float)(s * inv);
-        msum += s * inv;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (unsigned e = a0; e < nel && e < a0 + nw; ++e) msum += es[e - a0];
+        __syncthreads();
     }
-    if (mean_pwr) mean_pwr[k] = (float)(msum / (double)nel);
+    if (threadIdx.x == 0 && mean_pwr) mean_pwr[k] = (float)(msum / (double)nel);
 }
 
-extern "C" int atkgpu_pfb_cov(void *s, const void *y, unsigned nel, unsigned nchan, size_t m,
+extern "C" int atkgpu_pfb_cov(void *s, const void *y, unsigned nel, unsigned nchan, size_t m, size_t ypitch,
                               const uint8_t *gate, void *R, uint32_t *counts) {
     if (!nchan) return 0;
     hipLaunchKernelGGL(k_pfb_cov, dim3(nchan), dim3(256), 0, (hipStream_t)s, (const c2 *)y, nel, nchan, m,
-                       gate, (c2 *)R, counts);
+                       ypitch, gate, (c2 *)R, counts);
     return LAUNCH_OK();
 }
 
@@ -193,10 +209,11 @@ extern "C" int atkgpu_pfb_cov_reset(void *s, void *R, uint32_t *counts, unsigned
     return LAUNCH_OK();
 }
 
-extern "C" int atkgpu_pfb_power(void *s, const void *y, unsigned nel, unsigned nchan, size_t m,
+extern "C" int atkgpu_pfb_power(void *s, const void *y, unsigned nel, unsigned nchan, size_t m, size_t ypitch,
                                 float *pwr, float *mean_pwr) {
     if (!nchan) return 0;
-    hipLaunchKernelGGL(k_pfb_power, dim3((nchan + 127) / 128), dim3(128), 0, (hipStream_t)s,
-                       (const c2 *)y, nel, nchan, m, pwr, mean_pwr);
+    if (nel < 1) return -1;
+    hipLaunchKernelGGL(k_pfb_power, dim3(nchan), dim3(64 * (nel < 16 ? nel : 16)), 0, (hipStream_t)s,
+                       (const c2 *)y, nel, nchan, m, ypitch, pwr, mean_pwr);
     return LAUNCH_OK();
 }

@@ -342,7 +342,7 @@ void atkdsp_pfb_cov_accumulate(const atkdsp_cf32 *out, unsigned nel, unsigned nc
         if (!rc) rc = atkgpu_h2d(s->stream, d_R, R + (size_t)k0 * pairs, rbytes);
         if (!rc) rc = atkgpu_h2d(s->stream, d_cnt, counts + k0, nk * sizeof(uint32_t));
         if (!rc && gate) rc = atkgpu_h2d(s->stream, d_gate, gate + k0, nk);
-        if (!rc) rc = atkgpu_pfb_cov(s->stream, s->d_a, nel, nk, m, d_gate, d_R, d_cnt);
+        if (!rc) rc = atkgpu_pfb_cov(s->stream, s->d_a, nel, nk, m, m, d_gate, d_R, d_cnt);
         if (!rc) rc = atkgpu_d2h(s->stream, R + (size_t)k0 * pairs, d_R, rbytes);
         if (!rc) rc = atkgpu_d2h(s->stream, counts + k0, d_cnt, nk * sizeof(uint32_t));
         if (!rc) rc = atkgpu_sync(s->stream);
@@ -379,7 +379,7 @@ void atkdsp_pfb_channel_power(const atkdsp_cf32 *out, unsigned nel, unsigned nch
         for (unsigned e = 0; e < nel && !rc; ++e)
             rc = atkgpu_h2d(s->stream, (atkdsp_cf32 *)s->d_a + (size_t)e * nk * m,
                             out + ((size_t)e * nchan + k0) * m, (size_t)nk * m * sizeof(atkdsp_cf32));
-        if (!rc) rc = atkgpu_pfb_power(s->stream, s->d_a, nel, nk, m, d_p, d_mp);
+        if (!rc) rc = atkgpu_pfb_power(s->stream, s->d_a, nel, nk, m, m, d_p, d_mp);
         for (unsigned e = 0; e < nel && !rc && pwr; ++e)
             rc = atkgpu_d2h(s->stream, pwr + (size_t)e * nchan + k0, d_p + (size_t)e * nk, nk * sizeof(float));
         if (!rc && mean_pwr) rc = atkgpu_d2h(s->stream, mean_pwr + k0, d_mp, nk * sizeof(float));

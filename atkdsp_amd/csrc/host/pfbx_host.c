@@ -27,7 +27,7 @@ struct atkgpu_pfbx {
     int kos, bps, mx;
     size_t keep, spitch, m_cap;
     void *stream;
-    void *ev[4];
+    void *ev[5];
     unsigned char *d_raw;      /* FP32 form: [S][block_max*bps] upload blocks */
     unsigned char *d_rows[2];  /* FP32 form: cf32 rows; MX forms: raw rows; [S][spitch] samples */
     int cur;
@@ -36,6 +36,9 @@ struct atkgpu_pfbx {
     float scale;
     atkgpu_fftplan *plan;
     atkdsp_cf32 *d_out;        /* [S][M][m_cap] */
+    atkdsp_cf32 *d_R;          /* coherent: [M][S][S] */
+    uint32_t *d_cnt;           /* coherent: [M] */
+    float *d_pwr, *d_mean;     /* coherent: [S][M], [M] */
     void *desc_h[2], *desc_d[2];
     size_t desc_n[2];          /* block size each descriptor set was built for (0: none) */
     unsigned lc;
@@ -59,9 +62,20 @@ void atkgpu_pfbx_destroy(atkgpu_pfbx *v) {
     atkgpu_dfree(v->d_out);
     for (int i = 0; i < 2; ++i) atkgpu_hfree(v->desc_h[i]);
     atkgpu_fftplan_destroy(v->plan);
-    for (int i = 0; i < 4; ++i) atkgpu_event_destroy(v->ev[i]);
+    atkgpu_dfree(v->d_R);
+    atkgpu_dfree(v->d_cnt);
+    atkgpu_dfree(v->d_pwr);
+    atkgpu_dfree(v->d_mean);
+    for (int i = 0; i < 5; ++i) atkgpu_event_destroy(v->ev[i]);
     if (v->stream) atkgpu_stream_destroy(v->stream);
     free(v);
+}
+
+int atkgpu_pfbx_cov_reset(atkgpu_pfbx *v) {
+    if (!v || !v->cfg.coherent) return ATKDSP_E_ARG;
+    int rc = atkgpu_pfb_cov_reset(v->stream, v->d_R, v->d_cnt, v->S, v->M, NULL);
+    if (!rc) rc = atkgpu_sync(v->stream);
+    return rc ? ATKDSP_E_FFT : ATKDSP_OK;
 }
 
 int atkgpu_pfbx_reset(atkgpu_pfbx *v) {
@@ -77,6 +91,7 @@ int atkgpu_pfbx_reset(atkgpu_pfbx *v) {
     int rc = 0;
     const size_t row_bytes = (size_t)v->S * v->spitch * (v->mx ? (size_t)v->bps : sizeof(atkdsp_cf32));
     for (int i = 0; i < 2 && !rc; ++i) rc = atkgpu_fill_u32(v->stream, v->d_rows[i], 0u, row_bytes / 4);
+    if (!rc && v->cfg.coherent) rc = atkgpu_pfb_cov_reset(v->stream, v->d_R, v->d_cnt, v->S, v->M, NULL);
     if (!rc) rc = atkgpu_sync(v->stream);
     return rc ? ATKDSP_E_FFT : ATKDSP_OK;
 }
@@ -107,7 +122,14 @@ atkgpu_pfbx *atkgpu_pfbx_create(const atkgpu_pfbx_cfg *cfg, unsigned S) {
     v->m_cap = round16((cfg->block_max + v->hop) / v->hop + 1);
     const size_t ntap = (size_t)kos * v->hop;
     int ok = !atkgpu_stream_create(&v->stream);
-    for (int i = 0; ok && i < 4; ++i) ok = !atkgpu_event_create(&v->ev[i]);
+    for (int i = 0; ok && i < 5; ++i) ok = !atkgpu_event_create(&v->ev[i]);
+    if (ok && cfg->coherent) {
+        v->d_R = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)v->M * S * S * sizeof(atkdsp_cf32));
+        v->d_cnt = (uint32_t *)atkgpu_dmalloc((size_t)v->M * sizeof(uint32_t));
+        v->d_pwr = (float *)atkgpu_dmalloc((size_t)S * v->M * sizeof(float));
+        v->d_mean = (float *)atkgpu_dmalloc((size_t)v->M * sizeof(float));
+        ok = v->d_R && v->d_cnt && v->d_pwr && v->d_mean;
+    }
     const size_t row_bytes = (size_t)S * v->spitch * (mx ? (size_t)bps : sizeof(atkdsp_cf32));
     if (ok) {
         v->d_rows[0] = (unsigned char *)atkgpu_dmalloc(row_bytes);
@@ -265,13 +287,18 @@ ptrdiff_t atkgpu_pfbx_run(atkgpu_pfbx *v, size_t n) {
                                   (unsigned)v->kos, v->d_taps, tw, NULL, 0, v->d_out, mp, 0);
     }
     if (!rc) rc = atkgpu_event_record(v->ev[2], v->stream);
+    if (!rc && m && v->cfg.coherent) {
+        rc = atkgpu_pfb_cov(v->stream, v->d_out, v->S, v->M, m, mp, NULL, v->d_R, v->d_cnt);
+        if (!rc) rc = atkgpu_pfb_power(v->stream, v->d_out, v->S, v->M, m, mp, v->d_pwr, v->d_mean);
+    }
+    if (!rc) rc = atkgpu_event_record(v->ev[3], v->stream);
     if (!rc && n) {
         int cblk = 0;
         for (unsigned s = 0; s < v->S; ++s) cblk += cj[s].nblk;
         rc = v->mx ? atkgpu_copy_bytes_jobs(v->stream, DEV(cj), (int)v->S, cblk)
                    : atkgpu_copy_jobs(v->stream, DEV(cj), (int)v->S, cblk);
     }
-    if (!rc) rc = atkgpu_event_record(v->ev[3], v->stream);
+    if (!rc) rc = atkgpu_event_record(v->ev[4], v->stream);
 #undef DEV
     if (rc) return ATKDSP_E_FFT;
     v->lc = (unsigned)(avail - m * v->hop);
@@ -287,16 +314,35 @@ int atkgpu_pfbx_sync(atkgpu_pfbx *v) {
     return atkgpu_sync(v->stream) ? ATKDSP_E_FFT : ATKDSP_OK;
 }
 
-int atkgpu_pfbx_timing(atkgpu_pfbx *v, float *ms_run, float *ms_unpack, float *ms_pfb) {
+int atkgpu_pfbx_timing(atkgpu_pfbx *v, float *ms_run, float *ms_unpack, float *ms_pfb, float *ms_w) {
     if (!v || !v->ran) return ATKDSP_E_ARG;
-    float a = 0, b = 0, c = 0;
-    if (atkgpu_event_elapsed_ms(v->ev[0], v->ev[3], &a) || atkgpu_event_elapsed_ms(v->ev[0], v->ev[1], &b) ||
-        atkgpu_event_elapsed_ms(v->ev[1], v->ev[2], &c))
+    float a = 0, b = 0, c = 0, d = 0;
+    if (atkgpu_event_elapsed_ms(v->ev[0], v->ev[4], &a) || atkgpu_event_elapsed_ms(v->ev[0], v->ev[1], &b) ||
+        atkgpu_event_elapsed_ms(v->ev[1], v->ev[2], &c) || atkgpu_event_elapsed_ms(v->ev[2], v->ev[3], &d))
         return ATKDSP_E_FFT;
     if (ms_run) *ms_run = a;
     if (ms_unpack) *ms_unpack = b;
     if (ms_pfb) *ms_pfb = c;
+    if (ms_w) *ms_w = d;
     return ATKDSP_OK;
+}
+
+int atkgpu_pfbx_read_cov(atkgpu_pfbx *v, atkdsp_cf32 *R, uint32_t *counts) {
+    if (!v || !v->cfg.coherent) return ATKDSP_E_ARG;
+    int rc = 0;
+    if (R) rc = atkgpu_d2h(v->stream, R, v->d_R, (size_t)v->M * v->S * v->S * sizeof(atkdsp_cf32));
+    if (!rc && counts) rc = atkgpu_d2h(v->stream, counts, v->d_cnt, (size_t)v->M * sizeof(uint32_t));
+    if (!rc) rc = atkgpu_sync(v->stream);
+    return rc ? ATKDSP_E_FFT : ATKDSP_OK;
+}
+
+int atkgpu_pfbx_read_power(atkgpu_pfbx *v, float *pwr, float *mean_pwr) {
+    if (!v || !v->cfg.coherent || !v->last_m) return ATKDSP_E_ARG;
+    int rc = 0;
+    if (pwr) rc = atkgpu_d2h(v->stream, pwr, v->d_pwr, (size_t)v->S * v->M * sizeof(float));
+    if (!rc && mean_pwr) rc = atkgpu_d2h(v->stream, mean_pwr, v->d_mean, (size_t)v->M * sizeof(float));
+    if (!rc) rc = atkgpu_sync(v->stream);
+    return rc ? ATKDSP_E_FFT : ATKDSP_OK;
 }
 
 ptrdiff_t atkgpu_pfbx_read(atkgpu_pfbx *v, unsigned s, atkdsp_cf32 *out, size_t cap) {

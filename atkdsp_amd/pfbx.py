@@ -26,7 +26,7 @@ FORMS = {"fp32": 0, "bf16x2": 1, "bf16": 2, "fp16x2": 3, "fp16": 4}
 class _Cfg(C.Structure):
     _fields_ = [("fmt", C.c_int), ("fs", C.c_double), ("nchan", C.c_uint), ("oversample", C.c_uint),
                 ("taps_per_poly", C.c_uint), ("atten_db", C.c_double), ("form", C.c_int),
-                ("block_max", C.c_size_t), ("off_re", C.c_float), ("off_im", C.c_float)]
+                ("block_max", C.c_size_t), ("off_re", C.c_float), ("off_im", C.c_float), ("coherent", C.c_int)]
 
 
 def _bind(lib) -> None:
@@ -41,7 +41,10 @@ def _bind(lib) -> None:
         "atkgpu_pfbx_upload": (C.c_int, [vp, C.c_uint, vp, sz]),
         "atkgpu_pfbx_run": (C.c_ssize_t, [vp, sz]),
         "atkgpu_pfbx_sync": (C.c_int, [vp]),
-        "atkgpu_pfbx_timing": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_float)]),
+        "atkgpu_pfbx_timing": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float)]),
+        "atkgpu_pfbx_read_cov": (C.c_int, [vp, P(_Cf32), P(C.c_uint32)]),
+        "atkgpu_pfbx_read_power": (C.c_int, [vp, P(C.c_float), P(C.c_float)]),
+        "atkgpu_pfbx_cov_reset": (C.c_int, [vp]),
         "atkgpu_pfbx_read": (C.c_ssize_t, [vp, C.c_uint, P(_Cf32), sz]),
     }
     for name, (res, args) in sig.items():
@@ -52,13 +55,13 @@ def _bind(lib) -> None:
 
 class PfbX:
     def __init__(self, form="bf16x2", fmt="cu8", fs=2.4e6, nchan=64, oversample=1, taps_per_poly=8,
-                 atten_db=80.0, streams=1, block_max=1 << 20, offset: complex = 0j):
+                 atten_db=80.0, streams=1, block_max=1 << 20, offset: complex = 0j, coherent=False):
         self._lib = load()
         _bind(self._lib)
         if form not in FORMS:
             raise AtkDspError(f"unknown form {form!r}")
         cfg = _Cfg(_fmt(fmt), float(fs), int(nchan), int(oversample), int(taps_per_poly), float(atten_db),
-                   FORMS[form], int(block_max), float(offset.real), float(offset.imag))
+                   FORMS[form], int(block_max), float(offset.real), float(offset.imag), int(bool(coherent)))
         self._h = self._lib.atkgpu_pfbx_create(C.byref(cfg), int(streams))
         if not self._h:
             raise AtkDspError(f"atkgpu_pfbx_create failed (no GPU, or form {form} does not support this "
@@ -89,10 +92,30 @@ class PfbX:
         _check(self._lib.atkgpu_pfbx_sync(self._h), "pfbx_sync")
 
     def timing(self):
-        """(ms of the whole run, ms of the unpack, ms of the filterbank kernel) of the last run."""
-        a, b, c = C.c_float(), C.c_float(), C.c_float()
-        _check(self._lib.atkgpu_pfbx_timing(self._h, C.byref(a), C.byref(b), C.byref(c)), "pfbx_timing")
-        return a.value, b.value, c.value
+        """(ms of the whole run, of the unpack, of the filterbank kernel, of W1 + W2) of the last run."""
+        a, b, c, d = C.c_float(), C.c_float(), C.c_float(), C.c_float()
+        _check(self._lib.atkgpu_pfbx_timing(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)),
+               "pfbx_timing")
+        return a.value, b.value, c.value, d.value
+
+    def read_cov(self):
+        """W1 accumulated so far: R [nchan][nel][nel] complex64, counts [nchan] uint32."""
+        R = np.empty((self.nchan, self.streams, self.streams), np.complex64)
+        cnt = np.empty(self.nchan, np.uint32)
+        _check(self._lib.atkgpu_pfbx_read_cov(self._h, R.ctypes.data_as(C.POINTER(_Cf32)),
+                                              cnt.ctypes.data_as(C.POINTER(C.c_uint32))), "pfbx_read_cov")
+        return R, cnt
+
+    def read_power(self):
+        """W2 of the last run: pwr [nel][nchan], mean_pwr [nchan] float32."""
+        p = np.empty((self.streams, self.nchan), np.float32)
+        mp = np.empty(self.nchan, np.float32)
+        _check(self._lib.atkgpu_pfbx_read_power(self._h, p.ctypes.data_as(C.POINTER(C.c_float)),
+                                                mp.ctypes.data_as(C.POINTER(C.c_float))), "pfbx_read_power")
+        return p, mp
+
+    def cov_reset(self) -> None:
+        _check(self._lib.atkgpu_pfbx_cov_reset(self._h), "pfbx_cov_reset")
 
     def read(self, s: int) -> np.ndarray:
         out = np.empty(self.nchan * self._m_cap, np.complex64)

@@ -92,6 +92,43 @@ def precision_forms(S: int = 256, n: int = 1_200_000, steps: int = 10):
                     "history copy); errors against the FP64 oracle are in tests/test_gpu_precision.py"}
 
 
+def coherent_array(n: int = 1_000_000, nel: int = 8, steps: int = 10):
+    """The coherent 8-element form: eight phase-coherent cf32 elements at
+    20 MSPS, one block of n samples each per run, resident in HBM, through
+    one filterbank launch (2048 channels, 12 taps/branch, 2x), then W1 and W2
+    on the device output."""
+    from atkdsp_amd import synth
+    from atkdsp_amd.pfbx import PfbX
+    els = synth.c5_elements(n=n, nel=nel)
+    px = PfbX(form="fp32", fmt="cf32", fs=20e6, nchan=2048, oversample=2, taps_per_poly=12, streams=nel,
+              block_max=n, coherent=True)
+    for e in range(nel):
+        px.upload(e, els[e].tobytes())
+    px.sync()
+    for _ in range(2):
+        px.run(n)
+    px.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m = px.run(n)
+    px.sync()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    ev = []
+    for _ in range(3):
+        px.run(n)
+        ev.append(px.timing())
+    run_ms, unpack_ms, pfb_ms, w_ms = (float(np.mean([e[i] for e in ev])) for i in range(4))
+    px.close()
+    # compulsory bytes of the filterbank launch: every element's input and history once, output once
+    pfb_bytes = nel * ((n + 2048 * 12 - 1024) * 8 + m * 2048 * 8)
+    return {"workload": f"{nel} elements x {n} cf32 samples (20 MSPS), PFB 2048 ch, 12 taps/branch, 2x, W1 + W2",
+            "ms_per_run": round(wall, 4), "device_ms": round(run_ms, 4), "unpack_ms": round(unpack_ms, 4),
+            "filterbank_ms": round(pfb_ms, 4), "w1_w2_ms": round(w_ms, 4), "hops": m,
+            "Msamples_per_s": round(nel * n / (wall * 1e-3) / 1e6, 1),
+            "filterbank_GBps": round(pfb_bytes / (pfb_ms * 1e-3) / 1e9, 1),
+            "realtime_x": round((n / 20e6) / (wall * 1e-3), 1)}
+
+
 def nco_phase_run(seconds: float = 10.0, fs: float = 40e6, p: int = 1234, q: int = 10007, chunk: int = 65536):
     """10 s of 40 MSPS through the device-resident NCO in 65,536-sample runs,
     f = fs * p / q: the FP64 phase after every run against the exact phase
@@ -282,6 +319,7 @@ def main() -> int:
     if not args.no_extras:
         extras["precision_form"] = precision_forms()
         extras["nco_phase_run"] = nco_phase_run()
+        extras["coherent_array"] = coherent_array()
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, streams[0])
     uni = serve.get("uniform", {})
     out = {

@@ -144,6 +144,10 @@ typedef struct {
     int form;                  /* ATKGPU_FORM_* */
     size_t block_max;          /* samples per stream per run, a multiple of 8 */
     float off_re, off_im;      /* the fixed offset subtracted by the unpack */
+    int coherent;              /* 1: the streams are phase-coherent elements of one array; every
+                                  run also accumulates W1 (per-channel cross-spectra over all
+                                  elements, atkdsp_pfb_cov_accumulate's arithmetic) and computes
+                                  W2 (atkdsp_pfb_channel_power) from the device output */
 } atkgpu_pfbx_cfg;
 typedef struct atkgpu_pfbx atkgpu_pfbx;
 ATKGPU_API atkgpu_pfbx *atkgpu_pfbx_create(const atkgpu_pfbx_cfg *cfg, unsigned nstreams);
@@ -155,10 +159,17 @@ ATKGPU_API int          atkgpu_pfbx_upload(atkgpu_pfbx *v, unsigned s, const voi
 /* one block of n samples for every stream; returns the hops produced */
 ATKGPU_API ptrdiff_t    atkgpu_pfbx_run(atkgpu_pfbx *v, size_t n);
 ATKGPU_API int          atkgpu_pfbx_sync(atkgpu_pfbx *v);
-/* device times of the last run: whole run, the unpack (FP32 form only) and the filterbank kernel */
-ATKGPU_API int          atkgpu_pfbx_timing(atkgpu_pfbx *v, float *ms_run, float *ms_unpack, float *ms_pfb);
+/* device times of the last run: whole run, the unpack (FP32 form only), the filterbank kernel,
+ * and W1 + W2 (coherent only) */
+ATKGPU_API int          atkgpu_pfbx_timing(atkgpu_pfbx *v, float *ms_run, float *ms_unpack, float *ms_pfb,
+                                           float *ms_w);
 /* stream s's last output, [nchan][m] */
 ATKGPU_API ptrdiff_t    atkgpu_pfbx_read(atkgpu_pfbx *v, unsigned s, atkdsp_cf32 *out, size_t cap);
+/* coherent arrays: W1 accumulated since create / cov_reset, R [nchan][nel][nel] and counts
+ * [nchan]; W2 of the last run, pwr [nel][nchan] and mean_pwr [nchan] (either may be NULL) */
+ATKGPU_API int          atkgpu_pfbx_read_cov(atkgpu_pfbx *v, atkdsp_cf32 *R, uint32_t *counts);
+ATKGPU_API int          atkgpu_pfbx_read_power(atkgpu_pfbx *v, float *pwr, float *mean_pwr);
+ATKGPU_API int          atkgpu_pfbx_cov_reset(atkgpu_pfbx *v);
 
 /* ---- the device-resident NCO: a long run fed from device buffers ---------------
  * The FP64 phase lives in device memory across runs (two slots used in turn)
