@@ -1,21 +1,24 @@
 /* The serving loop (include/atkgpu.h): many independent receiver streams,
  * each with its own format and filterbank shape, sending chunks of any size.
  *
- *   admission   a stream's device state (two history rows, taps, an output
- *               ring) is allocated when it is added and charged against a
- *               fixed HBM budget; a stream that does not fit is refused.
+ *   admission   a stream's device state (two history rows per element, taps,
+ *               an output ring, and for a coherent array its W1/W2) is
+ *               allocated when it is added and charged against a fixed HBM
+ *               budget; a stream that does not fit is refused.
  *   submit      reserves a ticket and staging space under the lock, copies the
  *               chunk's bytes into the pinned staging ring (which the device
  *               reads directly, mapped) outside it, then publishes the chunk.
- *               Submitters of different streams copy in parallel. It
- *               allocates nothing and makes no HIP call.
+ *               Submitters of different streams copy in parallel. Each element
+ *               of an array chunk starts on a 64-byte boundary. It allocates
+ *               nothing and makes no HIP call.
  *   dispatcher  a thread forms batches continuously: at most one chunk per
  *               stream (each stream's chunks stay in order), taken round-robin
  *               over the streams that have work, while earlier batches still
- *               run. A batch is one ragged unpack launch (every format), one
- *               fused filterbank launch per (M, os, taps) class and one history
- *               launch, all on one HIP stream, so a stream's chunks execute in
- *               order.
+ *               run. A batch is one ragged unpack launch (every format and
+ *               element), one fused filterbank launch per (M, os, taps) class,
+ *               a W1 and a W2 launch per coherent array stream, and one
+ *               history launch, all on one HIP stream, so a stream's chunks
+ *               execute in order.
  *   completion  a thread polls each batch's event in order (sleeping a few
  *               microseconds between polls, so no core is held spinning),
  *               stamps the host monotonic clock and records each chunk's
@@ -41,11 +44,13 @@ int atkgpu_pfb_prototype(double *h, unsigned nchan, unsigned K, double atten_db)
 #define POLL_NS 10000          /* completion poll interval */
 #define QCAP 256               /* pending chunks per stream */
 #define LAT_CAP (1u << 22)
+#define NEL_MAX 16             /* elements of an array stream */
 
 typedef struct {
     long long ticket;
     size_t off;                /* staging byte offset (monotonic) */
-    size_t nbytes, n;
+    size_t nbytes, n;          /* staged bytes, samples per element */
+    size_t estride;            /* staging bytes between elements */
     double t_submit;
     int ready;                 /* bytes are in staging: the dispatcher may take it */
 } chunk_t;
@@ -53,14 +58,17 @@ typedef struct {
 typedef struct {
     int used, removing;
     atkgpu_stream_cfg cfg;
-    unsigned M, os, K, hop, ov, keep;
+    unsigned M, os, K, hop, ov, keep, nel;
     int kos, bps;
     size_t m_max, row_len, bytes;
-    atkdsp_cf32 *rows[2];
+    atkdsp_cf32 *rows[2];      /* each [nel][row_len] */
     int cur;
     float *taps;
     const void *tw;
-    atkdsp_cf32 *out;
+    atkdsp_cf32 *out;          /* [out_slots][nel][M][m_max] */
+    atkdsp_cf32 *R;            /* coherent: [M][nel][nel] */
+    uint32_t *cnt;             /* coherent: [M] */
+    float *pwr, *mean;         /* coherent: [nel][M], [M] */
     unsigned lc;
     long long b;
     long long submitted, dispatched, completed;
@@ -74,13 +82,16 @@ typedef struct {
 typedef struct {
     int sid, fmt;
     chunk_t c;
-    unsigned M, os, kos, ov, keep, lc;
-    size_t m;
+    unsigned M, os, kos, ov, keep, lc, nel;
+    size_t m, row_len, m_max;
     long long b0;
-    atkdsp_cf32 *row, *next, *out;
+    atkdsp_cf32 *row, *next, *out;     /* element 0; element e at + e*row_len (rows), + e*M*m_max (out) */
     const float *taps;
     const void *tw;
     float off_re, off_im;
+    atkdsp_cf32 *R;
+    uint32_t *cnt;
+    float *pwr, *mean;
 } bitem;
 
 typedef struct {
@@ -97,7 +108,7 @@ typedef struct {
 
 struct atkgpu_server {
     size_t budget, used_bytes;
-    unsigned max_streams, max_batch;
+    unsigned max_streams, max_batch, jobcap;   /* jobcap: element jobs per batch */
     sstream *st;
     unsigned char *stage_h, *stage_d;
     size_t stage_cap, stage_head, stage_tail;    /* monotonic byte counters */
@@ -124,15 +135,19 @@ static double now_s(void) {
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+static size_t round16(size_t x) { return (x + 15) & ~(size_t)15; }
+static size_t round64(size_t x) { return (x + 63) & ~(size_t)63; }
+
 static int plan_index(unsigned M) {
     for (int i = 0; i < 6; ++i)
         if (M == (64u << i)) return i;
     return -1;
 }
 
-/* descriptor buffer layout for one batch: unpack jobs, pfb jobs, copy jobs */
-static size_t desc_bytes(unsigned max_batch) {
-    return (size_t)max_batch * (sizeof(atkgpu_unpack_job) + sizeof(atkgpu_pfb_job) + sizeof(atkgpu_copy_job)) + 256;
+/* descriptor buffer layout for one batch: unpack, pfb and copy jobs, one of
+ * each per element of each chunk */
+static size_t desc_bytes(unsigned jobcap) {
+    return (size_t)jobcap * (sizeof(atkgpu_unpack_job) + sizeof(atkgpu_pfb_job) + sizeof(atkgpu_copy_job)) + 256;
 }
 
 static void *disp_main(void *arg);
@@ -146,6 +161,7 @@ atkgpu_server *atkgpu_server_create(size_t hbm_budget, unsigned max_streams, uns
     v->budget = hbm_budget;
     v->max_streams = max_streams;
     v->max_batch = max_batch;
+    v->jobcap = max_batch + 4 * NEL_MAX;        /* room for array streams beside single ones */
     v->st = (sstream *)calloc(max_streams, sizeof(sstream));
     v->stage_cap = staging & ~(size_t)63;
     v->stage_h = (unsigned char *)atkgpu_hmalloc_mapped(staging, (void **)&v->stage_d);
@@ -158,7 +174,7 @@ atkgpu_server *atkgpu_server_create(size_t hbm_budget, unsigned max_streams, uns
     for (int i = 0; ok && i < NINFLIGHT; ++i) {
         batch_t *b = &v->batch[i];
         ok = !atkgpu_event_create(&b->event) && (b->items = (bitem *)calloc(max_batch, sizeof(bitem))) &&
-             (b->desc_h = atkgpu_hmalloc_mapped(desc_bytes(max_batch), &b->desc_d));
+             (b->desc_h = atkgpu_hmalloc_mapped(desc_bytes(v->jobcap), &b->desc_d));
     }
     if (!ok) { atkgpu_server_destroy(v); return NULL; }
     pthread_mutex_init(&v->mu, NULL);
@@ -177,6 +193,10 @@ static void stream_free(sstream *s) {
     atkgpu_dfree(s->rows[1]);
     atkgpu_dfree(s->taps);
     atkgpu_dfree(s->out);
+    atkgpu_dfree(s->R);
+    atkgpu_dfree(s->cnt);
+    atkgpu_dfree(s->pwr);
+    atkgpu_dfree(s->mean);
     free(s->slot_m);
     memset(s, 0, sizeof *s);
 }
@@ -217,16 +237,21 @@ int atkgpu_server_add_stream(atkgpu_server *v, const atkgpu_stream_cfg *cfg) {
     if (!v || !cfg) return ATKDSP_E_ARG;
     const int bps = atkdsp_bytes_per_sample(cfg->fmt);
     const int kos = atkgpu_pfb_fused_kos(cfg->nchan, cfg->oversample, cfg->taps_per_poly);
-    if (!bps || !kos || cfg->taps_per_poly < 2 || cfg->chunk_max < 1 || cfg->out_slots < 1 || !(cfg->fs > 0))
+    const unsigned nel = cfg->nel ? cfg->nel : 1;
+    if (!bps || !kos || cfg->taps_per_poly < 2 || cfg->chunk_max < 1 || cfg->out_slots < 1 || !(cfg->fs > 0) ||
+        nel > NEL_MAX || (cfg->coherent && nel < 2))
         return ATKDSP_E_ARG;
     const unsigned M = cfg->nchan, os = cfg->oversample, K = cfg->taps_per_poly;
     const unsigned hop = M / os, ov = M * K - hop;
     const unsigned keep = ov + hop;
     /* output rows padded to 16 hops (128 bytes): whole sectors per segment */
-    const size_t m_max = (((cfg->chunk_max + hop) / hop + 1) + 15) & ~(size_t)15;
+    const size_t m_max = round16((cfg->chunk_max + hop) / hop + 1);
     const size_t row_len = (size_t)keep + cfg->chunk_max;
-    const size_t bytes = 2 * row_len * sizeof(atkdsp_cf32) + (size_t)kos * hop * sizeof(float) +
-                         (size_t)cfg->out_slots * M * m_max * sizeof(atkdsp_cf32);
+    size_t bytes = 2 * (size_t)nel * row_len * sizeof(atkdsp_cf32) + (size_t)kos * hop * sizeof(float) +
+                   (size_t)cfg->out_slots * nel * M * m_max * sizeof(atkdsp_cf32);
+    if (cfg->coherent)
+        bytes += (size_t)M * nel * nel * sizeof(atkdsp_cf32) + (size_t)M * sizeof(uint32_t) +
+                 (size_t)(nel + 1) * M * sizeof(float);
     pthread_mutex_lock(&v->mu);
     int sid = -1;
     for (unsigned i = 0; i < v->max_streams; ++i)
@@ -242,23 +267,33 @@ int atkgpu_server_add_stream(atkgpu_server *v, const atkgpu_stream_cfg *cfg) {
     pthread_mutex_unlock(&v->mu);
 
     s->cfg = *cfg;
-    s->M = M; s->os = os; s->K = K; s->hop = hop; s->ov = ov; s->keep = keep;
+    s->cfg.nel = nel;
+    s->M = M; s->os = os; s->K = K; s->hop = hop; s->ov = ov; s->keep = keep; s->nel = nel;
     s->kos = kos; s->bps = bps; s->m_max = m_max; s->row_len = row_len; s->bytes = bytes;
     s->tw = atkgpu_fftplan_twiddles(v->plans[plan_index(M)]);
-    s->rows[0] = (atkdsp_cf32 *)atkgpu_dmalloc(row_len * sizeof(atkdsp_cf32));
-    s->rows[1] = (atkdsp_cf32 *)atkgpu_dmalloc(row_len * sizeof(atkdsp_cf32));
+    s->rows[0] = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)nel * row_len * sizeof(atkdsp_cf32));
+    s->rows[1] = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)nel * row_len * sizeof(atkdsp_cf32));
     s->taps = (float *)atkgpu_dmalloc((size_t)kos * hop * sizeof(float));
-    s->out = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)cfg->out_slots * M * m_max * sizeof(atkdsp_cf32));
+    s->out = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)cfg->out_slots * nel * M * m_max * sizeof(atkdsp_cf32));
     s->slot_m = (size_t *)calloc(cfg->out_slots, sizeof(size_t));
+    int rc = (!s->rows[0] || !s->rows[1] || !s->taps || !s->out || !s->slot_m) ? ATKDSP_E_NOMEM : 0;
+    if (!rc && cfg->coherent) {
+        s->R = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)M * nel * nel * sizeof(atkdsp_cf32));
+        s->cnt = (uint32_t *)atkgpu_dmalloc((size_t)M * sizeof(uint32_t));
+        s->pwr = (float *)atkgpu_dmalloc((size_t)nel * M * sizeof(float));
+        s->mean = (float *)atkgpu_dmalloc((size_t)M * sizeof(float));
+        if (!s->R || !s->cnt || !s->pwr || !s->mean) rc = ATKDSP_E_NOMEM;
+    }
     double *h = (double *)malloc((size_t)M * K * sizeof(double));
     float *hf = (float *)calloc((size_t)kos * hop, sizeof(float));
-    int rc = (!s->rows[0] || !s->rows[1] || !s->taps || !s->out || !s->slot_m || !h || !hf) ? ATKDSP_E_NOMEM : 0;
+    if (!rc && (!h || !hf)) rc = ATKDSP_E_NOMEM;
     if (!rc) rc = atkgpu_pfb_prototype(h, M, K, cfg->atten_db);
     if (!rc) {
         for (size_t q = 0; q < (size_t)M * K; ++q) hf[q] = (float)h[q];
         /* history rows start at zero (the reference callocs its history) */
         rc = atkgpu_h2d(v->stream, s->taps, hf, (size_t)kos * hop * sizeof(float));
-        if (!rc) rc = atkgpu_fill_u32(v->stream, s->rows[0], 0, row_len * 2);
+        if (!rc) rc = atkgpu_fill_u32(v->stream, s->rows[0], 0, (size_t)nel * row_len * 2);
+        if (!rc && cfg->coherent) rc = atkgpu_pfb_cov_reset(v->stream, s->R, s->cnt, nel, M, NULL);
         if (!rc) rc = atkgpu_sync(v->stream);
     }
     free(h);
@@ -290,8 +325,8 @@ int atkgpu_server_remove_stream(atkgpu_server *v, int sid) {
 static int stage_alloc(atkgpu_server *v, size_t nbytes, size_t *off) {
     /* every chunk starts 64-byte aligned, so the device's vector loads of any
      * format are naturally aligned */
-    nbytes = (nbytes + 63) & ~(size_t)63;
-    size_t head = (v->stage_head + 63) & ~(size_t)63;
+    nbytes = round64(nbytes);
+    size_t head = round64(v->stage_head);
     const size_t pos = head % v->stage_cap;
     if (pos + nbytes > v->stage_cap) head += v->stage_cap - pos;       /* skip to the ring start */
     if (head + nbytes - v->stage_tail > v->stage_cap) return 0;
@@ -324,12 +359,15 @@ long long atkgpu_server_submit(atkgpu_server *v, int sid, const void *raw, size_
     pthread_mutex_lock(&v->mu);
     sstream *s = &v->st[sid];
     if (!s->used || s->removing || v->failed) { pthread_mutex_unlock(&v->mu); return v->failed ? ATKDSP_E_FFT : ATKDSP_E_ARG; }
-    const size_t n = nbytes / (size_t)s->bps;
-    if (n > s->cfg.chunk_max || n * (size_t)s->bps + 64 > v->stage_cap) {
+    /* an array chunk: nel element blocks of n samples, element-major */
+    const size_t n = nbytes / ((size_t)s->bps * s->nel);
+    const size_t eb = n * (size_t)s->bps;             /* bytes of one element */
+    const size_t estride = s->nel > 1 ? round64(eb) : eb;
+    const size_t nb = estride * (s->nel - 1) + eb;
+    if (n > s->cfg.chunk_max || nb + 64 > v->stage_cap) {
         pthread_mutex_unlock(&v->mu);
         return ATKDSP_E_ARG;
     }
-    const size_t nb = n * (size_t)s->bps;
     size_t off = 0;
     /* backpressure: wait for a queue entry and staging space */
     while (v->running && !v->failed && (s->qn >= QCAP || !stage_alloc(v, nb ? nb : 1, &off)))
@@ -342,14 +380,17 @@ long long atkgpu_server_submit(atkgpu_server *v, int sid, const void *raw, size_
     c->off = off;
     c->nbytes = nb;
     c->n = n;
+    c->estride = estride;
     c->t_submit = t;
     c->ready = 0;              /* queued in ticket order, not yet takeable */
     s->qn++;
     if (!v->chunks && !v->nlat) v->t_first = t;
+    const unsigned nel = s->nel;
     pthread_mutex_unlock(&v->mu);
     /* the copy runs unlocked; the queue entry and staging span are this
      * call's alone until it is marked ready */
-    if (nb) memcpy(v->stage_h + (off % v->stage_cap), raw, nb);
+    unsigned char *dst = v->stage_h + (off % v->stage_cap);
+    for (unsigned e = 0; e < nel && eb; ++e) memcpy(dst + e * estride, (const unsigned char *)raw + e * eb, eb);
     pthread_mutex_lock(&v->mu);
     c->ready = 1;
     pthread_cond_signal(&v->cv_work);
@@ -357,37 +398,40 @@ long long atkgpu_server_submit(atkgpu_server *v, int sid, const void *raw, size_
     return ticket;
 }
 
-/* Launch one batch (the mutex is not held): ragged unpack per format, fused
- * filterbank per (M, os, kos) class, then every stream's kept prefix. */
+/* Launch one batch (the mutex is not held): ragged unpack of every element,
+ * fused filterbank per (M, os, kos) class, W1/W2 per coherent array, then
+ * every element's kept prefix. */
 static int launch_batch(atkgpu_server *v, batch_t *bt) {
     const int n = bt->n;
     atkgpu_unpack_job *uj = (atkgpu_unpack_job *)bt->desc_h;
-    atkgpu_pfb_job *pj = (atkgpu_pfb_job *)(uj + v->max_batch);
-    atkgpu_copy_job *cj = (atkgpu_copy_job *)(pj + v->max_batch);
+    atkgpu_pfb_job *pj = (atkgpu_pfb_job *)(uj + v->jobcap);
+    atkgpu_copy_job *cj = (atkgpu_copy_job *)(pj + v->jobcap);
     char *dbase = (char *)bt->desc_d, *hbase = (char *)bt->desc_h;
 #define DEV(p) ((void *)(dbase + ((char *)(p) - hbase)))
     int rc = 0;
-    /* unpack, every format in one launch (staging offsets are 64-byte
-     * aligned and keep is a multiple of M >= 64, so both ends are 16-byte
-     * aligned for the kernel's word loads and stores) */
+    /* unpack, every format and element in one launch (staging offsets are
+     * 64-byte aligned and keep is a multiple of M >= 64, so both ends are
+     * 16-byte aligned for the kernel's word loads and stores) */
     int nu = 0, ublk = 0;
     for (int i = 0; i < n; ++i) {
         const bitem *it = &bt->items[i];
         if (it->c.n == 0) continue;
-        atkgpu_unpack_job *j = &uj[nu++];
-        j->raw = v->stage_d + (it->c.off % v->stage_cap);
-        j->dst = it->row + it->keep;
-        j->off_re = it->off_re;
-        j->off_im = it->off_im;
-        j->n = (int)it->c.n;
-        j->blk0 = ublk;
-        j->nblk = (j->n + ATKGPU_RAGGED_SPB - 1) / ATKGPU_RAGGED_SPB;
-        j->fmt = it->fmt;
-        ublk += j->nblk;
-        if ((((size_t)j->raw) | ((size_t)j->dst)) & 15) rc = ATKDSP_E_ARG;
+        for (unsigned e = 0; e < it->nel; ++e) {
+            atkgpu_unpack_job *j = &uj[nu++];
+            j->raw = v->stage_d + (it->c.off % v->stage_cap) + e * it->c.estride;
+            j->dst = it->row + e * it->row_len + it->keep;
+            j->off_re = it->off_re;
+            j->off_im = it->off_im;
+            j->n = (int)it->c.n;
+            j->blk0 = ublk;
+            j->nblk = (j->n + ATKGPU_RAGGED_SPB - 1) / ATKGPU_RAGGED_SPB;
+            j->fmt = it->fmt;
+            ublk += j->nblk;
+            if ((((size_t)j->raw) | ((size_t)j->dst)) & 15) rc = ATKDSP_E_ARG;
+        }
     }
     if (!rc && nu) rc = atkgpu_unpack_jobs(v->stream, DEV(uj), nu, ublk);
-    /* the filterbank, one launch per shape class present */
+    /* the filterbank, one launch per shape class present, one job per element */
     int np = 0;
     unsigned char done[4096];
     memset(done, 0, sizeof done);
@@ -402,32 +446,44 @@ static int launch_batch(atkgpu_server *v, batch_t *bt) {
             if (done[k] || it->M != a->M || it->os != a->os || it->kos != a->kos) continue;
             done[k] = 1;
             if (!it->m) continue;
-            atkgpu_pfb_job *j = &pj[np++];
-            j->s = it->row + (it->keep - it->ov - it->lc);
-            j->out = it->out;
-            j->taps = it->taps;
-            j->b0 = it->b0;
-            j->nvalid = (int)(it->ov + it->lc + it->c.n);
-            j->m = (int)it->m;
-            j->opitch = (int)((it->m + 15) & ~(size_t)15);
-            j->blk0 = blk;
-            j->nblk = (int)((it->m + G - 1) / G);
-            j->pad_ = 0;
-            blk += j->nblk;
+            for (unsigned e = 0; e < it->nel; ++e) {
+                atkgpu_pfb_job *j = &pj[np++];
+                j->s = it->row + e * it->row_len + (it->keep - it->ov - it->lc);
+                j->out = it->out + (size_t)e * it->M * it->m_max;
+                j->taps = it->taps;
+                j->b0 = it->b0;
+                j->nvalid = (int)(it->ov + it->lc + it->c.n);
+                j->m = (int)it->m;
+                j->opitch = (int)it->m_max;
+                j->blk0 = blk;
+                j->nblk = (int)((it->m + G - 1) / G);
+                j->pad_ = 0;
+                blk += j->nblk;
+            }
         }
         if (np > p0) rc = atkgpu_pfb_fused_jobs(v->stream, DEV(&pj[p0]), np - p0, blk, a->M, a->os, a->kos, a->tw);
+    }
+    /* W1 accumulated and W2 of this chunk, per coherent array: the element
+     * outputs sit [nel][M][m_max] in the slot */
+    for (int i = 0; i < n && !rc; ++i) {
+        const bitem *it = &bt->items[i];
+        if (!it->R || !it->m) continue;
+        rc = atkgpu_pfb_cov(v->stream, it->out, it->nel, it->M, it->m, it->m_max, NULL, it->R, it->cnt);
+        if (!rc) rc = atkgpu_pfb_power(v->stream, it->out, it->nel, it->M, it->m, it->m_max, it->pwr, it->mean);
     }
     /* kept prefix: the last keep samples of [row, row + keep + n) to the next row */
     int nc = 0, blk = 0;
     for (int i = 0; i < n; ++i) {
         const bitem *it = &bt->items[i];
-        atkgpu_copy_job *j = &cj[nc++];
-        j->src = it->row + it->c.n;
-        j->dst = it->next;
-        j->n = (int)it->keep;
-        j->blk0 = blk;
-        j->nblk = (j->n + ATKGPU_RAGGED_SPB - 1) / ATKGPU_RAGGED_SPB;
-        blk += j->nblk;
+        for (unsigned e = 0; e < it->nel; ++e) {
+            atkgpu_copy_job *j = &cj[nc++];
+            j->src = it->row + e * it->row_len + it->c.n;
+            j->dst = it->next + e * it->row_len;
+            j->n = (int)it->keep;
+            j->blk0 = blk;
+            j->nblk = (j->n + ATKGPU_RAGGED_SPB - 1) / ATKGPU_RAGGED_SPB;
+            blk += j->nblk;
+        }
     }
     if (!rc && nc) rc = atkgpu_copy_jobs(v->stream, DEV(cj), nc, blk);
 #undef DEV
@@ -435,17 +491,21 @@ static int launch_batch(atkgpu_server *v, batch_t *bt) {
     return rc;
 }
 
-/* take up to max_batch chunks, one per stream, round-robin; mutex held */
+/* take up to max_batch chunks (and jobcap element jobs), one per stream,
+ * round-robin; mutex held */
 static int form_batch(atkgpu_server *v, batch_t *bt) {
     int n = 0;
+    unsigned jobs = 0;
     const unsigned S = v->max_streams;
     for (unsigned t = 0; t < S && n < (int)v->max_batch && n < 4096; ++t) {
         const unsigned sid = (v->rr + t) % S;
         sstream *s = &v->st[sid];
         if (!s->used || !s->qn || !s->q[s->qh].ready) continue;
+        if (jobs + s->nel > v->jobcap) continue;
         const chunk_t c = s->q[s->qh];
         s->qh = (s->qh + 1) % QCAP;
         s->qn--;
+        jobs += s->nel;
         bitem *it = &bt->items[n++];
         it->sid = (int)sid;
         it->fmt = s->cfg.fmt;
@@ -456,16 +516,23 @@ static int form_batch(atkgpu_server *v, batch_t *bt) {
         it->ov = s->ov;
         it->keep = s->keep;
         it->lc = s->lc;
+        it->nel = s->nel;
+        it->row_len = s->row_len;
+        it->m_max = s->m_max;
         it->b0 = s->b;
         it->m = ((size_t)s->lc + c.n) / s->hop;
         it->row = s->rows[s->cur];
         it->next = s->rows[s->cur ^ 1];
         const unsigned slot = (unsigned)(c.ticket % s->cfg.out_slots);
-        it->out = s->out + (size_t)slot * s->M * s->m_max;
+        it->out = s->out + (size_t)slot * s->nel * s->M * s->m_max;
         it->taps = s->taps;
         it->tw = s->tw;
         it->off_re = s->cfg.off_re;
         it->off_im = s->cfg.off_im;
+        it->R = s->R;
+        it->cnt = s->cnt;
+        it->pwr = s->pwr;
+        it->mean = s->mean;
         /* advance the stream */
         s->slot_m[slot] = it->m;
         s->lc = (unsigned)(((size_t)s->lc + c.n) - it->m * s->hop);
@@ -536,7 +603,7 @@ static void *comp_main(void *arg) {
             s->completed++;
             if (v->nlat < LAT_CAP) v->lat[v->nlat++] = (t - it->c.t_submit) * 1e3;
             v->chunks++;
-            v->samples += (long long)it->c.n;
+            v->samples += (long long)it->c.n * it->nel;
             stage_release(v, it->c.off);
         }
         v->t_last = t;
@@ -568,14 +635,45 @@ ptrdiff_t atkgpu_server_read(atkgpu_server *v, int sid, long long ticket, atkdsp
     /* the slot still holds this ticket's result until out_slots later chunks run */
     if (ticket + (long long)s->cfg.out_slots < s->dispatched) { pthread_mutex_unlock(&v->mu); return ATKDSP_E_ARG; }
     const unsigned slot = (unsigned)(ticket % s->cfg.out_slots);
-    const size_t m = s->slot_m[slot];
-    const atkdsp_cf32 *src = s->out + (size_t)slot * s->M * s->m_max;
-    const unsigned M = s->M;
+    const size_t m = s->slot_m[slot], m_max = s->m_max;
+    const unsigned M = s->M, nel = s->nel;
+    const atkdsp_cf32 *src = s->out + (size_t)slot * nel * M * m_max;
     pthread_mutex_unlock(&v->mu);
-    if (cap < (size_t)M * m) return ATKDSP_E_CAP;
-    rc = atkgpu_d2h_2d_sync(out, m * sizeof(atkdsp_cf32), src, ((m + 15) & ~(size_t)15) * sizeof(atkdsp_cf32),
-                            m * sizeof(atkdsp_cf32), M);
+    if (cap < (size_t)nel * M * m) return ATKDSP_E_CAP;
+    /* [nel][M] rows of m hops, from device rows of m_max */
+    rc = atkgpu_d2h_2d_sync(out, m * sizeof(atkdsp_cf32), src, m_max * sizeof(atkdsp_cf32), m * sizeof(atkdsp_cf32),
+                            (size_t)nel * M);
     return rc ? ATKDSP_E_FFT : (ptrdiff_t)m;
+}
+
+int atkgpu_server_read_cov(atkgpu_server *v, int sid, atkdsp_cf32 *R, uint32_t *counts) {
+    if (!v || sid < 0 || (unsigned)sid >= v->max_streams) return ATKDSP_E_ARG;
+    pthread_mutex_lock(&v->mu);
+    sstream *s = &v->st[sid];
+    if (!s->used || !s->R) { pthread_mutex_unlock(&v->mu); return ATKDSP_E_ARG; }
+    while (s->completed < s->submitted && !v->failed) pthread_cond_wait(&v->cv_done, &v->mu);
+    const size_t nR = (size_t)s->M * s->nel * s->nel, M = s->M;
+    atkdsp_cf32 *dR = s->R;
+    uint32_t *dc = s->cnt;
+    const int failed = v->failed;
+    pthread_mutex_unlock(&v->mu);
+    if (failed) return ATKDSP_E_FFT;
+    int rc = R ? atkgpu_d2h_sync(R, dR, nR * sizeof(atkdsp_cf32)) : 0;
+    if (!rc && counts) rc = atkgpu_d2h_sync(counts, dc, M * sizeof(uint32_t));
+    return rc ? ATKDSP_E_FFT : ATKDSP_OK;
+}
+
+int atkgpu_server_read_power(atkgpu_server *v, int sid, float *pwr, float *mean_pwr) {
+    if (!v || sid < 0 || (unsigned)sid >= v->max_streams) return ATKDSP_E_ARG;
+    pthread_mutex_lock(&v->mu);
+    sstream *s = &v->st[sid];
+    if (!s->used || !s->pwr || !s->completed) { pthread_mutex_unlock(&v->mu); return ATKDSP_E_ARG; }
+    const size_t M = s->M, nel = s->nel;
+    float *dp = s->pwr, *dm = s->mean;
+    pthread_mutex_unlock(&v->mu);
+    int rc = pwr ? atkgpu_d2h_sync(pwr, dp, nel * M * sizeof(float)) : 0;
+    if (!rc && mean_pwr) rc = atkgpu_d2h_sync(mean_pwr, dm, M * sizeof(float));
+    return rc ? ATKDSP_E_FFT : ATKDSP_OK;
 }
 
 int atkgpu_server_drain(atkgpu_server *v) {

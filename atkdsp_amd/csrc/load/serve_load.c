@@ -13,9 +13,12 @@
  *
  * With --mix uniform every stream is instead 40 MSPS ci16 into a 1024-channel,
  * 16-tap, 2x oversampled filterbank (the full-rate single-receiver shape).
+ * --arrays A adds A coherent arrays of 8 cf32 elements at 20 MSPS into
+ * 2048-channel, 12-tap, 2x filterbanks with W1 / W2 on the device; a chunk
+ * holds all 8 elements.
  *
  * usage: serve_load [--seconds S] [--streams N] [--speed X] [--budget-gib G]
- *                   [--max-batch B] [--seed K] [--mix c4|uniform]
+ *                   [--max-batch B] [--seed K] [--mix c4|uniform] [--arrays A]
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -32,6 +35,7 @@
 
 typedef struct {
     int sid, fmt, bps;
+    unsigned nel;
     double fs;
     unsigned M, K;
     unsigned char *src;        /* SRC_SAMPLES samples of this stream's format */
@@ -63,7 +67,7 @@ static double urand(unsigned long long *x) { return ((double)(splitmix(x) >> 11)
 
 /* noise at -30 dBFS plus a few tones, quantised to the stream's format */
 static void make_source(lstream *s) {
-    s->src = (unsigned char *)malloc((size_t)SRC_SAMPLES * s->bps);
+    s->src = (unsigned char *)malloc((size_t)SRC_SAMPLES * s->bps * s->nel);
     const double sig = 0.0224;
     double pr[3], pi[3], wr[3], wi[3], a[3];
     for (int i = 0; i < 3; ++i) {
@@ -82,6 +86,12 @@ static void make_source(lstream *s) {
             const double nr = pr[i] * wr[i] - pi[i] * wi[i];
             pi[i] = pr[i] * wi[i] + pi[i] * wr[i];
             pr[i] = nr;
+        }
+        for (unsigned e = 1; e < s->nel; ++e) {       /* array elements: the same wave, phase-shifted */
+            const double c = cos(0.7 * e), sn = sin(0.7 * e);
+            float *p = (float *)(s->src + (size_t)e * SRC_SAMPLES * 8);
+            p[2 * n] = (float)(re * c - im * sn);
+            p[2 * n + 1] = (float)(re * sn + im * c);
         }
         switch (s->fmt) {
         case ATKDSP_FMT_CU8:
@@ -111,7 +121,7 @@ static void make_source(lstream *s) {
 
 static void *submitter(void *arg) {
     const int tid = (int)(long)arg;
-    unsigned char *tmp = (unsigned char *)malloc((size_t)CHUNK_MAX * 8);
+    unsigned char *tmp = (unsigned char *)malloc((size_t)CHUNK_MAX * 8 * 8);
     for (;;) {
         /* the earliest next arrival among this thread's streams */
         int best = -1;
@@ -129,14 +139,19 @@ static void *submitter(void *arg) {
             nanosleep(&ts, NULL);
         }
         const size_t n = 1 + (size_t)(splitmix(&s->rng) % CHUNK_MAX);
-        const size_t bytes = n * (size_t)s->bps;
+        const size_t bytes = n * (size_t)s->bps * s->nel;
         const unsigned char *p;
-        if (s->pos + n <= SRC_SAMPLES) {
+        if (s->nel == 1 && s->pos + n <= SRC_SAMPLES) {
             p = s->src + s->pos * s->bps;
         } else {
-            const size_t a = SRC_SAMPLES - s->pos;
-            memcpy(tmp, s->src + s->pos * s->bps, a * s->bps);
-            memcpy(tmp + a * s->bps, s->src, (n - a) * s->bps);
+            /* element-major chunk: each element's n samples from its own source */
+            for (unsigned e = 0; e < s->nel; ++e) {
+                const unsigned char *src = s->src + (size_t)e * SRC_SAMPLES * s->bps;
+                unsigned char *d = tmp + (size_t)e * n * s->bps;
+                const size_t a = SRC_SAMPLES - s->pos < n ? SRC_SAMPLES - s->pos : n;
+                memcpy(d, src + s->pos * s->bps, a * s->bps);
+                if (a < n) memcpy(d + a * s->bps, src, (n - a) * s->bps);
+            }
             p = tmp;
         }
         s->pos = (s->pos + n) % SRC_SAMPLES;
@@ -154,7 +169,7 @@ int main(int argc, char **argv) {
     double seconds = 5.0, speed = 1.0, budget_gib = 32.0;
     int streams = 128, max_batch = 256;
     unsigned long long seed = 4000;
-    int uniform = 0;
+    int uniform = 0, arrays = 0;
     for (int i = 1; i + 1 < argc; i += 2) {
         if (!strcmp(argv[i], "--seconds")) seconds = atof(argv[i + 1]);
         else if (!strcmp(argv[i], "--streams")) streams = atoi(argv[i + 1]);
@@ -163,18 +178,20 @@ int main(int argc, char **argv) {
         else if (!strcmp(argv[i], "--max-batch")) max_batch = atoi(argv[i + 1]);
         else if (!strcmp(argv[i], "--seed")) seed = strtoull(argv[i + 1], NULL, 10);
         else if (!strcmp(argv[i], "--mix")) uniform = !strcmp(argv[i + 1], "uniform");
+        else if (!strcmp(argv[i], "--arrays")) arrays = atoi(argv[i + 1]);
     }
-    g_srv = atkgpu_server_create((size_t)(budget_gib * (1u << 30)), (unsigned)streams, (unsigned)max_batch,
-                                 (size_t)512 << 20);
+    g_srv = atkgpu_server_create((size_t)(budget_gib * (1u << 30)), (unsigned)(streams + arrays),
+                                 (unsigned)max_batch, (size_t)1024 << 20);
     if (!g_srv) { fprintf(stderr, "serve_load: no server (no GPU?)\n"); return 2; }
     static const double rates[4] = {2.4e6, 10e6, 20e6, 40e6};
     static const int fmts[4] = {ATKDSP_FMT_CU8, ATKDSP_FMT_CI8, ATKDSP_FMT_CI16, ATKDSP_FMT_CF32};
     static const unsigned Ms[3] = {64, 256, 1024};
-    g_st = (lstream *)calloc(streams, sizeof(lstream));
+    g_st = (lstream *)calloc(streams + arrays, sizeof(lstream));
     int admitted = 0, refused = 0;
     double total_rate = 0;
-    for (int s = 0; s < streams; ++s) {
+    for (int s = 0; s < streams + arrays; ++s) {
         lstream *ls = &g_st[admitted];
+        ls->nel = 1;
         ls->rng = seed + (unsigned long long)s;
         ls->fs = rates[s % 4];
         ls->fmt = fmts[(s / 4) % 4];
@@ -188,13 +205,22 @@ int main(int argc, char **argv) {
             ls->M = 1024;
             ls->K = 16;
         }
-        atkgpu_stream_cfg cfg = {ls->fmt, ls->fs, ls->M, 2, ls->K, 80.0, 0.0f, 0.0f, CHUNK_MAX, 2};
+        if (s >= streams) {                          /* a coherent 8-element array */
+            ls->fs = 20e6;
+            ls->fmt = ATKDSP_FMT_CF32;
+            ls->bps = 8;
+            ls->M = 2048;
+            ls->K = 12;
+            ls->nel = 8;
+        }
+        atkgpu_stream_cfg cfg = {ls->fmt, ls->fs, ls->M, 2, ls->K, 80.0, 0.0f, 0.0f, CHUNK_MAX, 2, ls->nel,
+                                 ls->nel > 1};
         const int sid = atkgpu_server_add_stream(g_srv, &cfg);
         if (sid < 0) { ++refused; continue; }
         ls->sid = sid;
         make_source(ls);
         ls->next_t = -log(urand(&ls->rng)) * ((CHUNK_MAX + 1) / 2.0) / (ls->fs * speed);
-        total_rate += ls->fs;
+        total_rate += ls->fs * ls->nel;
         ++admitted;
     }
     g_ns = admitted;
@@ -209,11 +235,11 @@ int main(int argc, char **argv) {
     atkgpu_server_drain(g_srv);
     atkgpu_serve_stats st;
     atkgpu_server_stats(g_srv, &st, 0);
-    printf("{\"mix\": \"%s\", \"streams\": %d, \"refused\": %d, \"offered_msps\": %.2f, \"speed\": %.2f, \"seconds\": %.2f, "
+    printf("{\"mix\": \"%s\", \"arrays\": %d, \"streams\": %d, \"refused\": %d, \"offered_msps\": %.2f, \"speed\": %.2f, \"seconds\": %.2f, "
            "\"chunks\": %lld, \"samples\": %lld, \"batches\": %lld, \"msamples_per_s\": %.2f, "
            "\"p50_ms\": %.4f, \"p99_ms\": %.4f, \"p999_ms\": %.4f, \"max_ms\": %.4f, \"mean_ms\": %.4f, "
            "\"hbm_used_gib\": %.3f, \"hbm_budget_gib\": %.1f}\n",
-           uniform ? "uniform" : "c4", admitted, refused, total_rate / 1e6, speed, seconds, st.chunks, st.samples, st.batches,
+           uniform ? "uniform" : "c4", arrays, admitted, refused, total_rate / 1e6, speed, seconds, st.chunks, st.samples, st.batches,
            st.busy_s > 0 ? st.samples / st.busy_s / 1e6 : 0.0, st.p50_ms, st.p99_ms, st.p999_ms, st.max_ms,
            st.mean_ms, st.hbm_used / 1073741824.0, st.hbm_budget / 1073741824.0);
     atkgpu_server_destroy(g_srv);

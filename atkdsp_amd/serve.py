@@ -18,7 +18,8 @@ from . import AtkDspError, _Cf32, _check, _fmt, load
 class _StreamCfg(C.Structure):
     _fields_ = [("fmt", C.c_int), ("fs", C.c_double), ("nchan", C.c_uint), ("oversample", C.c_uint),
                 ("taps_per_poly", C.c_uint), ("atten_db", C.c_double), ("off_re", C.c_float),
-                ("off_im", C.c_float), ("chunk_max", C.c_size_t), ("out_slots", C.c_uint)]
+                ("off_im", C.c_float), ("chunk_max", C.c_size_t), ("out_slots", C.c_uint), ("nel", C.c_uint),
+                ("coherent", C.c_int)]
 
 
 class _Stats(C.Structure):
@@ -41,6 +42,8 @@ def _bind(lib) -> None:
         "atkgpu_server_wait": (C.c_int, [vp, C.c_int, C.c_longlong]),
         "atkgpu_server_read": (C.c_ssize_t, [vp, C.c_int, C.c_longlong, P(_Cf32), sz]),
         "atkgpu_server_drain": (C.c_int, [vp]),
+        "atkgpu_server_read_cov": (C.c_int, [vp, C.c_int, P(_Cf32), P(C.c_uint32)]),
+        "atkgpu_server_read_power": (C.c_int, [vp, C.c_int, P(C.c_float), P(C.c_float)]),
         "atkgpu_server_stats": (C.c_int, [vp, P(_Stats), C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -69,12 +72,15 @@ class Server:
         self.close()
 
     def add_stream(self, fmt="ci16", fs=40e6, nchan=1024, oversample=2, taps_per_poly=16, atten_db=80.0,
-                   offset: complex = 0j, chunk_max=65536, out_slots=4) -> int:
+                   offset: complex = 0j, chunk_max=65536, out_slots=4, nel=1, coherent=False) -> int:
+        """nel > 1: an array of phase-coherent elements (a chunk is nel element
+        blocks, element-major); coherent: W1 / W2 accumulated on the device."""
         cfg = _StreamCfg(_fmt(fmt), float(fs), int(nchan), int(oversample), int(taps_per_poly), float(atten_db),
-                         float(offset.real), float(offset.imag), int(chunk_max), int(out_slots))
+                         float(offset.real), float(offset.imag), int(chunk_max), int(out_slots), int(nel),
+                         int(bool(coherent)))
         sid = self._lib.atkgpu_server_add_stream(self._h, C.byref(cfg))
         _check(sid, "server_add_stream")
-        self._nchan[sid] = int(nchan)
+        self._nchan[sid] = (int(nchan), int(nel))
         return sid
 
     def try_add_stream(self, **kw):
@@ -97,11 +103,31 @@ class Server:
         _check(self._lib.atkgpu_server_wait(self._h, int(sid), int(ticket)), "server_wait")
 
     def read(self, sid: int, ticket: int, max_hops: int = 1 << 17) -> np.ndarray:
-        M = self._nchan[sid]
-        out = np.empty(M * max_hops, np.complex64)
+        """[nchan][m] for a single stream, [nel][nchan][m] for an array."""
+        M, nel = self._nchan[sid]
+        out = np.empty(nel * M * max_hops, np.complex64)
         m = _check(self._lib.atkgpu_server_read(self._h, int(sid), int(ticket),
                                                 out.ctypes.data_as(C.POINTER(_Cf32)), out.size), "server_read")
-        return out[: M * m].reshape(M, m)
+        y = out[: nel * M * m]
+        return y.reshape(M, m) if nel == 1 else y.reshape(nel, M, m)
+
+    def read_cov(self, sid: int):
+        """W1 of a coherent array over every chunk submitted so far: R [nchan][nel][nel], counts [nchan]."""
+        M, nel = self._nchan[sid]
+        R = np.empty((M, nel, nel), np.complex64)
+        cnt = np.empty(M, np.uint32)
+        _check(self._lib.atkgpu_server_read_cov(self._h, int(sid), R.ctypes.data_as(C.POINTER(_Cf32)),
+                                                cnt.ctypes.data_as(C.POINTER(C.c_uint32))), "server_read_cov")
+        return R, cnt
+
+    def read_power(self, sid: int):
+        """W2 of a coherent array's latest completed chunk: pwr [nel][nchan], mean_pwr [nchan]."""
+        M, nel = self._nchan[sid]
+        p = np.empty((nel, M), np.float32)
+        mp = np.empty(M, np.float32)
+        _check(self._lib.atkgpu_server_read_power(self._h, int(sid), p.ctypes.data_as(C.POINTER(C.c_float)),
+                                                  mp.ctypes.data_as(C.POINTER(C.c_float))), "server_read_power")
+        return p, mp
 
     def drain(self) -> None:
         _check(self._lib.atkgpu_server_drain(self._h), "server_drain")

@@ -18,7 +18,7 @@
  *   - NaN means "unmeasured" (no fast-math anywhere).
  *
  * The library also exports every other name of the reference ABI (334 in
- * all) as a documented stub: *_create returns NULL, signed counts return
+ * all; 38 are implemented here) as a documented stub: *_create returns NULL, signed counts return
  * ATKDSP_E_ARG, unsigned counts 0, floating results NaN, void functions do
  * nothing. Those are outside the wideband front end.
  */

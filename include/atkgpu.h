@@ -85,15 +85,24 @@ ATKGPU_API long long     atkgpu_chain_frames(const atkgpu_chain *c);   /* frames
  * order. The offset (off_re, off_im) is fixed per stream, so the output does
  * not depend on how the stream is chunked or what else is in a batch.
  * Latency = completion of the chunk's batch minus its submit call, both on
- * the host monotonic clock. */
+ * the host monotonic clock.
+ *
+ * A stream may be an array of nel phase-coherent elements sharing the
+ * prototype: a chunk then holds nel element blocks of n samples each,
+ * element-major, and its result is [nel][nchan][m]. A coherent stream also
+ * accumulates W1 (per-channel nel x nel cross-spectra,
+ * atkdsp_pfb_cov_accumulate's arithmetic) over all its chunks and keeps W2
+ * (atkdsp_pfb_channel_power) of its latest chunk, in device memory. */
 typedef struct {
     int fmt;                    /* enum atkdsp_format */
     double fs;
     unsigned nchan, oversample, taps_per_poly;  /* nchan 64..2048 (power of 2), os 1 or 2, K*os <= 32 */
     double atten_db;
     float off_re, off_im;       /* offset removed at unpack */
-    size_t chunk_max;           /* largest chunk, samples */
+    size_t chunk_max;           /* largest chunk, samples (per element) */
     unsigned out_slots;         /* chunk results kept per stream */
+    unsigned nel;               /* elements per chunk, 1..16 (0 means 1) */
+    int coherent;               /* W1 / W2 for an array stream */
 } atkgpu_stream_cfg;
 
 typedef struct {
@@ -121,6 +130,11 @@ ATKGPU_API int       atkgpu_server_wait(atkgpu_server *v, int sid, long long tic
  * until out_slots later chunks of the stream have been dispatched. */
 ATKGPU_API ptrdiff_t atkgpu_server_read(atkgpu_server *v, int sid, long long ticket, atkdsp_cf32 *out, size_t cap);
 ATKGPU_API int       atkgpu_server_drain(atkgpu_server *v);
+/* a coherent stream's W1 over every chunk submitted so far (waits for them):
+ * R [nchan][nel][nel], counts [nchan]; and W2 of its latest completed chunk,
+ * pwr [nel][nchan], mean_pwr [nchan]. Either output may be NULL. */
+ATKGPU_API int       atkgpu_server_read_cov(atkgpu_server *v, int sid, atkdsp_cf32 *R, uint32_t *counts);
+ATKGPU_API int       atkgpu_server_read_power(atkgpu_server *v, int sid, float *pwr, float *mean_pwr);
 ATKGPU_API int       atkgpu_server_stats(atkgpu_server *v, atkgpu_serve_stats *st, int reset);
 
 /* ---- the filterbank in matrix-core form: S streams of one shape ----------------

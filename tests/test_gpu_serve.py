@@ -103,3 +103,52 @@ def test_admission_against_the_hbm_budget(gpu):
     with pytest.raises(gpu.AtkDspError):
         srv.submit(b, np.zeros(65537 * 4, np.uint8))         # chunk over chunk_max
     srv.close()
+
+
+def test_coherent_array_stream_in_the_mix(gpu):
+    """An array stream (4 phase-coherent cf32 elements per chunk) served
+    beside single streams: its outputs equal the drop-in filterbank over the
+    same elements, and its W1 / W2 equal the drop-in calls on those outputs,
+    chunk by chunk, to the bit."""
+    from atkdsp_amd import synth
+    from atkdsp_amd.serve import Server
+    rng = np.random.default_rng(21)
+    nel, M, K, n = 4, 256, 8, 60_000
+    els = synth.c5_elements(n=n, nel=nel, fs=20e6)
+    singles = _streams(nsamp=60_000)[:2]
+    srv = Server(hbm_budget=4 << 30, max_streams=8, max_batch=64, staging_bytes=64 << 20)
+    aid = srv.add_stream(fmt="cf32", fs=20e6, nchan=M, oversample=2, taps_per_poly=K, chunk_max=20_000,
+                         out_slots=64, nel=nel, coherent=True)
+    sids = [srv.add_stream(fmt=s[0], fs=FS, nchan=s[2], oversample=2, taps_per_poly=s[3], offset=s[4],
+                           chunk_max=20_000, out_slots=64) for s in singles]
+    cuts = [0]
+    while cuts[-1] < n:
+        cuts.append(min(n, cuts[-1] + int(rng.integers(1, 20_000))))
+    tickets = []
+    for c in range(len(cuts) - 1):
+        a, b = cuts[c], cuts[c + 1]
+        chunk = np.concatenate([e[a:b] for e in els])              # element-major
+        tickets.append(srv.submit(aid, chunk))
+        for i, sid in enumerate(sids):                              # interleave other traffic
+            raw, sc = singles[i][5], singles[i][6]
+            if c + 1 < len(sc):
+                bps = O.BPS[singles[i][1]]
+                srv.submit(sid, raw[sc[c] * bps:sc[c + 1] * bps])
+    parts = [srv.read(aid, t) for t in tickets]
+    y = np.concatenate([p for p in parts if p.shape[2]], axis=2)
+    ref = gpu.Pfb(20e6, M, 2, K, 80.0, max_nel=nel).process_n(els, nel)
+    assert y.shape == ref.shape and np.array_equal(y, ref)
+    R_ref = np.zeros((M, nel, nel), np.complex64)
+    c_ref = np.zeros(M, np.uint32)
+    for p in parts:
+        if p.shape[2]:
+            gpu.pfb_cov_accumulate(p, nel, M, None, R_ref, c_ref)
+    R, cnt = srv.read_cov(aid)
+    assert np.array_equal(cnt, c_ref) and np.array_equal(R, R_ref)
+    last = [p for p in parts if p.shape[2]][-1]
+    pw, mp = srv.read_power(aid)
+    pw_ref, mp_ref = gpu.pfb_channel_power(last, nel, M)
+    assert np.array_equal(pw, pw_ref) and np.array_equal(mp, mp_ref)
+    with pytest.raises(gpu.AtkDspError):
+        srv.read_cov(sids[0])                                       # not an array
+    srv.close()
