@@ -31,7 +31,11 @@ struct PfbShape {
     static constexpr int PT = M >= 256 ? 16 : 8;
     static constexpr int NT = M / PT;                        // FFT team
     static constexpr int TEAMS = 256 / NT;
+#ifdef ATKGPU_PFB_GP0
+    static constexpr int GP0 = ATKGPU_PFB_GP0;
+#else
     static constexpr int GP0 = COLS >= 4 ? 4 : 8;
+#endif
     // hops a subset folds: at least enough that every FFT team has a hop
     static constexpr int GP = SUB * GP0 >= TEAMS ? GP0 : TEAMS / SUB;
     static constexpr int G = SUB * GP;                       // hops per group
@@ -74,7 +78,10 @@ struct PfbFusedArgs {
 };
 
 template <int M, int OS, int KOS>
-__global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
+#ifndef ATKGPU_PFB_WAVES
+#define ATKGPU_PFB_WAVES 2
+#endif
+__global__ void __launch_bounds__(256, ATKGPU_PFB_WAVES) k_pfb_fused(PfbFusedArgs a) {
     using S = PfbShape<M, OS>;
     constexpr int HOP = S::HOP, G = S::G, GP = S::GP, ROW = S::ROW;
     __shared__ cpx lds[G * ROW];
