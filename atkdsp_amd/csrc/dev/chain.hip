@@ -228,18 +228,34 @@ __global__ void __launch_bounds__(256) k_disp_reduce(DispArgs a) {
     const int fhop = (int)a.fhop;
     const int f0 = fb * a.fpb;
     const int f1 = min(f0 + a.fpb, (int)a.frames);
-    // each team takes every TEAMS-th frame of the block's range
+    // this thread's twiddles for every stage, once for all frames (when every
+    // stage has the full radix; other sizes read them from LDS per stage)
+    constexpr bool TWR = TeamTw<N, PT>::FULL;
+    TeamTw<N, PT> ttw;
+    if constexpr (TWR) team_twiddles<N, PT>(ttw, twl, j);
+    // each team takes every TEAMS-th frame of the block's range; the next
+    // frame's samples are loaded while this frame is transformed
+    cpx nx[PT];
+    {
+        const int f = f0 + tm;
+        const cpx *fr = row + (f < f1 ? f : f0) * fhop + j;
+#pragma unroll
+        for (int r = 0; r < PT; ++r) nx[r] = fr[r * NT];
+    }
     for (int fbase = f0; fbase < f1; fbase += TEAMS) {
         const int f = fbase + tm;
         const bool act = f < f1;
-        const cpx *fr = row + (act ? f : f0) * fhop + j;
         cpx v[PT];
 #pragma unroll
-        for (int r = 0; r < PT; ++r) {
-            const cpx x = fr[r * NT];
-            v[r] = act ? cmake(x.x * w[r], x.y * w[r]) : cmake(0.f, 0.f);
+        for (int r = 0; r < PT; ++r) v[r] = act ? cmake(nx[r].x * w[r], nx[r].y * w[r]) : cmake(0.f, 0.f);
+        {
+            const int fn = f + TEAMS;
+            const cpx *fr = row + (fn < f1 ? fn : f0) * fhop + j;
+#pragma unroll
+            for (int r = 0; r < PT; ++r) nx[r] = fr[r * NT];
         }
-        team_fft<N, PT>(v, lds + tm * TL, twl, j);
+        if constexpr (TWR) team_fft_tw<N, PT>(v, lds + tm * TL, ttw, j);
+        else team_fft<N, PT>(v, lds + tm * TL, twl, j);
 #pragma unroll
         for (int r = 0; r < PT; ++r) {
             const float pw = (v[r].x * v[r].x + v[r].y * v[r].y) * inv2;

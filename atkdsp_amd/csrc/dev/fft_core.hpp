@@ -160,6 +160,32 @@ FFT_HD void team_stage(cpx *v, cpx *lds, const cpx *tw, int j, int Ns, bool last
     }
 }
 
+// The same full-radix stage (one butterfly a thread, N = PT^S) with the
+// thread's twiddles for this stage given in w[r-1] = tw[r * step]: a thread
+// transforming many frames holds them in registers. Value for value the
+// arithmetic of team_stage<N, PT, PT>.
+template <int N, int PT>
+FFT_HD void team_stage_twiddles(cpx *w, const cpx *tw, int j, int Ns) {
+    const int k = j % Ns;
+    const int step = k * (N / (Ns * PT));
+    for (int r = 1; r < PT; ++r) w[r - 1] = tw[r * step];
+}
+
+template <int N, int PT>
+FFT_HD void team_stage_w(cpx *v, cpx *lds, const cpx *w, int j, int Ns, bool last) {
+    const int k = j % Ns;
+    if (Ns > 1 && k != 0) {
+#pragma unroll
+        for (int r = 1; r < PT; ++r) v[r] = cmul(v[r], w[r - 1]);
+    }
+    dft_pow2<PT>(v);
+    if (!last) {
+        const int base = (j / Ns) * Ns * PT + k;
+#pragma unroll
+        for (int m = 0; m < PT; ++m) lds[lpad(base + m * Ns)] = v[m];
+    }
+}
+
 template <int N, int PT>
 FFT_HD void team_load(cpx *v, const cpx *lds, int j) {
     constexpr int NT = N / PT;

@@ -17,6 +17,9 @@ def test_team_fft_emulation_matches_dft(tmp_path):
                     str(ROOT / "tests" / "cpu_emul" / "team_fft_emul.cpp"), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
     rows = [ln.split() for ln in out.splitlines()]
-    assert [int(r[0]) for r in rows] == [64, 128, 256, 512, 1024, 2048, 4096]
-    for n, err in rows:
+    sizes = [r for r in rows if not r[0].startswith("tw")]
+    assert [int(r[0]) for r in sizes] == [64, 128, 256, 512, 1024, 2048, 4096]
+    for n, err in sizes:
         assert float(err) < 3e-7, (n, err)
+    # the register-twiddle stages equal the table-twiddle stages bit for bit
+    assert [r for r in rows if r[0].startswith("tw")] == [["tw4096", "0"], ["tw256", "0"]]
