@@ -39,6 +39,8 @@ int         atkgpu_event_record(void *e, void *s);
 int         atkgpu_event_elapsed_ms(void *e0, void *e1, float *ms);
 int         atkgpu_event_sync(void *e);
 int         atkgpu_event_query(void *e);           /* 1 done, 0 pending, <0 error */
+int         atkgpu_event_create_order(void **e);   /* no timing: for stream ordering */
+int         atkgpu_stream_wait_event(void *s, void *e);
 int         atkgpu_d2h_sync(void *dst, const void *src, size_t bytes);
 int         atkgpu_d2h_2d_sync(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
                                size_t height);
@@ -136,7 +138,10 @@ int atkgpu_chain_keep(void *s, const void *src, size_t spitch, void *dst, size_t
 
 /* ---- ragged batches of independent streams (serve.hip) ----
  * Job arrays live in device-visible memory (mapped pinned host memory is
- * fine); job j owns blocks [blk0, blk0 + nblk) of the launch. */
+ * fine). Filterbank jobs are per block: entry b is the element block b
+ * works on, with the block's tile in blk0 and the element's tile count in
+ * nblk. Unpack jobs are per job (job j owns launch blocks [blk0, blk0 + nblk))
+ * or, with per_block set, per block (entry b's blk0 is block b's tile). */
 typedef struct {               /* layout of PfbJob in pfb_fused.hpp */
     const void *s;             /* stream rows (cf32), hop i's window at s + i*hop */
     void *out;                 /* channel k, hop i at out[k*opitch + i] */
@@ -148,18 +153,43 @@ typedef struct {
     const void *raw;           /* device-visible raw bytes, 16-byte aligned */
     void *dst;                 /* cf32 out, 16-byte aligned */
     float off_re, off_im;      /* offset folded into the constants */
-    int n, blk0, nblk, fmt;    /* fmt: ATKDSP_FMT_* of this job's bytes */
+    int n, blk0, nblk, fmt;    /* fmt: ATKDSP_FMT_* of this job's bytes, or ATKGPU_JOB_COPY */
 } atkgpu_unpack_job;
+#define ATKGPU_JOB_COPY 255        /* an unpack-launch job that copies n cf32 samples raw -> dst */
 typedef struct {
     const void *src;
     void *dst;
     int n, blk0, nblk, pad_;
 } atkgpu_copy_job;
 #define ATKGPU_RAGGED_SPB 2048     /* samples a block of the ragged unpack/copy handles */
+/* served display: one block transforms frames [0, nfr) at first + f*fhop of
+ * one element and writes its partial line (N floats, display order); the
+ * element's last block to finish merges all of its blocks' partials in order
+ * into the running line and writes every line they complete. The element's
+ * frames [fa, fa + nf) are split into runs of at most fpb frames, none
+ * crossing a line boundary (a multiple of F); block b of the element takes
+ * the b-th run and its partial sits at part0 + b*ATKGPU_DISP_PART_STRIDE. */
+typedef struct {
+    const void *first;         /* cf32: this block's first frame */
+    const float *win;          /* N window values, or NULL */
+    float *part0;              /* the element's first partial line */
+    float *acc;                /* the element's running line (read and written by the merging block) */
+    float *ring;               /* line ring of the element: line L at ring + (L mod slots)*lstride */
+    unsigned *counter;         /* the element's finished-block count (0 between launches) */
+    long long fa;              /* the element's first frame */
+    int nf, F, fpb, nblk;      /* its frames, frames per line, frames per run, blocks */
+    int b, nfr, fhop;          /* this block's run index, frames, frame hop */
+    int cols, slots, lstride;
+    float scale;               /* 1/F for the average detector, else 1 */
+    int pad_;
+} atkgpu_disp_blk;
+#define ATKGPU_DISP_PART_STRIDE 4096   /* floats between consecutive blocks' partial lines */
+int atkgpu_disp_jobs(void *s, const void *blks, int nblk, unsigned N, int det, const void *tw);
 int atkgpu_pfb_group(unsigned M, unsigned os);      /* hops per block of the fused kernel */
 int atkgpu_pfb_fused_jobs(void *s, const void *jobs, int njobs, int nblocks, unsigned M, unsigned os,
                           unsigned kos, const void *tw);
-int atkgpu_unpack_jobs(void *s, const void *jobs, int njobs, int nblocks);   /* any mix of formats */
+/* any mix of formats and prefix copies; max_grid > 0 caps the launch's blocks */
+int atkgpu_unpack_jobs(void *s, const void *jobs, int njobs, int nblocks, int per_block, int max_grid);
 int atkgpu_copy_jobs(void *s, const void *jobs, int njobs, int nblocks);
 int atkgpu_copy_bytes_jobs(void *s, const void *jobs, int njobs, int nblocks);   /* n counts bytes, 4096 per block */
 

@@ -24,6 +24,7 @@
 #include "dev_common.hpp"
 #include "fft_core.hpp"
 #include "fft_team.hpp"
+#include "disp.hpp"
 
 // ---- unpack with a per-stream offset ---------------------------------------------
 struct ChainUnpackArgs {
@@ -176,117 +177,20 @@ struct DispArgs {
     float *partial;             // [S][nfb][N] display order (fftshifted)
 };
 
-template <int N>
-struct DispShape {
-    static constexpr int PT = 16;
-    static constexpr int NT = N / PT;
-    static constexpr int TEAMS = NT >= 256 ? 1 : 256 / NT;
-};
-
-// the reference's detector step (src/fft.c:184-189) as a select, so the
-// accumulation has no branches: the first frame takes pw, MAX keeps the first
-// of equals (pw > acc), MIN likewise (pw < acc), AVG sums
-template <int DET>
-__device__ __forceinline__ float det_step(float acc, float pw, bool first) {
-    float nv;
-    if constexpr (DET == 0) nv = pw > acc ? pw : acc;
-    else if constexpr (DET == 1) nv = pw < acc ? pw : acc;
-    else nv = acc + pw;
-    return first ? pw : nv;
-}
-
-template <int DET>
-__device__ __forceinline__ float det_merge(float v, float u) {
-    if constexpr (DET == 0) return u > v ? u : v;
-    else if constexpr (DET == 1) return u < v ? u : v;
-    else return v + u;
-}
-
 template <int N, int DET>
 __global__ void __launch_bounds__(256) k_disp_reduce(DispArgs a) {
-#pragma clang fp contract(off)
     using D = DispShape<N>;
-    constexpr int PT = D::PT, NT = D::NT, TEAMS = D::TEAMS;
-    constexpr int TL = lpad_size<N>();
-    __shared__ cpx lds[TEAMS * TL];
-    __shared__ float red[TEAMS > 1 ? TEAMS * N : 1];
+    __shared__ cpx lds[D::TEAMS * D::TL];
+    __shared__ float red[D::TEAMS > 1 ? D::TEAMS * N : 1];
     __shared__ cpx twl[N];                 // twiddles: every stage reads them from LDS
     const int s = blockIdx.y, fb = blockIdx.x;
-    const int tm = threadIdx.x / NT, j = threadIdx.x % NT;
-    const cpx *row = a.row0 + (long long)s * a.spitch;
     for (int i = threadIdx.x; i < N; i += 256) twl[i] = a.tw[i];
     __syncthreads();
-    float w[PT];
-#pragma unroll
-    for (int r = 0; r < PT; ++r) w[r] = a.win ? a.win[j + r * NT] : 1.0f;
-    const float inv2 = 1.0f / ((float)N * (float)N);
-    float acc[PT];
-#pragma unroll
-    for (int r = 0; r < PT; ++r) acc[r] = 0.0f;
-    bool first = true;
     // frame offsets within a stream row fit 32 bits
-    const int fhop = (int)a.fhop;
     const int f0 = fb * a.fpb;
     const int f1 = min(f0 + a.fpb, (int)a.frames);
-    // this thread's twiddles for every stage, once for all frames (when every
-    // stage has the full radix; other sizes read them from LDS per stage)
-    constexpr bool TWR = TeamTw<N, PT>::FULL;
-    TeamTw<N, PT> ttw;
-    if constexpr (TWR) team_twiddles<N, PT>(ttw, twl, j);
-    // each team takes every TEAMS-th frame of the block's range; the next
-    // frame's samples are loaded while this frame is transformed
-    cpx nx[PT];
-    {
-        const int f = f0 + tm;
-        const cpx *fr = row + (f < f1 ? f : f0) * fhop + j;
-#pragma unroll
-        for (int r = 0; r < PT; ++r) nx[r] = fr[r * NT];
-    }
-    for (int fbase = f0; fbase < f1; fbase += TEAMS) {
-        const int f = fbase + tm;
-        const bool act = f < f1;
-        cpx v[PT];
-#pragma unroll
-        for (int r = 0; r < PT; ++r) v[r] = act ? cmake(nx[r].x * w[r], nx[r].y * w[r]) : cmake(0.f, 0.f);
-        {
-            const int fn = f + TEAMS;
-            const cpx *fr = row + (fn < f1 ? fn : f0) * fhop + j;
-#pragma unroll
-            for (int r = 0; r < PT; ++r) nx[r] = fr[r * NT];
-        }
-        if constexpr (TWR) team_fft_tw<N, PT>(v, lds + tm * TL, ttw, j);
-        else team_fft<N, PT>(v, lds + tm * TL, twl, j);
-#pragma unroll
-        for (int r = 0; r < PT; ++r) {
-            const float pw = (v[r].x * v[r].x + v[r].y * v[r].y) * inv2;
-            const float nv = det_step<DET>(acc[r], pw, first);
-            acc[r] = act ? nv : acc[r];
-        }
-        first = first && !act;
-    }
-    // combine the teams in team order, then write the block's partial line in
-    // display order: FFT bin k = j + r*NT sits at display bin (k + N/2) mod N
-    float *out = a.partial + ((long long)s * a.nfb + fb) * N;
-    if constexpr (TEAMS == 1) {
-#pragma unroll
-        for (int r = 0; r < PT; ++r) out[j + ((r + PT / 2) % PT) * NT] = first ? 0.0f : acc[r];
-    } else {
-        // a team with no frame (only at the very end of a stream) contributes nothing:
-        // mark it with NaN, skipped below
-#pragma unroll
-        for (int r = 0; r < PT; ++r) red[tm * N + j + r * NT] = first ? __builtin_nanf("") : acc[r];
-        __syncthreads();
-        for (int k = threadIdx.x; k < N; k += 256) {
-            float v = red[k];
-            for (int t = 1; t < TEAMS; ++t) {
-                const float u = red[t * N + k];
-                if (u != u) continue;
-                if (v != v) { v = u; continue; }
-                v = det_merge<DET>(v, u);
-            }
-            out[(k + N / 2) % N] = (v != v) ? 0.0f : v;
-        }
-    }
+    disp_frames<N, DET>(a.row0 + (long long)s * a.spitch, (int)a.fhop, f0, f1, a.win, twl, lds, red,
+                        a.partial + ((long long)s * a.nfb + fb) * N);
 }
 
 // per (stream, bin): combine the frame blocks in order, then dB (reduce's
@@ -318,18 +222,7 @@ k_disp_post(const float *__restrict__ line_db, int N, int cols, float *__restric
     const int s = blockIdx.x;
     const float *in = line_db + (long long)s * N;
     float *d = dec + (long long)s * cols;
-    for (int c = threadIdx.x; c < cols; c += 256) {
-        if (cols >= N) {
-            d[c] = in[c < N ? c : N - 1];
-            continue;
-        }
-        const int per = N / cols;
-        const int a0 = c * per, b0 = (c + 1 == cols) ? N : a0 + per;
-        float best = in[a0];
-        for (int i = a0 + 1; i < b0; ++i)
-            if (in[i] > best) best = in[i];
-        d[c] = best;
-    }
+    disp_decimate(in, N, cols, d);
     if (!trace || nout <= 0) return;
     __syncthreads();
     // stitch: segment bin i at seg_lo + (i + 0.5) seg_step, kept over the whole

@@ -47,8 +47,9 @@ struct PfbShape {
     static_assert(NT <= 256 && G % TEAMS == 0, "shape");
 };
 
-// One element of a ragged batch: its own stream, output, taps and counters.
-// Blocks [blk0, blk0 + nblk) of the launch work on it, one group each.
+// One block of a ragged batch: the fields of the element it works on (its own
+// stream, output, taps and counters), its tile (group) in blk0 and the
+// element's tile count in nblk.
 struct PfbJob {
     const cpx *s;
     cpx *out;              // channel k, hop i at out[k*opitch + i]
@@ -59,7 +60,7 @@ struct PfbJob {
 };
 
 struct PfbFusedArgs {
-    const PfbJob *jobs;    // non-null: a ragged batch of njobs elements
+    const PfbJob *jobs;    // non-null: a ragged batch, one descriptor per block
     int njobs;
     const cpx *s;          // stream rows, element e at s + e*spitch
     long long spitch;
@@ -98,15 +99,12 @@ __global__ void __launch_bounds__(256, ATKGPU_PFB_WAVES) k_pfb_fused(PfbFusedArg
     const float *taps;
     long long b0;
     if (a.jobs) {
-        // ragged batch: find the job whose block range holds lb
-        int lo = 0, hi = a.njobs - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (a.jobs[mid].blk0 <= lb) lo = mid; else hi = mid - 1;
-        }
-        const PfbJob &jb = a.jobs[lo];
+        // ragged batch: one descriptor per block (its job's fields, its tile
+        // in blk0 and the job's tile count in nblk), so a block makes one read
+        // of possibly host-resident memory before it starts
+        const PfbJob &jb = a.jobs[lb];
         e = 0;
-        tile = lb - jb.blk0;
+        tile = jb.blk0;
         s = jb.s;
         oute = jb.out;
         taps = jb.taps;

@@ -104,6 +104,19 @@ extern "C" int atkgpu_event_create(void **e) {
     return 0;
 }
 
+// an event for ordering only (no timestamps: cheaper to record and wait on)
+extern "C" int atkgpu_event_create_order(void **e) {
+    hipEvent_t ev;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return -4;
+    *e = (void *)ev;
+    return 0;
+}
+
+// work submitted to s after this call waits for e's last record
+extern "C" int atkgpu_stream_wait_event(void *s, void *e) {
+    return hipStreamWaitEvent((hipStream_t)s, (hipEvent_t)e, 0) == hipSuccess ? 0 : -4;
+}
+
 extern "C" void atkgpu_event_destroy(void *e) {
     if (e) (void)hipEventDestroy((hipEvent_t)e);
 }

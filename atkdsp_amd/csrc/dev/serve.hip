@@ -6,9 +6,11 @@
 //                 src/convert.c:33-111); raw bytes may sit in pinned host
 //                 memory mapped for the device (read over PCIe, no copy call)
 //   k_srv_copy    moves the kept prefix of each stream to its other row
+//   k_srv_disp    display lines of the streams that have them
 #include <hip/hip_runtime.h>
 #include "atkgpu_dev.h"
 #include "dev_common.hpp"
+#include "disp.hpp"
 
 template <typename J>
 __device__ __forceinline__ int find_job(const J *jobs, int njobs, int b) {
@@ -89,20 +91,34 @@ __device__ __forceinline__ void srv_unpack_range(const atkgpu_unpack_job &jb, in
     }
 }
 
-// Every format in one launch: a block belongs to one job, so the switch is
-// uniform over the block.
-__global__ void __launch_bounds__(256) k_srv_unpack(const atkgpu_unpack_job *__restrict__ jobs, int njobs) {
-    const int b = blockIdx.x;
-    const atkgpu_unpack_job jb = jobs[find_job(jobs, njobs, b)];
-    const int i0 = (b - jb.blk0) * ATKGPU_RAGGED_SPB;
+// Every format in one launch, and the copies of kept prefixes beside them: a
+// block belongs to one job, so the switch is uniform over the block.
+// The grid may be smaller than the job blocks: a block then takes every
+// gridDim.x-th of them, which bounds the PCIe reads in flight (enough for
+// the link's bandwidth; many more only queue ahead of other kernels' HBM
+// traffic).
+// With per_block set, entry b is block b's own (its tile in blk0): one read
+// of the descriptor memory instead of a search through it.
+__global__ void __launch_bounds__(256) k_srv_unpack(const atkgpu_unpack_job *__restrict__ jobs, int njobs,
+                                                    int nblocks, int per_block) {
+  for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const atkgpu_unpack_job jb = jobs[per_block ? b : find_job(jobs, njobs, b)];
+    const int i0 = (per_block ? jb.blk0 : b - jb.blk0) * ATKGPU_RAGGED_SPB;
     const int i1 = min(i0 + ATKGPU_RAGGED_SPB, jb.n);
     switch (jb.fmt) {
+    case ATKGPU_JOB_COPY: {
+        const f2 *src = (const f2 *)jb.raw;
+        f2 *dst = (f2 *)jb.dst;
+        for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) dst[i] = src[i];
+        break;
+    }
     case FMT_CU8: srv_unpack_range<FMT_CU8>(jb, i0, i1); break;
     case FMT_CI8: srv_unpack_range<FMT_CI8>(jb, i0, i1); break;
     case FMT_CI16: srv_unpack_range<FMT_CI16>(jb, i0, i1); break;
     case FMT_CI16Q11: srv_unpack_range<FMT_CI16Q11>(jb, i0, i1); break;
     default: srv_unpack_range<FMT_CF32>(jb, i0, i1); break;
     }
+  }
 }
 
 __global__ void __launch_bounds__(256) k_srv_copy(const atkgpu_copy_job *__restrict__ jobs, int njobs) {
@@ -115,10 +131,12 @@ __global__ void __launch_bounds__(256) k_srv_copy(const atkgpu_copy_job *__restr
     for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) dst[i] = src[i];
 }
 
-extern "C" int atkgpu_unpack_jobs(void *s, const void *jobs, int njobs, int nblocks) {
+extern "C" int atkgpu_unpack_jobs(void *s, const void *jobs, int njobs, int nblocks, int per_block,
+                                  int max_grid) {
     if (!njobs || !nblocks) return 0;
-    hipLaunchKernelGGL(k_srv_unpack, dim3(nblocks), dim3(256), 0, (hipStream_t)s,
-                       (const atkgpu_unpack_job *)jobs, njobs);
+    const int grid = (max_grid > 0 && max_grid < nblocks) ? max_grid : nblocks;
+    hipLaunchKernelGGL(k_srv_unpack, dim3(grid), dim3(256), 0, (hipStream_t)s,
+                       (const atkgpu_unpack_job *)jobs, njobs, nblocks, per_block);
     return LAUNCH_OK();
 }
 
@@ -140,4 +158,103 @@ extern "C" void *atkgpu_hmalloc_mapped(size_t bytes, void **dev_ptr) {
     }
     *dev_ptr = d;
     return p;
+}
+
+// ---- served display lines -----------------------------------------------------------
+// k_srv_disp: one block per run of frames of one element (no run crosses a
+// line boundary) -> its partial line. The element's last block to finish
+// (counted with a device-scope atomic after a release fence) merges every
+// partial of the element in run order into the running line, which lives in
+// device memory across chunks, and at each completed line writes dB
+// (src/fft.c:286-292) and decimate_max (src/display.c:19-37) into the ring.
+// The runs are re-derived from (fa, nf, F, fpb) exactly as the host split
+// them, so the merge needs no per-run marks.
+template <int N, int DET>
+__device__ __forceinline__ void srv_disp_merge(const atkgpu_disp_blk &jb, float *line) {
+#pragma clang fp contract(off)
+    constexpr int NB = N / 256;                // bins a thread: k = threadIdx.x + i*256
+    const long long F = jb.F, fe = jb.fa + jb.nf;
+    float v[NB];
+    const bool cont = (jb.fa % F) != 0;        // the first frame continues a line
+#pragma unroll
+    for (int i = 0; i < NB; ++i) v[i] = cont ? jb.acc[threadIdx.x + i * 256] : 0.0f;
+    int b = 0;
+    for (long long f = jb.fa; f < fe;) {
+        long long se = (f / F + 1) * F;
+        if (se > fe) se = fe;
+        for (long long f0 = f; f0 < se; f0 += jb.fpb, ++b) {
+            const float *p = jb.part0 + (long long)b * ATKGPU_DISP_PART_STRIDE;
+            const bool st = (f0 % F) == 0;
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                const float u = p[threadIdx.x + i * 256];
+                v[i] = st ? u : det_merge<DET>(v[i], u);
+            }
+        }
+        if (se % F == 0) {                     // line se/F - 1 is complete
+#pragma unroll
+            for (int i = 0; i < NB; ++i) line[threadIdx.x + i * 256] = disp_db(v[i], jb.scale);
+            __syncthreads();
+            disp_decimate(line, N, jb.cols, jb.ring + (long long)((se / F - 1) % jb.slots) * jb.lstride);
+            __syncthreads();
+        }
+        f = se;
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) jb.acc[threadIdx.x + i * 256] = v[i];
+}
+
+template <int N, int DET>
+__global__ void __launch_bounds__(256) k_srv_disp(const atkgpu_disp_blk *__restrict__ blks, const cpx *tw) {
+    using D = DispShape<N>;
+    static_assert(D::TEAMS * D::TL * 2 >= N, "the merge's dB line reuses the FFT exchange buffer");
+    __shared__ cpx lds[D::TEAMS * D::TL];
+    __shared__ float red[D::TEAMS > 1 ? D::TEAMS * N : 1];
+    __shared__ cpx twl[N];
+    __shared__ int last;
+    const atkgpu_disp_blk jb = blks[blockIdx.x];
+    for (int i = threadIdx.x; i < N; i += 256) twl[i] = tw[i];
+    __syncthreads();
+    disp_frames<N, DET>((const cpx *)jb.first, jb.fhop, 0, jb.nfr, jb.win, twl, lds, red,
+                        jb.part0 + (long long)jb.b * ATKGPU_DISP_PART_STRIDE);
+    if (jb.nblk > 1) {
+        // publish this partial, then count it; the block that completes the
+        // count sees every partial (acquire) and resets the counter for the
+        // element's next chunk (a later launch on the same stream)
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned old = __hip_atomic_fetch_add(jb.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            last = old == (unsigned)(jb.nblk - 1);
+            if (last) __hip_atomic_store(jb.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        if (!last) return;
+        __threadfence();
+    } else {
+        __syncthreads();                       // the partial is this block's own
+    }
+    srv_disp_merge<N, DET>(jb, (float *)lds);
+}
+
+template <int N>
+static int launch_srv_disp(const atkgpu_disp_blk *blks, int nblk, int det, const cpx *tw, hipStream_t s) {
+    switch (det) {
+    case 0: hipLaunchKernelGGL((k_srv_disp<N, 0>), dim3(nblk), dim3(256), 0, s, blks, tw); break;
+    case 1: hipLaunchKernelGGL((k_srv_disp<N, 1>), dim3(nblk), dim3(256), 0, s, blks, tw); break;
+    case 2: hipLaunchKernelGGL((k_srv_disp<N, 2>), dim3(nblk), dim3(256), 0, s, blks, tw); break;
+    default: return -1;
+    }
+    return LAUNCH_OK();
+}
+
+extern "C" int atkgpu_disp_jobs(void *s, const void *blks, int nblk, unsigned N, int det, const void *tw) {
+    if (nblk <= 0) return 0;
+    const atkgpu_disp_blk *b = (const atkgpu_disp_blk *)blks;
+    switch (N) {
+    case 1024: return launch_srv_disp<1024>(b, nblk, det, (const cpx *)tw, (hipStream_t)s);
+    case 2048: return launch_srv_disp<2048>(b, nblk, det, (const cpx *)tw, (hipStream_t)s);
+    case 4096: return launch_srv_disp<4096>(b, nblk, det, (const cpx *)tw, (hipStream_t)s);
+    }
+    return -1;
 }

@@ -269,7 +269,7 @@ ptrdiff_t atkgpu_pfbx_run(atkgpu_pfbx *v, size_t n) {
     if (!rc && n && !v->mx) {
         int ublk = 0;
         for (unsigned s = 0; s < v->S; ++s) ublk += uj[s].nblk;
-        rc = atkgpu_unpack_jobs(v->stream, DEV(uj), (int)v->S, ublk);
+        rc = atkgpu_unpack_jobs(v->stream, DEV(uj), (int)v->S, ublk, 0, 0);
     }
     if (!rc) rc = atkgpu_event_record(v->ev[1], v->stream);
     if (!rc && m) {

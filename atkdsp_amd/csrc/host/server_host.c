@@ -14,11 +14,23 @@
  *   dispatcher  a thread forms batches continuously: at most one chunk per
  *               stream (each stream's chunks stay in order), taken round-robin
  *               over the streams that have work, while earlier batches still
- *               run. A batch is one ragged unpack launch (every format and
- *               element), one fused filterbank launch per (M, os, taps) class,
- *               a W1 and a W2 launch per coherent array stream, and one
- *               history launch, all on one HIP stream, so a stream's chunks
- *               execute in order.
+ *               run. A batch is
+ *                 on the ingest HIP stream: one ragged launch that copies each
+ *                 chunk's kept prefix (the last keep samples of the stream's
+ *                 previous row) and unpacks its bytes after it, every format
+ *                 and element together;
+ *                 on the compute HIP stream, after that: one fused filterbank
+ *                 launch per (M, os, taps) class, a W1 and a W2 launch per
+ *                 coherent array stream, and a display launch per (FFT
+ *                 length, detector) class (or, with DISPLAY_STREAM, the
+ *                 display on a third HIP stream beside the filterbank, joined
+ *                 before the batch's completion event).
+ *               Every launch reads one descriptor per block from mapped host
+ *               memory (a single PCIe round trip, no search).
+ *               Batch k's ingest writes only rows that batches up to k-2 read
+ *               (a stream alternates two rows, one chunk per batch), so it
+ *               waits for batch k-2's compute and overlaps batch k-1's: the
+ *               PCIe-bound unpack runs beside the HBM-bound filterbank.
  *   completion  a thread polls each batch's event in order (sleeping a few
  *               microseconds between polls, so no core is held spinning),
  *               stamps the host monotonic clock and records each chunk's
@@ -45,6 +57,21 @@ int atkgpu_pfb_prototype(double *h, unsigned nchan, unsigned K, double atten_db)
 #define QCAP 256               /* pending chunks per stream */
 #define LAT_CAP (1u << 22)
 #define NEL_MAX 16             /* elements of an array stream */
+#define NPLANS 7               /* FFT twiddles for 64..4096 points */
+/* blocks of the ingest launch: 4 waves each, one 16-byte load a lane in
+ * flight, so INGEST_GRID * 4 KiB of PCIe reads in flight */
+#ifndef INGEST_GRID
+#define INGEST_GRID 0
+#endif
+/* 1: the display lines run on their own HIP stream beside the filterbank;
+ * 0: after it on the compute stream (measured better at full load, where the
+ * concurrent kernels slow each other more than they overlap) */
+#ifndef DISPLAY_STREAM
+#define DISPLAY_STREAM 0
+#endif
+#define DISP_BLK_CAP 4096      /* display frame blocks per batch */
+#define UBLK_CAP 16384         /* ingest blocks per batch */
+#define PBLK_CAP 16384         /* filterbank blocks per batch */
 
 typedef struct {
     long long ticket;
@@ -69,9 +96,19 @@ typedef struct {
     atkdsp_cf32 *R;            /* coherent: [M][nel][nel] */
     uint32_t *cnt;             /* coherent: [M] */
     float *pwr, *mean;         /* coherent: [nel][M], [M] */
+    /* display (dN == 0: none) */
+    unsigned dN, dhop, dF, dcols, dslots;
+    int ddet;
+    float *dwin, *dacc, *dring;    /* [dN], [nel][dN], [dslots][nel][dcols] */
+    unsigned *dcnt;                /* [nel] finished-block counts of the display launch */
+    long long g_sub, g;            /* samples submitted / dispatched, per element */
+    long long fnext;               /* next display frame to transform */
+    long long lines_disp, lines_done;
     unsigned lc;
     long long b;
     long long submitted, dispatched, completed;
+    int has_prev;              /* a chunk has been dispatched */
+    size_t n_prev;             /* samples per element of the last dispatched chunk */
     size_t *slot_m;            /* hops of the result held in each output slot */
     chunk_t q[QCAP];
     unsigned qh, qn;           /* queue head, count */
@@ -85,20 +122,32 @@ typedef struct {
     unsigned M, os, kos, ov, keep, lc, nel;
     size_t m, row_len, m_max;
     long long b0;
-    atkdsp_cf32 *row, *next, *out;     /* element 0; element e at + e*row_len (rows), + e*M*m_max (out) */
+    atkdsp_cf32 *row, *prev, *out;     /* element 0; element e at + e*row_len (rows), + e*M*m_max (out) */
+    int has_prev;                      /* prev holds the stream's previous chunk (else the prefix is zero) */
+    size_t n_prev;                     /* its samples per element */
     const float *taps;
     const void *tw;
     float off_re, off_im;
     atkdsp_cf32 *R;
     uint32_t *cnt;
     float *pwr, *mean;
+    /* display: frames [fa, fa + nf) complete in this chunk */
+    unsigned dN, dhop, dF, dcols, dslots, dfpb;
+    int ddet, nf;
+    long long g0, fa, lines_after;
+    const float *dwin;
+    float *dacc, *dring;
+    unsigned *dcnt;
 } bitem;
 
 typedef struct {
-    void *event;
+    void *event;               /* the batch's compute is done */
+    void *uevent;              /* its ingest is done */
+    void *devent;              /* its display lines are done */
     int n;
     bitem *items;              /* max_batch */
     void *desc_h, *desc_d;     /* mapped descriptor buffer */
+    float *dpart;              /* display partial lines, DISP_BLK_CAP * ATKGPU_DISP_PART_STRIDE */
 } batch_t;
 
 typedef struct {
@@ -114,8 +163,8 @@ struct atkgpu_server {
     size_t stage_cap, stage_head, stage_tail;    /* monotonic byte counters */
     stage_rec *recs;
     size_t rec_cap, rec_head, rec_tail;
-    void *stream;
-    atkgpu_fftplan *plans[6];                    /* twiddles for M = 64..2048 */
+    void *stream, *ustream, *dstream;            /* compute, ingest, display */
+    atkgpu_fftplan *plans[NPLANS];               /* twiddles for 64..4096 points */
     batch_t batch[NINFLIGHT];
     unsigned b_submit, b_done;                   /* batches launched / completed */
     unsigned rr;                                 /* round-robin start */
@@ -139,16 +188,43 @@ static size_t round16(size_t x) { return (x + 15) & ~(size_t)15; }
 static size_t round64(size_t x) { return (x + 63) & ~(size_t)63; }
 
 static int plan_index(unsigned M) {
-    for (int i = 0; i < 6; ++i)
+    for (int i = 0; i < NPLANS; ++i)
         if (M == (64u << i)) return i;
     return -1;
 }
 
-/* descriptor buffer layout for one batch: unpack, pfb and copy jobs, one of
- * each per element of each chunk */
-static size_t desc_bytes(unsigned jobcap) {
-    return (size_t)jobcap * (sizeof(atkgpu_unpack_job) + sizeof(atkgpu_pfb_job) + sizeof(atkgpu_copy_job)) + 256;
+/* descriptor buffer layout for one batch, one descriptor per launch block:
+ * ingest (prefix copies and unpacks), filterbank, display */
+static size_t desc_bytes(void) {
+    return (size_t)UBLK_CAP * sizeof(atkgpu_unpack_job) + (size_t)PBLK_CAP * sizeof(atkgpu_pfb_job) +
+           (size_t)DISP_BLK_CAP * sizeof(atkgpu_disp_blk) + 256;
 }
+
+static unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+/* display frames complete once a stream has g samples */
+static long long disp_frames_at(long long g, unsigned N, unsigned hop) {
+    return g >= (long long)N ? (g - (long long)N) / hop + 1 : 0;
+}
+
+/* frame blocks for frames [fa, fa + nf): runs of at most fpb frames, none
+ * crossing a line boundary (a multiple of F) */
+static int disp_nblk(long long fa, int nf, unsigned F, unsigned fpb) {
+    int cnt = 0;
+    for (long long f = fa, fe = fa + nf; f < fe;) {
+        long long se = (f / F + 1) * F;
+        if (se > fe) se = fe;
+        cnt += (int)((se - f + fpb - 1) / fpb);
+        f = se;
+    }
+    return cnt;
+}
+
+/* frames per display block, per FFT team of the 256-thread block */
+#ifndef DISP_FRAMES_PER_TEAM
+#define DISP_FRAMES_PER_TEAM 4
+#endif
+static unsigned disp_fpb(unsigned N) { return DISP_FRAMES_PER_TEAM * (N >= 4096 ? 1u : 4096u / N); }
 
 static void *disp_main(void *arg);
 static void *comp_main(void *arg);
@@ -169,12 +245,16 @@ atkgpu_server *atkgpu_server_create(size_t hbm_budget, unsigned max_streams, uns
     v->recs = (stage_rec *)calloc(v->rec_cap, sizeof(stage_rec));
     v->lat = (double *)malloc(LAT_CAP * sizeof(double));
     v->lat_sort = (double *)malloc(LAT_CAP * sizeof(double));
-    int ok = v->st && v->stage_h && v->recs && v->lat && v->lat_sort && !atkgpu_stream_create(&v->stream);
-    for (int i = 0; ok && i < 6; ++i) ok = (v->plans[i] = atkgpu_fftplan_create(64u << i)) != NULL;
+    int ok = v->st && v->stage_h && v->recs && v->lat && v->lat_sort && !atkgpu_stream_create(&v->stream) &&
+             !atkgpu_stream_create(&v->ustream) && !atkgpu_stream_create(&v->dstream);
+    for (int i = 0; ok && i < NPLANS; ++i) ok = (v->plans[i] = atkgpu_fftplan_create(64u << i)) != NULL;
     for (int i = 0; ok && i < NINFLIGHT; ++i) {
         batch_t *b = &v->batch[i];
-        ok = !atkgpu_event_create(&b->event) && (b->items = (bitem *)calloc(max_batch, sizeof(bitem))) &&
-             (b->desc_h = atkgpu_hmalloc_mapped(desc_bytes(v->jobcap), &b->desc_d));
+        ok = !atkgpu_event_create_order(&b->event) && !atkgpu_event_create_order(&b->uevent) &&
+             !atkgpu_event_create_order(&b->devent) &&
+             (b->items = (bitem *)calloc(max_batch, sizeof(bitem))) &&
+             (b->desc_h = atkgpu_hmalloc_mapped(desc_bytes(), &b->desc_d)) &&
+             (b->dpart = (float *)atkgpu_dmalloc((size_t)DISP_BLK_CAP * ATKGPU_DISP_PART_STRIDE * sizeof(float)));
     }
     if (!ok) { atkgpu_server_destroy(v); return NULL; }
     pthread_mutex_init(&v->mu, NULL);
@@ -197,6 +277,10 @@ static void stream_free(sstream *s) {
     atkgpu_dfree(s->cnt);
     atkgpu_dfree(s->pwr);
     atkgpu_dfree(s->mean);
+    atkgpu_dfree(s->dwin);
+    atkgpu_dfree(s->dacc);
+    atkgpu_dfree(s->dring);
+    atkgpu_dfree(s->dcnt);
     free(s->slot_m);
     memset(s, 0, sizeof *s);
 }
@@ -215,16 +299,23 @@ void atkgpu_server_destroy(atkgpu_server *v) {
         pthread_join(v->th_disp, NULL);
         pthread_join(v->th_comp, NULL);
     }
+    if (v->ustream) atkgpu_sync(v->ustream);
+    if (v->dstream) atkgpu_sync(v->dstream);
     if (v->stream) atkgpu_sync(v->stream);
     for (unsigned i = 0; v->st && i < v->max_streams; ++i)
         if (v->st[i].used) stream_free(&v->st[i]);
     for (int i = 0; i < NINFLIGHT; ++i) {
         atkgpu_event_destroy(v->batch[i].event);
+        atkgpu_event_destroy(v->batch[i].uevent);
+        atkgpu_event_destroy(v->batch[i].devent);
         free(v->batch[i].items);
         atkgpu_hfree(v->batch[i].desc_h);
+        atkgpu_dfree(v->batch[i].dpart);
     }
-    for (int i = 0; i < 6; ++i) atkgpu_fftplan_destroy(v->plans[i]);
+    for (int i = 0; i < NPLANS; ++i) atkgpu_fftplan_destroy(v->plans[i]);
     if (v->stream) atkgpu_stream_destroy(v->stream);
+    if (v->ustream) atkgpu_stream_destroy(v->ustream);
+    if (v->dstream) atkgpu_stream_destroy(v->dstream);
     atkgpu_hfree(v->stage_h);
     free(v->recs);
     free(v->lat);
@@ -243,7 +334,26 @@ int atkgpu_server_add_stream(atkgpu_server *v, const atkgpu_stream_cfg *cfg) {
         return ATKDSP_E_ARG;
     const unsigned M = cfg->nchan, os = cfg->oversample, K = cfg->taps_per_poly;
     const unsigned hop = M / os, ov = M * K - hop;
-    const unsigned keep = ov + hop;
+    /* display: the frame length, hop, line and ring */
+    const unsigned dN = cfg->disp_nfft;
+    const unsigned dhop = cfg->disp_hop ? cfg->disp_hop : dN;
+    const unsigned dcols = cfg->disp_cols ? cfg->disp_cols : dN;
+    const unsigned dslots = cfg->disp_slots ? cfg->disp_slots : 4;
+    if (dN) {
+        if ((dN != 1024 && dN != 2048 && dN != 4096) || dhop > dN || cfg->disp_frames < 1 || dcols > dN ||
+            cfg->disp_window < 0 || cfg->disp_window > 3 || cfg->disp_detector < 0 || cfg->disp_detector > 2)
+            return ATKDSP_E_ARG;
+        /* a chunk's frame blocks must fit one batch with room to spare */
+        const size_t fmax = cfg->chunk_max / dhop + 1;
+        if ((size_t)nel * (fmax / disp_fpb(dN) + fmax / cfg->disp_frames + 4) > DISP_BLK_CAP / 2) return ATKDSP_E_ARG;
+    }
+    /* the kept prefix holds the filterbank's history and, with a display,
+     * the unfinished frame (< dN samples) */
+    const unsigned keep = (ov + hop) > dN ? ov + hop : dN;
+    /* a chunk's ingest and filterbank blocks must fit one batch with room to spare */
+    if ((size_t)nel * (cdiv(keep, ATKGPU_RAGGED_SPB) + cdiv(cfg->chunk_max, ATKGPU_RAGGED_SPB)) > UBLK_CAP / 2 ||
+        (size_t)nel * cdiv((cfg->chunk_max + hop) / hop, (size_t)atkgpu_pfb_group(M, os)) > PBLK_CAP / 2)
+        return ATKDSP_E_ARG;
     /* output rows padded to 16 hops (128 bytes): whole sectors per segment */
     const size_t m_max = round16((cfg->chunk_max + hop) / hop + 1);
     const size_t row_len = (size_t)keep + cfg->chunk_max;
@@ -252,6 +362,7 @@ int atkgpu_server_add_stream(atkgpu_server *v, const atkgpu_stream_cfg *cfg) {
     if (cfg->coherent)
         bytes += (size_t)M * nel * nel * sizeof(atkdsp_cf32) + (size_t)M * sizeof(uint32_t) +
                  (size_t)(nel + 1) * M * sizeof(float);
+    if (dN) bytes += ((size_t)(nel + 1) * dN + (size_t)dslots * nel * dcols + nel) * sizeof(float);
     pthread_mutex_lock(&v->mu);
     int sid = -1;
     for (unsigned i = 0; i < v->max_streams; ++i)
@@ -284,6 +395,20 @@ int atkgpu_server_add_stream(atkgpu_server *v, const atkgpu_stream_cfg *cfg) {
         s->mean = (float *)atkgpu_dmalloc((size_t)M * sizeof(float));
         if (!s->R || !s->cnt || !s->pwr || !s->mean) rc = ATKDSP_E_NOMEM;
     }
+    float *w = NULL;
+    if (!rc && dN) {
+        s->dN = dN; s->dhop = dhop; s->dF = cfg->disp_frames; s->dcols = dcols; s->dslots = dslots;
+        s->ddet = cfg->disp_detector;
+        s->dwin = (float *)atkgpu_dmalloc((size_t)dN * sizeof(float));
+        s->dacc = (float *)atkgpu_dmalloc((size_t)nel * dN * sizeof(float));
+        s->dring = (float *)atkgpu_dmalloc((size_t)dslots * nel * dcols * sizeof(float));
+        s->dcnt = (unsigned *)atkgpu_dmalloc((size_t)nel * sizeof(unsigned));
+        w = (float *)malloc((size_t)dN * sizeof(float));
+        if (!s->dwin || !s->dacc || !s->dring || !s->dcnt || !w) rc = ATKDSP_E_NOMEM;
+        if (!rc) rc = atkdsp_window(cfg->disp_window, dN, w);
+        if (!rc) rc = atkgpu_fill_u32(v->stream, s->dcnt, 0, nel);
+        if (!rc) rc = atkgpu_h2d(v->stream, s->dwin, w, (size_t)dN * sizeof(float));
+    }
     double *h = (double *)malloc((size_t)M * K * sizeof(double));
     float *hf = (float *)calloc((size_t)kos * hop, sizeof(float));
     if (!rc && (!h || !hf)) rc = ATKDSP_E_NOMEM;
@@ -298,6 +423,7 @@ int atkgpu_server_add_stream(atkgpu_server *v, const atkgpu_stream_cfg *cfg) {
     }
     free(h);
     free(hf);
+    free(w);
     if (rc) {
         pthread_mutex_lock(&v->mu);
         v->used_bytes -= bytes;
@@ -375,6 +501,7 @@ long long atkgpu_server_submit(atkgpu_server *v, int sid, const void *raw, size_
     if (!v->running || v->failed) { pthread_mutex_unlock(&v->mu); return ATKDSP_E_FFT; }
     const double t = now_s();
     const long long ticket = s->submitted++;
+    s->g_sub += (long long)n;
     chunk_t *c = &s->q[(s->qh + s->qn) % QCAP];
     c->ticket = ticket;
     c->off = off;
@@ -398,40 +525,60 @@ long long atkgpu_server_submit(atkgpu_server *v, int sid, const void *raw, size_
     return ticket;
 }
 
-/* Launch one batch (the mutex is not held): ragged unpack of every element,
- * fused filterbank per (M, os, kos) class, W1/W2 per coherent array, then
- * every element's kept prefix. */
-static int launch_batch(atkgpu_server *v, batch_t *bt) {
+/* Launch batch k (the mutex is not held): on the ingest stream, after batch
+ * k-2's compute, every element's kept prefix and unpack; after that ingest,
+ * the fused filterbank per (M, os, kos) class and W1/W2 per coherent array on
+ * the compute stream, and the display lines on the display stream, which the
+ * compute stream then waits for. */
+static int launch_batch(atkgpu_server *v, batch_t *bt, unsigned k) {
     const int n = bt->n;
     atkgpu_unpack_job *uj = (atkgpu_unpack_job *)bt->desc_h;
-    atkgpu_pfb_job *pj = (atkgpu_pfb_job *)(uj + v->jobcap);
-    atkgpu_copy_job *cj = (atkgpu_copy_job *)(pj + v->jobcap);
+    atkgpu_pfb_job *pj = (atkgpu_pfb_job *)(uj + UBLK_CAP);
     char *dbase = (char *)bt->desc_d, *hbase = (char *)bt->desc_h;
 #define DEV(p) ((void *)(dbase + ((char *)(p) - hbase)))
     int rc = 0;
-    /* unpack, every format and element in one launch (staging offsets are
-     * 64-byte aligned and keep is a multiple of M >= 64, so both ends are
-     * 16-byte aligned for the kernel's word loads and stores) */
-    int nu = 0, ublk = 0;
+    /* ingest, one launch: the kept prefix of each element (the last keep
+     * samples of [prev, prev + keep + n_prev)) into [row, row + keep), and the
+     * chunk's samples unpacked into [row + keep, row + keep + n), every format
+     * together. Staging offsets are 64-byte aligned and keep is a multiple of
+     * 64, so both ends of an unpack are 16-byte aligned for its word loads and
+     * stores. */
+    /* one descriptor per launch block (blk0 = the block's tile in its job) */
+    int nu = 0;
     for (int i = 0; i < n; ++i) {
         const bitem *it = &bt->items[i];
-        if (it->c.n == 0) continue;
         for (unsigned e = 0; e < it->nel; ++e) {
-            atkgpu_unpack_job *j = &uj[nu++];
-            j->raw = v->stage_d + (it->c.off % v->stage_cap) + e * it->c.estride;
-            j->dst = it->row + e * it->row_len + it->keep;
-            j->off_re = it->off_re;
-            j->off_im = it->off_im;
-            j->n = (int)it->c.n;
-            j->blk0 = ublk;
-            j->nblk = (j->n + ATKGPU_RAGGED_SPB - 1) / ATKGPU_RAGGED_SPB;
-            j->fmt = it->fmt;
-            ublk += j->nblk;
-            if ((((size_t)j->raw) | ((size_t)j->dst)) & 15) rc = ATKDSP_E_ARG;
+            atkgpu_unpack_job j;
+            if (it->has_prev) {
+                j.raw = it->prev + e * it->row_len + it->n_prev;
+                j.dst = it->row + e * it->row_len;
+                j.off_re = j.off_im = 0.0f;
+                j.n = (int)it->keep;
+                j.nblk = (j.n + ATKGPU_RAGGED_SPB - 1) / ATKGPU_RAGGED_SPB;
+                j.fmt = ATKGPU_JOB_COPY;
+                for (int t = 0; t < j.nblk; ++t) { j.blk0 = t; uj[nu++] = j; }
+            }
+            if (it->c.n == 0) continue;
+            j.raw = v->stage_d + (it->c.off % v->stage_cap) + e * it->c.estride;
+            j.dst = it->row + e * it->row_len + it->keep;
+            j.off_re = it->off_re;
+            j.off_im = it->off_im;
+            j.n = (int)it->c.n;
+            j.nblk = (j.n + ATKGPU_RAGGED_SPB - 1) / ATKGPU_RAGGED_SPB;
+            j.fmt = it->fmt;
+            if ((((size_t)j.raw) | ((size_t)j.dst)) & 15) rc = ATKDSP_E_ARG;
+            for (int t = 0; t < j.nblk; ++t) { j.blk0 = t; uj[nu++] = j; }
         }
     }
-    if (!rc && nu) rc = atkgpu_unpack_jobs(v->stream, DEV(uj), nu, ublk);
-    /* the filterbank, one launch per shape class present, one job per element */
+    /* batch k's ingest overwrites the rows batch k-2 read */
+    if (!rc && k >= 2) rc = atkgpu_stream_wait_event(v->ustream, v->batch[(k - 2) % NINFLIGHT].event);
+    if (!rc && nu) rc = atkgpu_unpack_jobs(v->ustream, DEV(uj), nu, nu, 1, INGEST_GRID);
+    if (!rc) rc = atkgpu_event_record(bt->uevent, v->ustream);
+    if (!rc) rc = atkgpu_stream_wait_event(v->stream, bt->uevent);
+    void *dst = DISPLAY_STREAM ? v->dstream : v->stream;
+    if (!rc && DISPLAY_STREAM) rc = atkgpu_stream_wait_event(v->dstream, bt->uevent);
+    /* the filterbank, one launch per shape class present, one descriptor per
+     * block (blk0 = its group in the element, nblk = the element's groups) */
     int np = 0;
     unsigned char done[4096];
     memset(done, 0, sizeof done);
@@ -439,7 +586,6 @@ static int launch_batch(atkgpu_server *v, batch_t *bt) {
         if (done[i]) continue;
         const bitem *a = &bt->items[i];
         const int p0 = np;
-        int blk = 0;
         const int G = atkgpu_pfb_group(a->M, a->os);
         for (int k = i; k < n; ++k) {
             const bitem *it = &bt->items[k];
@@ -447,21 +593,20 @@ static int launch_batch(atkgpu_server *v, batch_t *bt) {
             done[k] = 1;
             if (!it->m) continue;
             for (unsigned e = 0; e < it->nel; ++e) {
-                atkgpu_pfb_job *j = &pj[np++];
-                j->s = it->row + e * it->row_len + (it->keep - it->ov - it->lc);
-                j->out = it->out + (size_t)e * it->M * it->m_max;
-                j->taps = it->taps;
-                j->b0 = it->b0;
-                j->nvalid = (int)(it->ov + it->lc + it->c.n);
-                j->m = (int)it->m;
-                j->opitch = (int)it->m_max;
-                j->blk0 = blk;
-                j->nblk = (int)((it->m + G - 1) / G);
-                j->pad_ = 0;
-                blk += j->nblk;
+                atkgpu_pfb_job j;
+                j.s = it->row + e * it->row_len + (it->keep - it->ov - it->lc);
+                j.out = it->out + (size_t)e * it->M * it->m_max;
+                j.taps = it->taps;
+                j.b0 = it->b0;
+                j.nvalid = (int)(it->ov + it->lc + it->c.n);
+                j.m = (int)it->m;
+                j.opitch = (int)it->m_max;
+                j.nblk = (int)((it->m + G - 1) / G);
+                j.pad_ = 0;
+                for (int t = 0; t < j.nblk; ++t) { j.blk0 = t; pj[np++] = j; }
             }
         }
-        if (np > p0) rc = atkgpu_pfb_fused_jobs(v->stream, DEV(&pj[p0]), np - p0, blk, a->M, a->os, a->kos, a->tw);
+        if (np > p0) rc = atkgpu_pfb_fused_jobs(v->stream, DEV(&pj[p0]), np - p0, np - p0, a->M, a->os, a->kos, a->tw);
     }
     /* W1 accumulated and W2 of this chunk, per coherent array: the element
      * outputs sit [nel][M][m_max] in the slot */
@@ -471,22 +616,61 @@ static int launch_batch(atkgpu_server *v, batch_t *bt) {
         rc = atkgpu_pfb_cov(v->stream, it->out, it->nel, it->M, it->m, it->m_max, NULL, it->R, it->cnt);
         if (!rc) rc = atkgpu_pfb_power(v->stream, it->out, it->nel, it->M, it->m, it->m_max, it->pwr, it->mean);
     }
-    /* kept prefix: the last keep samples of [row, row + keep + n) to the next row */
-    int nc = 0, blk = 0;
-    for (int i = 0; i < n; ++i) {
-        const bitem *it = &bt->items[i];
-        for (unsigned e = 0; e < it->nel; ++e) {
-            atkgpu_copy_job *j = &cj[nc++];
-            j->src = it->row + e * it->row_len + it->c.n;
-            j->dst = it->next + e * it->row_len;
-            j->n = (int)it->keep;
-            j->blk0 = blk;
-            j->nblk = (j->n + ATKGPU_RAGGED_SPB - 1) / ATKGPU_RAGGED_SPB;
-            blk += j->nblk;
+    /* display: per (FFT length, detector) class present, the frame runs of
+     * every element (frame f of element e at row_e + keep + f*hop - g0); the
+     * element's last block merges them into its running line and ring */
+    atkgpu_disp_blk *db = (atkgpu_disp_blk *)(pj + PBLK_CAP);
+    int nb = 0;
+    for (unsigned N = 1024; N <= 4096 && !rc; N *= 2) {
+        for (int det = 0; det < 3 && !rc; ++det) {
+            const int b0 = nb;
+            for (int i = 0; i < n; ++i) {
+                const bitem *it = &bt->items[i];
+                if (!it->nf || it->dN != N || it->ddet != det) continue;
+                const unsigned F = it->dF, fpb = it->dfpb;
+                const int nblk = disp_nblk(it->fa, it->nf, F, fpb);
+                for (unsigned e = 0; e < it->nel; ++e) {
+                    const atkdsp_cf32 *row = it->row + e * it->row_len;
+                    float *part0 = bt->dpart + (size_t)nb * ATKGPU_DISP_PART_STRIDE;
+                    int b = 0;
+                    for (long long f = it->fa, fe = it->fa + it->nf; f < fe;) {
+                        long long se = (f / F + 1) * F;
+                        if (se > fe) se = fe;
+                        for (long long f0 = f; f0 < se; f0 += fpb, ++b) {
+                            const long long f1 = f0 + fpb < se ? f0 + fpb : se;
+                            atkgpu_disp_blk *d = &db[nb++];
+                            d->first = row + (it->keep + f0 * it->dhop - it->g0);
+                            d->win = it->dwin;
+                            d->part0 = part0;
+                            d->acc = it->dacc + (size_t)e * N;
+                            d->ring = it->dring + (size_t)e * it->dcols;
+                            d->counter = it->dcnt + e;
+                            d->fa = it->fa;
+                            d->nf = it->nf;
+                            d->F = (int)F;
+                            d->fpb = (int)fpb;
+                            d->nblk = nblk;
+                            d->b = b;
+                            d->nfr = (int)(f1 - f0);
+                            d->fhop = (int)it->dhop;
+                            d->cols = (int)it->dcols;
+                            d->slots = (int)it->dslots;
+                            d->lstride = (int)(it->nel * it->dcols);
+                            d->scale = det == 2 ? 1.0f / (float)F : 1.0f;
+                            d->pad_ = 0;
+                        }
+                        f = se;
+                    }
+                }
+            }
+            if (nb > b0)
+                rc = atkgpu_disp_jobs(dst, DEV(&db[b0]), nb - b0, N, det,
+                                      atkgpu_fftplan_twiddles(v->plans[plan_index(N)]));
         }
     }
-    if (!rc && nc) rc = atkgpu_copy_jobs(v->stream, DEV(cj), nc, blk);
 #undef DEV
+    if (!rc && nb && DISPLAY_STREAM) rc = atkgpu_event_record(bt->devent, v->dstream);
+    if (!rc && nb && DISPLAY_STREAM) rc = atkgpu_stream_wait_event(v->stream, bt->devent);
     if (!rc) rc = atkgpu_event_record(bt->event, v->stream);
     return rc;
 }
@@ -495,13 +679,31 @@ static int launch_batch(atkgpu_server *v, batch_t *bt) {
  * round-robin; mutex held */
 static int form_batch(atkgpu_server *v, batch_t *bt) {
     int n = 0;
-    unsigned jobs = 0;
+    unsigned jobs = 0, dblk = 0, ublk = 0, pblk = 0;
     const unsigned S = v->max_streams;
     for (unsigned t = 0; t < S && n < (int)v->max_batch && n < 4096; ++t) {
         const unsigned sid = (v->rr + t) % S;
         sstream *s = &v->st[sid];
         if (!s->used || !s->qn || !s->q[s->qh].ready) continue;
         if (jobs + s->nel > v->jobcap) continue;
+        /* display frames this chunk completes, and their blocks */
+        int nf = 0;
+        unsigned fpb = 0, nb = 0;
+        if (s->dN) {
+            const long long fe = disp_frames_at(s->g + (long long)s->q[s->qh].n, s->dN, s->dhop);
+            nf = (int)(fe - s->fnext);
+            fpb = disp_fpb(s->dN);
+            nb = s->nel * (unsigned)disp_nblk(s->fnext, nf, s->dF, fpb);
+            if (dblk + nb > DISP_BLK_CAP) continue;
+        }
+        /* its ingest and filterbank blocks */
+        const size_t cn = s->q[s->qh].n;
+        const unsigned nu = s->nel * ((s->has_prev ? cdiv(s->keep, ATKGPU_RAGGED_SPB) : 0u) + cdiv(cn, ATKGPU_RAGGED_SPB));
+        const unsigned np = s->nel * cdiv(((size_t)s->lc + cn) / s->hop, (size_t)atkgpu_pfb_group(s->M, s->os));
+        if (ublk + nu > UBLK_CAP || pblk + np > PBLK_CAP) continue;
+        ublk += nu;
+        pblk += np;
+        dblk += nb;
         const chunk_t c = s->q[s->qh];
         s->qh = (s->qh + 1) % QCAP;
         s->qn--;
@@ -522,7 +724,9 @@ static int form_batch(atkgpu_server *v, batch_t *bt) {
         it->b0 = s->b;
         it->m = ((size_t)s->lc + c.n) / s->hop;
         it->row = s->rows[s->cur];
-        it->next = s->rows[s->cur ^ 1];
+        it->prev = s->rows[s->cur ^ 1];
+        it->has_prev = s->has_prev;
+        it->n_prev = s->n_prev;
         const unsigned slot = (unsigned)(c.ticket % s->cfg.out_slots);
         it->out = s->out + (size_t)slot * s->nel * s->M * s->m_max;
         it->taps = s->taps;
@@ -533,11 +737,29 @@ static int form_batch(atkgpu_server *v, batch_t *bt) {
         it->cnt = s->cnt;
         it->pwr = s->pwr;
         it->mean = s->mean;
+        it->dN = s->dN;
+        it->nf = nf;
+        if (s->dN) {
+            it->dhop = s->dhop; it->dF = s->dF; it->dcols = s->dcols; it->dslots = s->dslots; it->dfpb = fpb;
+            it->ddet = s->ddet;
+            it->g0 = s->g;
+            it->fa = s->fnext;
+            it->dwin = s->dwin;
+            it->dacc = s->dacc;
+            it->dring = s->dring;
+            it->dcnt = s->dcnt;
+            s->fnext += nf;
+            s->lines_disp = s->fnext / s->dF;
+            it->lines_after = s->lines_disp;
+        }
+        s->g += (long long)c.n;
         /* advance the stream */
         s->slot_m[slot] = it->m;
         s->lc = (unsigned)(((size_t)s->lc + c.n) - it->m * s->hop);
         s->b += (long long)it->m;
         s->cur ^= 1;
+        s->has_prev = 1;
+        s->n_prev = c.n;
         s->dispatched++;
     }
     v->rr = (v->rr + 1) % S;
@@ -556,7 +778,7 @@ static void *disp_main(void *arg) {
         while (v->running && !form_batch(v, bt)) pthread_cond_wait(&v->cv_work, &v->mu);
         if (!v->running) break;
         pthread_mutex_unlock(&v->mu);
-        const int rc = launch_batch(v, bt);
+        const int rc = launch_batch(v, bt, v->b_submit);
         pthread_mutex_lock(&v->mu);
         if (rc) {
             v->failed = 1;
@@ -601,6 +823,7 @@ static void *comp_main(void *arg) {
             const bitem *it = &bt->items[i];
             sstream *s = &v->st[it->sid];
             s->completed++;
+            if (it->dN && it->lines_after > s->lines_done) s->lines_done = it->lines_after;
             if (v->nlat < LAT_CAP) v->lat[v->nlat++] = (t - it->c.t_submit) * 1e3;
             v->chunks++;
             v->samples += (long long)it->c.n * it->nel;
@@ -674,6 +897,37 @@ int atkgpu_server_read_power(atkgpu_server *v, int sid, float *pwr, float *mean_
     int rc = pwr ? atkgpu_d2h_sync(pwr, dp, nel * M * sizeof(float)) : 0;
     if (!rc && mean_pwr) rc = atkgpu_d2h_sync(mean_pwr, dm, M * sizeof(float));
     return rc ? ATKDSP_E_FFT : ATKDSP_OK;
+}
+
+long long atkgpu_server_lines(atkgpu_server *v, int sid) {
+    if (!v || sid < 0 || (unsigned)sid >= v->max_streams) return ATKDSP_E_ARG;
+    pthread_mutex_lock(&v->mu);
+    const sstream *s = &v->st[sid];
+    const long long r = (!s->used || !s->dN) ? ATKDSP_E_ARG : s->lines_done;
+    pthread_mutex_unlock(&v->mu);
+    return r;
+}
+
+int atkgpu_server_read_line(atkgpu_server *v, int sid, long long line, float *out, size_t cap) {
+    if (!v || !out || sid < 0 || (unsigned)sid >= v->max_streams || line < 0) return ATKDSP_E_ARG;
+    pthread_mutex_lock(&v->mu);
+    sstream *s = &v->st[sid];
+    /* the line must be completed by the chunks submitted so far */
+    if (!s->used || !s->dN || line >= disp_frames_at(s->g_sub, s->dN, s->dhop) / s->dF) {
+        pthread_mutex_unlock(&v->mu);
+        return ATKDSP_E_ARG;
+    }
+    while (s->lines_done <= line && !v->failed) pthread_cond_wait(&v->cv_done, &v->mu);
+    const int failed = v->failed;
+    /* the ring still holds it until dslots later lines are dispatched */
+    const int gone = line + (long long)s->dslots < s->lines_disp;
+    const size_t len = (size_t)s->nel * s->dcols;
+    const float *src = s->dring + (size_t)(line % s->dslots) * len;
+    pthread_mutex_unlock(&v->mu);
+    if (failed) return ATKDSP_E_FFT;
+    if (gone) return ATKDSP_E_ARG;
+    if (cap < len) return ATKDSP_E_CAP;
+    return atkgpu_d2h_sync(out, src, len * sizeof(float)) ? ATKDSP_E_FFT : ATKDSP_OK;
 }
 
 int atkgpu_server_drain(atkgpu_server *v) {

@@ -16,9 +16,14 @@
  * --arrays A adds A coherent arrays of 8 cf32 elements at 20 MSPS into
  * 2048-channel, 12-tap, 2x filterbanks with W1 / W2 on the device; a chunk
  * holds all 8 elements.
+ * Every single stream also makes display lines (--display 1, the default):
+ * 4096-point Hann frames at hop 2048, max-hold over 50 ms of frames, each line
+ * decimated to 1024 columns; a chunk's latency includes the lines it
+ * completes.
  *
  * usage: serve_load [--seconds S] [--streams N] [--speed X] [--budget-gib G]
  *                   [--max-batch B] [--seed K] [--mix c4|uniform] [--arrays A]
+ *                   [--display 0|1]
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -169,7 +174,7 @@ int main(int argc, char **argv) {
     double seconds = 5.0, speed = 1.0, budget_gib = 32.0;
     int streams = 128, max_batch = 256;
     unsigned long long seed = 4000;
-    int uniform = 0, arrays = 0;
+    int uniform = 0, arrays = 0, display = 1;
     for (int i = 1; i + 1 < argc; i += 2) {
         if (!strcmp(argv[i], "--seconds")) seconds = atof(argv[i + 1]);
         else if (!strcmp(argv[i], "--streams")) streams = atoi(argv[i + 1]);
@@ -179,6 +184,7 @@ int main(int argc, char **argv) {
         else if (!strcmp(argv[i], "--seed")) seed = strtoull(argv[i + 1], NULL, 10);
         else if (!strcmp(argv[i], "--mix")) uniform = !strcmp(argv[i + 1], "uniform");
         else if (!strcmp(argv[i], "--arrays")) arrays = atoi(argv[i + 1]);
+        else if (!strcmp(argv[i], "--display")) display = atoi(argv[i + 1]);
     }
     g_srv = atkgpu_server_create((size_t)(budget_gib * (1u << 30)), (unsigned)(streams + arrays),
                                  (unsigned)max_batch, (size_t)1024 << 20);
@@ -215,6 +221,16 @@ int main(int argc, char **argv) {
         }
         atkgpu_stream_cfg cfg = {ls->fmt, ls->fs, ls->M, 2, ls->K, 80.0, 0.0f, 0.0f, CHUNK_MAX, 2, ls->nel,
                                  ls->nel > 1};
+        if (display && ls->nel == 1) {
+            cfg.disp_nfft = 4096;
+            cfg.disp_hop = 2048;
+            cfg.disp_window = ATKDSP_WIN_HANN;
+            cfg.disp_detector = ATKDSP_DET_MAX;
+            const double fr = floor(0.05 * ls->fs / 2048 + 0.5);      /* 50 ms of frames */
+            cfg.disp_frames = fr < 1 ? 1u : (unsigned)fr;
+            cfg.disp_cols = 1024;
+            cfg.disp_slots = 4;
+        }
         const int sid = atkgpu_server_add_stream(g_srv, &cfg);
         if (sid < 0) { ++refused; continue; }
         ls->sid = sid;
@@ -235,13 +251,18 @@ int main(int argc, char **argv) {
     atkgpu_server_drain(g_srv);
     atkgpu_serve_stats st;
     atkgpu_server_stats(g_srv, &st, 0);
+    long long lines = 0;
+    for (int s = 0; s < admitted; ++s) {
+        const long long l = atkgpu_server_lines(g_srv, g_st[s].sid);
+        if (l > 0) lines += l;
+    }
     printf("{\"mix\": \"%s\", \"arrays\": %d, \"streams\": %d, \"refused\": %d, \"offered_msps\": %.2f, \"speed\": %.2f, \"seconds\": %.2f, "
            "\"chunks\": %lld, \"samples\": %lld, \"batches\": %lld, \"msamples_per_s\": %.2f, "
            "\"p50_ms\": %.4f, \"p99_ms\": %.4f, \"p999_ms\": %.4f, \"max_ms\": %.4f, \"mean_ms\": %.4f, "
-           "\"hbm_used_gib\": %.3f, \"hbm_budget_gib\": %.1f}\n",
+           "\"hbm_used_gib\": %.3f, \"hbm_budget_gib\": %.1f, \"display\": %d, \"lines\": %lld}\n",
            uniform ? "uniform" : "c4", arrays, admitted, refused, total_rate / 1e6, speed, seconds, st.chunks, st.samples, st.batches,
            st.busy_s > 0 ? st.samples / st.busy_s / 1e6 : 0.0, st.p50_ms, st.p99_ms, st.p999_ms, st.max_ms,
-           st.mean_ms, st.hbm_used / 1073741824.0, st.hbm_budget / 1073741824.0);
+           st.mean_ms, st.hbm_used / 1073741824.0, st.hbm_budget / 1073741824.0, display, lines);
     atkgpu_server_destroy(g_srv);
     for (int s = 0; s < admitted; ++s) free(g_st[s].src);
     free(g_st);

@@ -5,6 +5,10 @@
     t = srv.submit(sid, raw_bytes)        # returns a ticket at once
     y = srv.read(sid, t)                  # [nchan][m] once the chunk is done
     srv.stats()                           # chunks, samples, p50/p99/p99.9 latency
+
+A stream may carry a display (display=dict(nfft=4096, hop=2048, frames=977,
+cols=1024, ...)): line L is spectrum_reduce over frames [L*frames,
+(L+1)*frames) of the stream, decimated to cols columns; srv.read_line(sid, L).
 """
 from __future__ import annotations
 
@@ -19,7 +23,12 @@ class _StreamCfg(C.Structure):
     _fields_ = [("fmt", C.c_int), ("fs", C.c_double), ("nchan", C.c_uint), ("oversample", C.c_uint),
                 ("taps_per_poly", C.c_uint), ("atten_db", C.c_double), ("off_re", C.c_float),
                 ("off_im", C.c_float), ("chunk_max", C.c_size_t), ("out_slots", C.c_uint), ("nel", C.c_uint),
-                ("coherent", C.c_int)]
+                ("coherent", C.c_int), ("disp_nfft", C.c_uint), ("disp_hop", C.c_uint),
+                ("disp_window", C.c_int), ("disp_detector", C.c_int), ("disp_frames", C.c_uint),
+                ("disp_cols", C.c_uint), ("disp_slots", C.c_uint)]
+
+_WINDOWS = {"rect": 0, "hann": 1, "hamming": 2, "blackmanharris": 3, "bh": 3}
+_DETECTORS = {"max": 0, "min": 1, "avg": 2}
 
 
 class _Stats(C.Structure):
@@ -45,6 +54,8 @@ def _bind(lib) -> None:
         "atkgpu_server_read_cov": (C.c_int, [vp, C.c_int, P(_Cf32), P(C.c_uint32)]),
         "atkgpu_server_read_power": (C.c_int, [vp, C.c_int, P(C.c_float), P(C.c_float)]),
         "atkgpu_server_stats": (C.c_int, [vp, P(_Stats), C.c_int]),
+        "atkgpu_server_lines": (C.c_longlong, [vp, C.c_int]),
+        "atkgpu_server_read_line": (C.c_int, [vp, C.c_int, C.c_longlong, P(C.c_float), sz]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -62,6 +73,7 @@ class Server:
         if not self._h:
             raise AtkDspError("atkgpu_server_create failed (no GPU or out of memory)")
         self._nchan = {}
+        self._cols = {}
 
     def close(self):
         h, self._h = getattr(self, "_h", None), None
@@ -72,15 +84,27 @@ class Server:
         self.close()
 
     def add_stream(self, fmt="ci16", fs=40e6, nchan=1024, oversample=2, taps_per_poly=16, atten_db=80.0,
-                   offset: complex = 0j, chunk_max=65536, out_slots=4, nel=1, coherent=False) -> int:
+                   offset: complex = 0j, chunk_max=65536, out_slots=4, nel=1, coherent=False,
+                   display: dict | None = None) -> int:
         """nel > 1: an array of phase-coherent elements (a chunk is nel element
-        blocks, element-major); coherent: W1 / W2 accumulated on the device."""
+        blocks, element-major); coherent: W1 / W2 accumulated on the device.
+        display: dict(nfft, hop, window, detector, frames, cols, slots) for
+        display lines (window and detector by name or enum value)."""
+        d = dict(display or {})
+        nfft = int(d.get("nfft", 0))
+        win = d.get("window", "hann")
+        det = d.get("detector", "max")
         cfg = _StreamCfg(_fmt(fmt), float(fs), int(nchan), int(oversample), int(taps_per_poly), float(atten_db),
                          float(offset.real), float(offset.imag), int(chunk_max), int(out_slots), int(nel),
-                         int(bool(coherent)))
+                         int(bool(coherent)), nfft, int(d.get("hop", 0)),
+                         _WINDOWS[win] if isinstance(win, str) else int(win),
+                         _DETECTORS[det] if isinstance(det, str) else int(det),
+                         int(d.get("frames", 1)), int(d.get("cols", 0)), int(d.get("slots", 0)))
         sid = self._lib.atkgpu_server_add_stream(self._h, C.byref(cfg))
         _check(sid, "server_add_stream")
         self._nchan[sid] = (int(nchan), int(nel))
+        if nfft:
+            self._cols[sid] = int(d.get("cols", 0)) or nfft
         return sid
 
     def try_add_stream(self, **kw):
@@ -128,6 +152,20 @@ class Server:
         _check(self._lib.atkgpu_server_read_power(self._h, int(sid), p.ctypes.data_as(C.POINTER(C.c_float)),
                                                   mp.ctypes.data_as(C.POINTER(C.c_float))), "server_read_power")
         return p, mp
+
+    def lines(self, sid: int) -> int:
+        """display lines of the stream complete so far"""
+        return _check(int(self._lib.atkgpu_server_lines(self._h, int(sid))), "server_lines")
+
+    def read_line(self, sid: int, line: int) -> np.ndarray:
+        """display line `line` in dB: [cols] for a single stream, [nel][cols] for an array"""
+        _, nel = self._nchan[sid]
+        cols = self._cols[sid]
+        out = np.empty(nel * cols, np.float32)
+        _check(self._lib.atkgpu_server_read_line(self._h, int(sid), int(line),
+                                                 out.ctypes.data_as(C.POINTER(C.c_float)), out.size),
+               "server_read_line")
+        return out if nel == 1 else out.reshape(nel, cols)
 
     def drain(self) -> None:
         _check(self._lib.atkgpu_server_drain(self._h), "server_drain")

@@ -92,7 +92,16 @@ ATKGPU_API long long     atkgpu_chain_frames(const atkgpu_chain *c);   /* frames
  * element-major, and its result is [nel][nchan][m]. A coherent stream also
  * accumulates W1 (per-channel nel x nel cross-spectra,
  * atkdsp_pfb_cov_accumulate's arithmetic) over all its chunks and keeps W2
- * (atkdsp_pfb_channel_power) of its latest chunk, in device memory. */
+ * (atkdsp_pfb_channel_power) of its latest chunk, in device memory.
+ *
+ * A stream may also carry a display: every disp_frames frames of its samples
+ * (frame f = samples [f*disp_hop, f*disp_hop + disp_nfft) of the stream from
+ * its first sample) make one line, computed as atkdsp_spectrum_reduce over
+ * those frames (window, FFT, |X|^2/n^2, detector, dB, fftshifted) followed by
+ * atkdsp_decimate_max to disp_cols columns. Frames are transformed as the
+ * chunks that complete them arrive; the running detector line stays on the
+ * device, and a completed line goes into a ring of disp_slots lines. A chunk's
+ * latency covers the lines it completes. */
 typedef struct {
     int fmt;                    /* enum atkdsp_format */
     double fs;
@@ -103,6 +112,13 @@ typedef struct {
     unsigned out_slots;         /* chunk results kept per stream */
     unsigned nel;               /* elements per chunk, 1..16 (0 means 1) */
     int coherent;               /* W1 / W2 for an array stream */
+    unsigned disp_nfft;         /* display FFT length: 0 (no display), 1024, 2048 or 4096 */
+    unsigned disp_hop;          /* frame hop, 1..disp_nfft (0 means disp_nfft) */
+    int disp_window;            /* enum atkdsp_window */
+    int disp_detector;          /* enum atkdsp_detector */
+    unsigned disp_frames;       /* frames per line (>= 1) */
+    unsigned disp_cols;         /* columns per line, 1..disp_nfft (0 means disp_nfft) */
+    unsigned disp_slots;        /* lines kept per stream (0 means 4) */
 } atkgpu_stream_cfg;
 
 typedef struct {
@@ -136,6 +152,12 @@ ATKGPU_API int       atkgpu_server_drain(atkgpu_server *v);
 ATKGPU_API int       atkgpu_server_read_cov(atkgpu_server *v, int sid, atkdsp_cf32 *R, uint32_t *counts);
 ATKGPU_API int       atkgpu_server_read_power(atkgpu_server *v, int sid, float *pwr, float *mean_pwr);
 ATKGPU_API int       atkgpu_server_stats(atkgpu_server *v, atkgpu_serve_stats *st, int reset);
+/* display lines: how many are complete, and line L as [nel][disp_cols] dB
+ * values (waits for the chunk that completes it; L must be completed by the
+ * chunks submitted so far). Valid until disp_slots later lines of the stream
+ * have been dispatched. */
+ATKGPU_API long long atkgpu_server_lines(atkgpu_server *v, int sid);
+ATKGPU_API int       atkgpu_server_read_line(atkgpu_server *v, int sid, long long line, float *out, size_t cap);
 
 /* ---- the filterbank in matrix-core form: S streams of one shape ----------------
  * The polyphase fold as a matrix product on the MFMA units, from raw 8-bit
