@@ -162,29 +162,36 @@ typedef struct {
     int n, blk0, nblk, pad_;
 } atkgpu_copy_job;
 #define ATKGPU_RAGGED_SPB 2048     /* samples a block of the ragged unpack/copy handles */
-/* served display: one block transforms frames [0, nfr) at first + f*fhop of
- * one element and writes its partial line (N floats, display order); the
- * element's last block to finish merges all of its blocks' partials in order
- * into the running line and writes every line they complete. The element's
- * frames [fa, fa + nf) are split into runs of at most fpb frames, none
- * crossing a line boundary (a multiple of F); block b of the element takes
- * the b-th run and its partial sits at part0 + b*ATKGPU_DISP_PART_STRIDE. */
+/* served display, two launches per (FFT length, detector) class:
+ *   frames: each FFT team of a block transforms one frame (window, N-point
+ *           FFT, |X|^2/n^2 in float) and writes its power spectrum in display
+ *           order to P + t*N for the block's t-th frame;
+ *   lines:  one block per line segment of one element's chunk (its frames
+ *           belong to one line): starting from the running line of the
+ *           line's earlier frames (acc_in) or the first frame, the detector
+ *           over the segment's spectra in frame order; then the running line
+ *           for the next chunk (acc_out), or, when the segment ends its line,
+ *           the line's dB decimated to cols columns (`line`). The running
+ *           line is double buffered across chunks (read one buffer, write the
+ *           other), so a chunk's segments are independent blocks. */
 typedef struct {
-    const void *first;         /* cf32: this block's first frame */
+    const void *first;         /* cf32: the block's first frame */
     const float *win;          /* N window values, or NULL */
-    float *part0;              /* the element's first partial line */
-    float *acc;                /* the element's running line (read and written by the merging block) */
-    float *ring;               /* line ring of the element: line L at ring + (L mod slots)*lstride */
-    unsigned *counter;         /* the element's finished-block count (0 between launches) */
-    long long fa;              /* the element's first frame */
-    int nf, F, fpb, nblk;      /* its frames, frames per line, frames per run, blocks */
-    int b, nfr, fhop;          /* this block's run index, frames, frame hop */
-    int cols, slots, lstride;
-    float scale;               /* 1/F for the average detector, else 1 */
+    float *P;                  /* power spectra out, N floats per frame */
+    int nfr, fhop;             /* frames (at most the block's teams), frame hop */
+} atkgpu_frame_blk;
+typedef struct {
+    const float *P;            /* the segment's spectra, N floats per frame */
+    const float *acc_in;       /* running line before the segment, or NULL */
+    float *acc_out;            /* running line after it, or NULL when it ends a line */
+    float *line;               /* the finished line's columns, or NULL */
+    int nfr, cols;
+    float scale;               /* 1/frames-per-line for the average detector, else 1 */
     int pad_;
-} atkgpu_disp_blk;
-#define ATKGPU_DISP_PART_STRIDE 4096   /* floats between consecutive blocks' partial lines */
-int atkgpu_disp_jobs(void *s, const void *blks, int nblk, unsigned N, int det, const void *tw);
+} atkgpu_line_blk;
+int atkgpu_disp_teams(unsigned N);          /* frames one block of the frames launch transforms */
+int atkgpu_disp_frames(void *s, const void *blks, int nblk, unsigned N, const void *tw);
+int atkgpu_disp_lines(void *s, const void *blks, int nblk, unsigned N, int det);
 int atkgpu_pfb_group(unsigned M, unsigned os);      /* hops per block of the fused kernel */
 int atkgpu_pfb_fused_jobs(void *s, const void *jobs, int njobs, int nblocks, unsigned M, unsigned os,
                           unsigned kos, const void *tw);

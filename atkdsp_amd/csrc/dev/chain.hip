@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(256) k_disp_reduce(DispArgs a) {
     // frame offsets within a stream row fit 32 bits
     const int f0 = fb * a.fpb;
     const int f1 = min(f0 + a.fpb, (int)a.frames);
-    disp_frames<N, DET>(a.row0 + (long long)s * a.spitch, (int)a.fhop, f0, f1, a.win, twl, lds, red,
+    disp_frames<N, DET, 256>(a.row0 + (long long)s * a.spitch, (int)a.fhop, f0, f1, a.win, twl, lds, red,
                         a.partial + ((long long)s * a.nfb + fb) * N);
 }
 

@@ -9,11 +9,13 @@
 #include "fft_core.hpp"
 #include "fft_team.hpp"
 
-template <int N>
+// NTHR threads a block; TEAMS FFT teams of N/16 threads transform frames side
+// by side
+template <int N, int NTHR = 256>
 struct DispShape {
     static constexpr int PT = 16;
     static constexpr int NT = N / PT;
-    static constexpr int TEAMS = NT >= 256 ? 1 : 256 / NT;
+    static constexpr int TEAMS = NT >= NTHR ? 1 : NTHR / NT;
     static constexpr int TL = lpad_size<N>();
 };
 
@@ -49,14 +51,16 @@ __device__ __forceinline__ float disp_db(float v, float scale) {
 }
 
 // Frames [f0, f1) of one stream, frame f starting at row + f*fhop. Every thread
-// of the 256-thread block calls it (the team FFT synchronises the block).
+// of the NTHR-thread block calls it (the team FFT synchronises the block).
 // twl holds the N twiddles in LDS (loaded and synchronised by the caller);
-// lds holds TEAMS * TL and red TEAMS * N (TEAMS > 1) entries.
-template <int N, int DET>
+// lds holds TEAMS * TL and red TEAMS * N (TEAMS > 1) entries. The line goes to
+// out (global or LDS) in display order; with TEAMS > 1 it is complete when
+// the call returns, with TEAMS == 1 after the caller's barrier.
+template <int N, int DET, int NTHR = 256>
 __device__ __forceinline__ void disp_frames(const cpx *row, int fhop, int f0, int f1, const float *win,
                                             const cpx *twl, cpx *lds, float *red, float *out) {
 #pragma clang fp contract(off)
-    using D = DispShape<N>;
+    using D = DispShape<N, NTHR>;
     constexpr int PT = D::PT, NT = D::NT, TEAMS = D::TEAMS, TL = D::TL;
     const int tm = threadIdx.x / NT, j = threadIdx.x % NT;
     float w[PT];
@@ -114,7 +118,7 @@ __device__ __forceinline__ void disp_frames(const cpx *row, int fhop, int f0, in
 #pragma unroll
         for (int r = 0; r < PT; ++r) red[tm * N + j + r * NT] = first ? __builtin_nanf("") : acc[r];
         __syncthreads();
-        for (int k = threadIdx.x; k < N; k += 256) {
+        for (int k = threadIdx.x; k < N; k += NTHR) {
             float v = red[k];
             for (int t = 1; t < TEAMS; ++t) {
                 const float u = red[t * N + k];

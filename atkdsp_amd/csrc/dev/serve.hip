@@ -161,100 +161,130 @@ extern "C" void *atkgpu_hmalloc_mapped(size_t bytes, void **dev_ptr) {
 }
 
 // ---- served display lines -----------------------------------------------------------
-// k_srv_disp: one block per run of frames of one element (no run crosses a
-// line boundary) -> its partial line. The element's last block to finish
-// (counted with a device-scope atomic after a release fence) merges every
-// partial of the element in run order into the running line, which lives in
-// device memory across chunks, and at each completed line writes dB
-// (src/fft.c:286-292) and decimate_max (src/display.c:19-37) into the ring.
-// The runs are re-derived from (fa, nf, F, fpb) exactly as the host split
-// them, so the merge needs no per-run marks.
+//   k_srv_frames  one FFT team per frame: window, N-point team FFT, |X|^2/n^2
+//                 in float (reference src/fft.c:177-183), written in display
+//                 order (fftshifted, src/fft.c:190)
+//   k_srv_lines   one block per line segment: the detector over the segment's
+//                 spectra in frame order after the line's running value
+//                 (src/fft.c:184-189), then the running line, or the finished
+//                 line's dB (src/fft.c:286-292) through decimate_max
+//                 (src/display.c:19-37)
+// Both launches are wide (every frame, every segment of the batch at once)
+// and no block waits for another.
+template <int N>
+__global__ void __launch_bounds__(256) k_srv_frames(const atkgpu_frame_blk *__restrict__ blks, const cpx *tw) {
+#pragma clang fp contract(off)
+    using D = DispShape<N, 256>;
+    constexpr int PT = D::PT, NT = D::NT, TL = D::TL;
+    __shared__ cpx lds[D::TEAMS * TL];
+    __shared__ cpx twl[N];
+    const atkgpu_frame_blk jb = blks[blockIdx.x];
+    for (int i = threadIdx.x; i < N; i += 256) twl[i] = tw[i];
+    const int tm = threadIdx.x / NT, j = threadIdx.x % NT;
+    const bool act = tm < jb.nfr;
+    const cpx *fr = (const cpx *)jb.first + (act ? tm : 0) * jb.fhop + j;
+    cpx v[PT];
+#pragma unroll
+    for (int r = 0; r < PT; ++r) {
+        const float w = jb.win ? jb.win[j + r * NT] : 1.0f;
+        const cpx x = fr[r * NT];
+        v[r] = act ? cmake(x.x * w, x.y * w) : cmake(0.f, 0.f);
+    }
+    __syncthreads();
+    team_fft<N, PT>(v, lds + tm * TL, twl, j);
+    if (!act) return;
+    const float inv2 = 1.0f / ((float)N * (float)N);
+    float *P = jb.P + (long long)tm * N;
+    // FFT bin k = j + r*NT sits at display bin (k + N/2) mod N
+#pragma unroll
+    for (int r = 0; r < PT; ++r) P[j + ((r + PT / 2) % PT) * NT] = (v[r].x * v[r].x + v[r].y * v[r].y) * inv2;
+}
+
 template <int N, int DET>
-__device__ __forceinline__ void srv_disp_merge(const atkgpu_disp_blk &jb, float *line) {
+__global__ void __launch_bounds__(256) k_srv_lines(const atkgpu_line_blk *__restrict__ blks) {
 #pragma clang fp contract(off)
     constexpr int NB = N / 256;                // bins a thread: k = threadIdx.x + i*256
-    const long long F = jb.F, fe = jb.fa + jb.nf;
+    __shared__ float line[N];
+    const atkgpu_line_blk jb = blks[blockIdx.x];
     float v[NB];
-    const bool cont = (jb.fa % F) != 0;        // the first frame continues a line
+    int f = 0;
+    if (jb.acc_in) {
 #pragma unroll
-    for (int i = 0; i < NB; ++i) v[i] = cont ? jb.acc[threadIdx.x + i * 256] : 0.0f;
-    int b = 0;
-    for (long long f = jb.fa; f < fe;) {
-        long long se = (f / F + 1) * F;
-        if (se > fe) se = fe;
-        for (long long f0 = f; f0 < se; f0 += jb.fpb, ++b) {
-            const float *p = jb.part0 + (long long)b * ATKGPU_DISP_PART_STRIDE;
-            const bool st = (f0 % F) == 0;
-#pragma unroll
-            for (int i = 0; i < NB; ++i) {
-                const float u = p[threadIdx.x + i * 256];
-                v[i] = st ? u : det_merge<DET>(v[i], u);
-            }
-        }
-        if (se % F == 0) {                     // line se/F - 1 is complete
-#pragma unroll
-            for (int i = 0; i < NB; ++i) line[threadIdx.x + i * 256] = disp_db(v[i], jb.scale);
-            __syncthreads();
-            disp_decimate(line, N, jb.cols, jb.ring + (long long)((se / F - 1) % jb.slots) * jb.lstride);
-            __syncthreads();
-        }
-        f = se;
-    }
-#pragma unroll
-    for (int i = 0; i < NB; ++i) jb.acc[threadIdx.x + i * 256] = v[i];
-}
-
-template <int N, int DET>
-__global__ void __launch_bounds__(256) k_srv_disp(const atkgpu_disp_blk *__restrict__ blks, const cpx *tw) {
-    using D = DispShape<N>;
-    static_assert(D::TEAMS * D::TL * 2 >= N, "the merge's dB line reuses the FFT exchange buffer");
-    __shared__ cpx lds[D::TEAMS * D::TL];
-    __shared__ float red[D::TEAMS > 1 ? D::TEAMS * N : 1];
-    __shared__ cpx twl[N];
-    __shared__ int last;
-    const atkgpu_disp_blk jb = blks[blockIdx.x];
-    for (int i = threadIdx.x; i < N; i += 256) twl[i] = tw[i];
-    __syncthreads();
-    disp_frames<N, DET>((const cpx *)jb.first, jb.fhop, 0, jb.nfr, jb.win, twl, lds, red,
-                        jb.part0 + (long long)jb.b * ATKGPU_DISP_PART_STRIDE);
-    if (jb.nblk > 1) {
-        // publish this partial, then count it; the block that completes the
-        // count sees every partial (acquire) and resets the counter for the
-        // element's next chunk (a later launch on the same stream)
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const unsigned old = __hip_atomic_fetch_add(jb.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            last = old == (unsigned)(jb.nblk - 1);
-            if (last) __hip_atomic_store(jb.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __syncthreads();
-        if (!last) return;
-        __threadfence();
+        for (int i = 0; i < NB; ++i) v[i] = jb.acc_in[threadIdx.x + i * 256];
     } else {
-        __syncthreads();                       // the partial is this block's own
+#pragma unroll
+        for (int i = 0; i < NB; ++i) v[i] = jb.P[threadIdx.x + i * 256];
+        f = 1;
     }
-    srv_disp_merge<N, DET>(jb, (float *)lds);
+    // four frames' loads in flight at a time, merged in frame order
+    for (; f + 4 <= jb.nfr; f += 4) {
+        const float *p = jb.P + (long long)f * N + threadIdx.x;
+        float u[4][NB];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int i = 0; i < NB; ++i) u[q][i] = p[(long long)q * N + i * 256];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int i = 0; i < NB; ++i) v[i] = det_merge<DET>(v[i], u[q][i]);
+    }
+    for (; f < jb.nfr; ++f) {
+        const float *p = jb.P + (long long)f * N;
+#pragma unroll
+        for (int i = 0; i < NB; ++i) v[i] = det_merge<DET>(v[i], p[threadIdx.x + i * 256]);
+    }
+    if (jb.acc_out) {
+#pragma unroll
+        for (int i = 0; i < NB; ++i) jb.acc_out[threadIdx.x + i * 256] = v[i];
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) line[threadIdx.x + i * 256] = disp_db(v[i], jb.scale);
+    __syncthreads();
+    disp_decimate(line, N, jb.cols, jb.line);
 }
 
-template <int N>
-static int launch_srv_disp(const atkgpu_disp_blk *blks, int nblk, int det, const cpx *tw, hipStream_t s) {
-    switch (det) {
-    case 0: hipLaunchKernelGGL((k_srv_disp<N, 0>), dim3(nblk), dim3(256), 0, s, blks, tw); break;
-    case 1: hipLaunchKernelGGL((k_srv_disp<N, 1>), dim3(nblk), dim3(256), 0, s, blks, tw); break;
-    case 2: hipLaunchKernelGGL((k_srv_disp<N, 2>), dim3(nblk), dim3(256), 0, s, blks, tw); break;
+extern "C" int atkgpu_disp_teams(unsigned N) {
+    switch (N) {
+    case 1024: return DispShape<1024, 256>::TEAMS;
+    case 2048: return DispShape<2048, 256>::TEAMS;
+    case 4096: return DispShape<4096, 256>::TEAMS;
+    }
+    return 0;
+}
+
+extern "C" int atkgpu_disp_frames(void *s, const void *blks, int nblk, unsigned N, const void *tw) {
+    if (nblk <= 0) return 0;
+    const atkgpu_frame_blk *b = (const atkgpu_frame_blk *)blks;
+    const dim3 g(nblk), t(256);
+    switch (N) {
+    case 1024: hipLaunchKernelGGL(k_srv_frames<1024>, g, t, 0, (hipStream_t)s, b, (const cpx *)tw); break;
+    case 2048: hipLaunchKernelGGL(k_srv_frames<2048>, g, t, 0, (hipStream_t)s, b, (const cpx *)tw); break;
+    case 4096: hipLaunchKernelGGL(k_srv_frames<4096>, g, t, 0, (hipStream_t)s, b, (const cpx *)tw); break;
     default: return -1;
     }
     return LAUNCH_OK();
 }
 
-extern "C" int atkgpu_disp_jobs(void *s, const void *blks, int nblk, unsigned N, int det, const void *tw) {
+template <int N>
+static int launch_lines(const atkgpu_line_blk *b, int nblk, int det, hipStream_t s) {
+    switch (det) {
+    case 0: hipLaunchKernelGGL((k_srv_lines<N, 0>), dim3(nblk), dim3(256), 0, s, b); break;
+    case 1: hipLaunchKernelGGL((k_srv_lines<N, 1>), dim3(nblk), dim3(256), 0, s, b); break;
+    case 2: hipLaunchKernelGGL((k_srv_lines<N, 2>), dim3(nblk), dim3(256), 0, s, b); break;
+    default: return -1;
+    }
+    return LAUNCH_OK();
+}
+
+extern "C" int atkgpu_disp_lines(void *s, const void *blks, int nblk, unsigned N, int det) {
     if (nblk <= 0) return 0;
-    const atkgpu_disp_blk *b = (const atkgpu_disp_blk *)blks;
+    const atkgpu_line_blk *b = (const atkgpu_line_blk *)blks;
     switch (N) {
-    case 1024: return launch_srv_disp<1024>(b, nblk, det, (const cpx *)tw, (hipStream_t)s);
-    case 2048: return launch_srv_disp<2048>(b, nblk, det, (const cpx *)tw, (hipStream_t)s);
-    case 4096: return launch_srv_disp<4096>(b, nblk, det, (const cpx *)tw, (hipStream_t)s);
+    case 1024: return launch_lines<1024>(b, nblk, det, (hipStream_t)s);
+    case 2048: return launch_lines<2048>(b, nblk, det, (hipStream_t)s);
+    case 4096: return launch_lines<4096>(b, nblk, det, (hipStream_t)s);
     }
     return -1;
 }

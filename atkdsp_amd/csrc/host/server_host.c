@@ -20,11 +20,13 @@
  *                 previous row) and unpacks its bytes after it, every format
  *                 and element together;
  *                 on the compute HIP stream, after that: one fused filterbank
- *                 launch per (M, os, taps) class, a W1 and a W2 launch per
- *                 coherent array stream, and a display launch per (FFT
- *                 length, detector) class (or, with DISPLAY_STREAM, the
- *                 display on a third HIP stream beside the filterbank, joined
- *                 before the batch's completion event).
+ *                 launch per (M, os, taps) class, and a W1 and a W2 launch
+ *                 per coherent array stream;
+ *                 on the display HIP stream, also after the ingest and beside
+ *                 the filterbank: per (FFT length, detector) class, a frames
+ *                 launch (one FFT team per frame) and a lines launch (one
+ *                 block per line segment); the compute stream joins it before
+ *                 the batch's completion event.
  *               Every launch reads one descriptor per block from mapped host
  *               memory (a single PCIe round trip, no search).
  *               Batch k's ingest writes only rows that batches up to k-2 read
@@ -52,24 +54,34 @@
 
 int atkgpu_pfb_prototype(double *h, unsigned nchan, unsigned K, double atten_db);
 
-#define NINFLIGHT 3
+#ifndef NINFLIGHT
+#define NINFLIGHT 3            /* batches launched and not yet complete */
+#endif
 #define POLL_NS 10000          /* completion poll interval */
 #define QCAP 256               /* pending chunks per stream */
 #define LAT_CAP (1u << 22)
 #define NEL_MAX 16             /* elements of an array stream */
 #define NPLANS 7               /* FFT twiddles for 64..4096 points */
-/* blocks of the ingest launch: 4 waves each, one 16-byte load a lane in
- * flight, so INGEST_GRID * 4 KiB of PCIe reads in flight */
+/* blocks of the ingest launch (each strides over the batch's ingest blocks):
+ * 4 waves each, one 16-byte load a lane in flight, so INGEST_GRID * 4 KiB of
+ * PCIe reads in flight. Enough for the link; an uncapped grid (hundreds of
+ * blocks) keeps thousands of waves waiting on PCIe and measured 2-3x slower
+ * filterbank and display kernels beside it. */
 #ifndef INGEST_GRID
-#define INGEST_GRID 0
+#define INGEST_GRID 64
+#endif
+/* 0: the ingest runs on the compute stream, in line with the rest */
+#ifndef INGEST_STREAM
+#define INGEST_STREAM 1
 #endif
 /* 1: the display lines run on their own HIP stream beside the filterbank;
- * 0: after it on the compute stream (measured better at full load, where the
- * concurrent kernels slow each other more than they overlap) */
+ * 0: after it on the compute stream */
 #ifndef DISPLAY_STREAM
-#define DISPLAY_STREAM 0
+#define DISPLAY_STREAM 1
 #endif
-#define DISP_BLK_CAP 4096      /* display frame blocks per batch */
+#define DISP_BLK_CAP 4096      /* display line segments per batch */
+#define FBLK_CAP 8192          /* display frame blocks per batch */
+#define PFLOATS_CAP (32u << 20) /* display power spectra per batch, floats (128 MiB) */
 #define UBLK_CAP 16384         /* ingest blocks per batch */
 #define PBLK_CAP 16384         /* filterbank blocks per batch */
 
@@ -99,8 +111,8 @@ typedef struct {
     /* display (dN == 0: none) */
     unsigned dN, dhop, dF, dcols, dslots;
     int ddet;
-    float *dwin, *dacc, *dring;    /* [dN], [nel][dN], [dslots][nel][dcols] */
-    unsigned *dcnt;                /* [nel] finished-block counts of the display launch */
+    float *dwin, *dacc, *dring;    /* [dN], [2][nel][dN] (double-buffered running lines), [dslots][nel][dcols] */
+    int dpar;                      /* the running-line buffer the next chunk reads */
     long long g_sub, g;            /* samples submitted / dispatched, per element */
     long long fnext;               /* next display frame to transform */
     long long lines_disp, lines_done;
@@ -132,12 +144,11 @@ typedef struct {
     uint32_t *cnt;
     float *pwr, *mean;
     /* display: frames [fa, fa + nf) complete in this chunk */
-    unsigned dN, dhop, dF, dcols, dslots, dfpb;
-    int ddet, nf;
+    unsigned dN, dhop, dF, dcols, dslots;
+    int ddet, nf, dpar;
     long long g0, fa, lines_after;
     const float *dwin;
     float *dacc, *dring;
-    unsigned *dcnt;
 } bitem;
 
 typedef struct {
@@ -147,7 +158,7 @@ typedef struct {
     int n;
     bitem *items;              /* max_batch */
     void *desc_h, *desc_d;     /* mapped descriptor buffer */
-    float *dpart;              /* display partial lines, DISP_BLK_CAP * ATKGPU_DISP_PART_STRIDE */
+    float *dP;                 /* display power spectra, PFLOATS_CAP floats */
 } batch_t;
 
 typedef struct {
@@ -197,7 +208,7 @@ static int plan_index(unsigned M) {
  * ingest (prefix copies and unpacks), filterbank, display */
 static size_t desc_bytes(void) {
     return (size_t)UBLK_CAP * sizeof(atkgpu_unpack_job) + (size_t)PBLK_CAP * sizeof(atkgpu_pfb_job) +
-           (size_t)DISP_BLK_CAP * sizeof(atkgpu_disp_blk) + 256;
+           (size_t)FBLK_CAP * sizeof(atkgpu_frame_blk) + (size_t)DISP_BLK_CAP * sizeof(atkgpu_line_blk) + 256;
 }
 
 static unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
@@ -207,24 +218,13 @@ static long long disp_frames_at(long long g, unsigned N, unsigned hop) {
     return g >= (long long)N ? (g - (long long)N) / hop + 1 : 0;
 }
 
-/* frame blocks for frames [fa, fa + nf): runs of at most fpb frames, none
- * crossing a line boundary (a multiple of F) */
-static int disp_nblk(long long fa, int nf, unsigned F, unsigned fpb) {
-    int cnt = 0;
-    for (long long f = fa, fe = fa + nf; f < fe;) {
-        long long se = (f / F + 1) * F;
-        if (se > fe) se = fe;
-        cnt += (int)((se - f + fpb - 1) / fpb);
-        f = se;
-    }
-    return cnt;
+/* line segments of frames [fa, fa + nf): the runs between line boundaries
+ * (multiples of F) */
+static int disp_nseg(long long fa, int nf, unsigned F) {
+    if (nf <= 0) return 0;
+    const long long fe = fa + nf;
+    return (int)((fe - 1) / F - fa / F + 1);
 }
-
-/* frames per display block, per FFT team of the 256-thread block */
-#ifndef DISP_FRAMES_PER_TEAM
-#define DISP_FRAMES_PER_TEAM 4
-#endif
-static unsigned disp_fpb(unsigned N) { return DISP_FRAMES_PER_TEAM * (N >= 4096 ? 1u : 4096u / N); }
 
 static void *disp_main(void *arg);
 static void *comp_main(void *arg);
@@ -254,7 +254,7 @@ atkgpu_server *atkgpu_server_create(size_t hbm_budget, unsigned max_streams, uns
              !atkgpu_event_create_order(&b->devent) &&
              (b->items = (bitem *)calloc(max_batch, sizeof(bitem))) &&
              (b->desc_h = atkgpu_hmalloc_mapped(desc_bytes(), &b->desc_d)) &&
-             (b->dpart = (float *)atkgpu_dmalloc((size_t)DISP_BLK_CAP * ATKGPU_DISP_PART_STRIDE * sizeof(float)));
+             (b->dP = (float *)atkgpu_dmalloc((size_t)PFLOATS_CAP * sizeof(float)));
     }
     if (!ok) { atkgpu_server_destroy(v); return NULL; }
     pthread_mutex_init(&v->mu, NULL);
@@ -280,7 +280,6 @@ static void stream_free(sstream *s) {
     atkgpu_dfree(s->dwin);
     atkgpu_dfree(s->dacc);
     atkgpu_dfree(s->dring);
-    atkgpu_dfree(s->dcnt);
     free(s->slot_m);
     memset(s, 0, sizeof *s);
 }
@@ -308,9 +307,9 @@ void atkgpu_server_destroy(atkgpu_server *v) {
         atkgpu_event_destroy(v->batch[i].event);
         atkgpu_event_destroy(v->batch[i].uevent);
         atkgpu_event_destroy(v->batch[i].devent);
+        atkgpu_dfree(v->batch[i].dP);
         free(v->batch[i].items);
         atkgpu_hfree(v->batch[i].desc_h);
-        atkgpu_dfree(v->batch[i].dpart);
     }
     for (int i = 0; i < NPLANS; ++i) atkgpu_fftplan_destroy(v->plans[i]);
     if (v->stream) atkgpu_stream_destroy(v->stream);
@@ -343,9 +342,13 @@ int atkgpu_server_add_stream(atkgpu_server *v, const atkgpu_stream_cfg *cfg) {
         if ((dN != 1024 && dN != 2048 && dN != 4096) || dhop > dN || cfg->disp_frames < 1 || dcols > dN ||
             cfg->disp_window < 0 || cfg->disp_window > 3 || cfg->disp_detector < 0 || cfg->disp_detector > 2)
             return ATKDSP_E_ARG;
-        /* a chunk's frame blocks must fit one batch with room to spare */
+        /* a chunk's frames, frame blocks and line segments must fit one batch
+         * with room to spare */
         const size_t fmax = cfg->chunk_max / dhop + 1;
-        if ((size_t)nel * (fmax / disp_fpb(dN) + fmax / cfg->disp_frames + 4) > DISP_BLK_CAP / 2) return ATKDSP_E_ARG;
+        if ((size_t)nel * (fmax / cfg->disp_frames + 2) > DISP_BLK_CAP / 2 ||
+            (size_t)nel * fmax * dN > PFLOATS_CAP / 2 ||
+            (size_t)nel * (fmax / (size_t)atkgpu_disp_teams(dN) + 1) > FBLK_CAP / 2)
+            return ATKDSP_E_ARG;
     }
     /* the kept prefix holds the filterbank's history and, with a display,
      * the unfinished frame (< dN samples) */
@@ -362,7 +365,7 @@ int atkgpu_server_add_stream(atkgpu_server *v, const atkgpu_stream_cfg *cfg) {
     if (cfg->coherent)
         bytes += (size_t)M * nel * nel * sizeof(atkdsp_cf32) + (size_t)M * sizeof(uint32_t) +
                  (size_t)(nel + 1) * M * sizeof(float);
-    if (dN) bytes += ((size_t)(nel + 1) * dN + (size_t)dslots * nel * dcols + nel) * sizeof(float);
+    if (dN) bytes += ((size_t)(2 * nel + 1) * dN + (size_t)dslots * nel * dcols) * sizeof(float);
     pthread_mutex_lock(&v->mu);
     int sid = -1;
     for (unsigned i = 0; i < v->max_streams; ++i)
@@ -400,13 +403,11 @@ int atkgpu_server_add_stream(atkgpu_server *v, const atkgpu_stream_cfg *cfg) {
         s->dN = dN; s->dhop = dhop; s->dF = cfg->disp_frames; s->dcols = dcols; s->dslots = dslots;
         s->ddet = cfg->disp_detector;
         s->dwin = (float *)atkgpu_dmalloc((size_t)dN * sizeof(float));
-        s->dacc = (float *)atkgpu_dmalloc((size_t)nel * dN * sizeof(float));
+        s->dacc = (float *)atkgpu_dmalloc((size_t)2 * nel * dN * sizeof(float));
         s->dring = (float *)atkgpu_dmalloc((size_t)dslots * nel * dcols * sizeof(float));
-        s->dcnt = (unsigned *)atkgpu_dmalloc((size_t)nel * sizeof(unsigned));
         w = (float *)malloc((size_t)dN * sizeof(float));
-        if (!s->dwin || !s->dacc || !s->dring || !s->dcnt || !w) rc = ATKDSP_E_NOMEM;
+        if (!s->dwin || !s->dacc || !s->dring || !w) rc = ATKDSP_E_NOMEM;
         if (!rc) rc = atkdsp_window(cfg->disp_window, dN, w);
-        if (!rc) rc = atkgpu_fill_u32(v->stream, s->dcnt, 0, nel);
         if (!rc) rc = atkgpu_h2d(v->stream, s->dwin, w, (size_t)dN * sizeof(float));
     }
     double *h = (double *)malloc((size_t)M * K * sizeof(double));
@@ -571,10 +572,12 @@ static int launch_batch(atkgpu_server *v, batch_t *bt, unsigned k) {
         }
     }
     /* batch k's ingest overwrites the rows batch k-2 read */
-    if (!rc && k >= 2) rc = atkgpu_stream_wait_event(v->ustream, v->batch[(k - 2) % NINFLIGHT].event);
-    if (!rc && nu) rc = atkgpu_unpack_jobs(v->ustream, DEV(uj), nu, nu, 1, INGEST_GRID);
-    if (!rc) rc = atkgpu_event_record(bt->uevent, v->ustream);
-    if (!rc) rc = atkgpu_stream_wait_event(v->stream, bt->uevent);
+    void *ust = INGEST_STREAM ? v->ustream : v->stream;
+    if (!rc && INGEST_STREAM && k >= 2)
+        rc = atkgpu_stream_wait_event(v->ustream, v->batch[(k - 2) % NINFLIGHT].event);
+    if (!rc && nu) rc = atkgpu_unpack_jobs(ust, DEV(uj), nu, nu, 1, INGEST_GRID);
+    if (!rc) rc = atkgpu_event_record(bt->uevent, ust);
+    if (!rc && INGEST_STREAM) rc = atkgpu_stream_wait_event(v->stream, bt->uevent);
     void *dst = DISPLAY_STREAM ? v->dstream : v->stream;
     if (!rc && DISPLAY_STREAM) rc = atkgpu_stream_wait_event(v->dstream, bt->uevent);
     /* the filterbank, one launch per shape class present, one descriptor per
@@ -616,56 +619,59 @@ static int launch_batch(atkgpu_server *v, batch_t *bt, unsigned k) {
         rc = atkgpu_pfb_cov(v->stream, it->out, it->nel, it->M, it->m, it->m_max, NULL, it->R, it->cnt);
         if (!rc) rc = atkgpu_pfb_power(v->stream, it->out, it->nel, it->M, it->m, it->m_max, it->pwr, it->mean);
     }
-    /* display: per (FFT length, detector) class present, the frame runs of
-     * every element (frame f of element e at row_e + keep + f*hop - g0); the
-     * element's last block merges them into its running line and ring */
-    atkgpu_disp_blk *db = (atkgpu_disp_blk *)(pj + PBLK_CAP);
-    int nb = 0;
+    /* display: per (FFT length, detector) class present, the frames launch
+     * (each element's frames in order, spectra contiguous) and the lines
+     * launch (one block per line segment; frame f of element e at row_e +
+     * keep + f*hop - g0; running lines read from buffer dpar, written to the
+     * other) */
+    atkgpu_frame_blk *fb = (atkgpu_frame_blk *)(pj + PBLK_CAP);
+    atkgpu_line_blk *lb = (atkgpu_line_blk *)(fb + FBLK_CAP);
+    int nfb = 0, nb = 0;
+    size_t pofs = 0;
     for (unsigned N = 1024; N <= 4096 && !rc; N *= 2) {
+        const int TEAMS = atkgpu_disp_teams(N);
         for (int det = 0; det < 3 && !rc; ++det) {
-            const int b0 = nb;
+            const int f0b = nfb, l0b = nb;
             for (int i = 0; i < n; ++i) {
                 const bitem *it = &bt->items[i];
                 if (!it->nf || it->dN != N || it->ddet != det) continue;
-                const unsigned F = it->dF, fpb = it->dfpb;
-                const int nblk = disp_nblk(it->fa, it->nf, F, fpb);
+                const long long F = it->dF;
                 for (unsigned e = 0; e < it->nel; ++e) {
                     const atkdsp_cf32 *row = it->row + e * it->row_len;
-                    float *part0 = bt->dpart + (size_t)nb * ATKGPU_DISP_PART_STRIDE;
-                    int b = 0;
+                    float *Pe = bt->dP + pofs;
+                    pofs += (size_t)it->nf * N;
+                    for (int t = 0; t < it->nf; t += TEAMS) {
+                        atkgpu_frame_blk *d = &fb[nfb++];
+                        d->first = row + (it->keep + (it->fa + t) * it->dhop - it->g0);
+                        d->win = it->dwin;
+                        d->P = Pe + (size_t)t * N;
+                        d->nfr = it->nf - t < TEAMS ? it->nf - t : TEAMS;
+                        d->fhop = (int)it->dhop;
+                    }
+                    const float *acc_in = it->dacc + ((size_t)it->dpar * it->nel + e) * N;
+                    float *acc_out = it->dacc + ((size_t)(it->dpar ^ 1) * it->nel + e) * N;
                     for (long long f = it->fa, fe = it->fa + it->nf; f < fe;) {
                         long long se = (f / F + 1) * F;
                         if (se > fe) se = fe;
-                        for (long long f0 = f; f0 < se; f0 += fpb, ++b) {
-                            const long long f1 = f0 + fpb < se ? f0 + fpb : se;
-                            atkgpu_disp_blk *d = &db[nb++];
-                            d->first = row + (it->keep + f0 * it->dhop - it->g0);
-                            d->win = it->dwin;
-                            d->part0 = part0;
-                            d->acc = it->dacc + (size_t)e * N;
-                            d->ring = it->dring + (size_t)e * it->dcols;
-                            d->counter = it->dcnt + e;
-                            d->fa = it->fa;
-                            d->nf = it->nf;
-                            d->F = (int)F;
-                            d->fpb = (int)fpb;
-                            d->nblk = nblk;
-                            d->b = b;
-                            d->nfr = (int)(f1 - f0);
-                            d->fhop = (int)it->dhop;
-                            d->cols = (int)it->dcols;
-                            d->slots = (int)it->dslots;
-                            d->lstride = (int)(it->nel * it->dcols);
-                            d->scale = det == 2 ? 1.0f / (float)F : 1.0f;
-                            d->pad_ = 0;
-                        }
+                        const int ends = se % F == 0;
+                        atkgpu_line_blk *d = &lb[nb++];
+                        d->P = Pe + (size_t)(f - it->fa) * N;
+                        d->acc_in = (f % F) ? acc_in : NULL;
+                        d->acc_out = ends ? NULL : acc_out;
+                        d->line = ends ? it->dring + ((size_t)((se / F - 1) % it->dslots) * it->nel + e) * it->dcols
+                                       : NULL;
+                        d->nfr = (int)(se - f);
+                        d->cols = (int)it->dcols;
+                        d->scale = det == 2 ? 1.0f / (float)F : 1.0f;
+                        d->pad_ = 0;
                         f = se;
                     }
                 }
             }
-            if (nb > b0)
-                rc = atkgpu_disp_jobs(dst, DEV(&db[b0]), nb - b0, N, det,
-                                      atkgpu_fftplan_twiddles(v->plans[plan_index(N)]));
+            if (nfb > f0b)
+                rc = atkgpu_disp_frames(dst, DEV(&fb[f0b]), nfb - f0b, N,
+                                        atkgpu_fftplan_twiddles(v->plans[plan_index(N)]));
+            if (!rc && nb > l0b) rc = atkgpu_disp_lines(dst, DEV(&lb[l0b]), nb - l0b, N, det);
         }
     }
 #undef DEV
@@ -679,7 +685,8 @@ static int launch_batch(atkgpu_server *v, batch_t *bt, unsigned k) {
  * round-robin; mutex held */
 static int form_batch(atkgpu_server *v, batch_t *bt) {
     int n = 0;
-    unsigned jobs = 0, dblk = 0, ublk = 0, pblk = 0;
+    unsigned jobs = 0, dblk = 0, ublk = 0, pblk = 0, fblk = 0;
+    size_t pfl = 0;
     const unsigned S = v->max_streams;
     for (unsigned t = 0; t < S && n < (int)v->max_batch && n < 4096; ++t) {
         const unsigned sid = (v->rr + t) % S;
@@ -688,13 +695,15 @@ static int form_batch(atkgpu_server *v, batch_t *bt) {
         if (jobs + s->nel > v->jobcap) continue;
         /* display frames this chunk completes, and their blocks */
         int nf = 0;
-        unsigned fpb = 0, nb = 0;
+        unsigned nb = 0, nfb = 0;
+        size_t npf = 0;
         if (s->dN) {
             const long long fe = disp_frames_at(s->g + (long long)s->q[s->qh].n, s->dN, s->dhop);
             nf = (int)(fe - s->fnext);
-            fpb = disp_fpb(s->dN);
-            nb = s->nel * (unsigned)disp_nblk(s->fnext, nf, s->dF, fpb);
-            if (dblk + nb > DISP_BLK_CAP) continue;
+            nb = s->nel * (unsigned)disp_nseg(s->fnext, nf, s->dF);
+            nfb = s->nel * cdiv((size_t)nf, (size_t)atkgpu_disp_teams(s->dN));
+            npf = (size_t)s->nel * nf * s->dN;
+            if (dblk + nb > DISP_BLK_CAP || fblk + nfb > FBLK_CAP || pfl + npf > PFLOATS_CAP) continue;
         }
         /* its ingest and filterbank blocks */
         const size_t cn = s->q[s->qh].n;
@@ -704,6 +713,8 @@ static int form_batch(atkgpu_server *v, batch_t *bt) {
         ublk += nu;
         pblk += np;
         dblk += nb;
+        fblk += nfb;
+        pfl += npf;
         const chunk_t c = s->q[s->qh];
         s->qh = (s->qh + 1) % QCAP;
         s->qn--;
@@ -740,14 +751,15 @@ static int form_batch(atkgpu_server *v, batch_t *bt) {
         it->dN = s->dN;
         it->nf = nf;
         if (s->dN) {
-            it->dhop = s->dhop; it->dF = s->dF; it->dcols = s->dcols; it->dslots = s->dslots; it->dfpb = fpb;
+            it->dhop = s->dhop; it->dF = s->dF; it->dcols = s->dcols; it->dslots = s->dslots;
             it->ddet = s->ddet;
             it->g0 = s->g;
             it->fa = s->fnext;
             it->dwin = s->dwin;
             it->dacc = s->dacc;
             it->dring = s->dring;
-            it->dcnt = s->dcnt;
+            it->dpar = s->dpar;
+            if (nf) s->dpar ^= 1;          /* the chunk leaves its running line in the other buffer */
             s->fnext += nf;
             s->lines_disp = s->fnext / s->dF;
             it->lines_after = s->lines_disp;
