@@ -341,9 +341,11 @@ def main() -> int:
                    "nfft": NFFT, "fft_hop": FHOP, "parallelism": "single GPU"},
         "p50_chunk_latency_ms": uni.get("p50_ms"),
         "p99_chunk_latency_ms": uni.get("p99_ms"),
-        "latency_note": (f"serving loop, {S} streams of 40 MSPS ci16 into M=1024 K=16 2x filterbanks, chunks of "
-                         "1..65536 samples arriving as Poisson processes at the real rate; latency = host "
-                         "submit call to the chunk's batch completing on the device, host monotonic clock"),
+        "latency_note": (f"serving loop, {S} streams of 40 MSPS ci16 into M=1024 K=16 2x filterbanks and "
+                         "4096-point display lines (hop 2048, Hann, max-hold over 50 ms, 1024 columns), chunks "
+                         "of 1..65536 samples arriving as Poisson processes at the real rate; latency = host "
+                         "submit call to the device completion of the chunk's batch (its filterbank output "
+                         "and every display line it completes), host monotonic clock"),
         "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                             "p99": round(float(np.percentile(lat, 99)), 3),
                             "note": "one chained step (a 50 ms block of every stream) submitted to completed"},
