@@ -100,6 +100,26 @@ def _bind(lib) -> None:
         "atkdsp_nco_init": (None, [P(_Nco), d, d]),
         "atkdsp_nco_set_freq": (None, [P(_Nco), d, d]),
         "atkdsp_nco_mix": (None, [P(_Nco), cf, cf, sz]),
+        "atkdsp_design_lowpass": (C.c_ssize_t, [d, d, d, d, fp, sz]),
+        "atkdsp_arb_resampler_create": (vp, [d, d, d, u]),
+        "atkdsp_arb_resampler_destroy": (None, [vp]),
+        "atkdsp_arb_resampler_reset": (None, [vp]),
+        "atkdsp_arb_resampler_set_ratio": (None, [vp, d]),
+        "atkdsp_arb_resampler_ratio": (d, [vp]),
+        "atkdsp_arb_resampler_out_max": (sz, [vp, sz]),
+        "atkdsp_arb_resampler_process": (C.c_ssize_t, [vp, cf, sz, cf, sz]),
+        "atkdsp_osx_create": (vp, [d, sz, d, u]),
+        "atkdsp_osx_destroy": (None, [vp]),
+        "atkdsp_osx_reset": (None, [vp]),
+        "atkdsp_osx_add": (C.c_int, [vp, d, d, d]),
+        "atkdsp_osx_retune": (C.c_int, [vp, C.c_int, d]),
+        "atkdsp_osx_remove": (C.c_int, [vp, C.c_int]),
+        "atkdsp_osx_channels": (C.c_int, [vp]),
+        "atkdsp_osx_chan_rate": (d, [vp, C.c_int]),
+        "atkdsp_osx_out_max": (sz, [vp, C.c_int, sz]),
+        "atkdsp_osx_process": (C.c_ssize_t, [vp, cf, sz, P(cf), P(sz), sz]),
+        "atkdsp_osx_process_n": (C.c_ssize_t, [vp, P(cf), u, sz, P(cf), P(sz), sz]),
+        "atkdsp_osx_describe": (C.c_int, [vp, C.c_char_p, sz]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -521,10 +541,139 @@ def pfb_channel_power(out, nel, nchan):
     return pwr.reshape(nel, nchan), meanp
 
 
+def design_lowpass(fp_hz: float, fs_hz: float, atten_db: float, rate: float) -> np.ndarray:
+    """Kaiser-windowed sinc lowpass (include/atkdsp.h:505-506): odd length, unit DC gain."""
+    lib = load()
+    n = _check(int(lib.atkdsp_design_lowpass(float(fp_hz), float(fs_hz), float(atten_db), float(rate), None, 0)),
+               "design_lowpass")
+    out = np.zeros(n, np.float32)
+    _check(int(lib.atkdsp_design_lowpass(float(fp_hz), float(fs_hz), float(atten_db), float(rate), _fp(out), n)),
+           "design_lowpass")
+    return out
+
+
+class ArbResampler:
+    """Resample by any positive out/in ratio (atkdsp_arb_resampler, reference
+    include/atkdsp.h:368-377): a polyphase bank with linear interpolation
+    between phases; the position and history carry across calls."""
+
+    def __init__(self, in_rate: float, out_rate: float, atten_db: float = 60.0, nphase: int = 64):
+        self._lib = load()
+        self._h = self._lib.atkdsp_arb_resampler_create(float(in_rate), float(out_rate), float(atten_db),
+                                                        int(nphase))
+        if not self._h:
+            raise AtkDspError("arb_resampler_create failed (no GPU or bad rates)")
+        self.in_rate, self.out_rate = float(in_rate), float(out_rate)
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._lib.atkdsp_arb_resampler_destroy(h)
+
+    def reset(self) -> None:
+        self._lib.atkdsp_arb_resampler_reset(self._h)
+
+    def set_ratio(self, ratio: float) -> None:
+        self._lib.atkdsp_arb_resampler_set_ratio(self._h, float(ratio))
+        self.out_rate = self.in_rate * float(ratio)
+
+    def ratio(self) -> float:
+        return float(self._lib.atkdsp_arb_resampler_ratio(self._h))
+
+    def out_max(self, n_in: int) -> int:
+        return int(self._lib.atkdsp_arb_resampler_out_max(self._h, int(n_in)))
+
+    def process(self, x, out: np.ndarray | None = None) -> np.ndarray:
+        x = _cf32(x, "x")
+        if out is None:
+            out = np.empty(self.out_max(x.size), np.complex64)
+        got = _check(int(self._lib.atkdsp_arb_resampler_process(self._h, _cfp(x), x.size, _cfp(out), out.size)),
+                     "arb_resampler_process")
+        return out[:got]
+
+
+class Osx:
+    """Overlap-save extractor (atkdsp_osx, reference include/atkdsp.h:3252-3289):
+    arbitrary channels at arbitrary centres from one forward FFT per block,
+    each landed on its own out_rate; add/retune/remove between calls without
+    disturbing the others; process_n runs N elements phase-coherently."""
+
+    def __init__(self, fs, nfft=4096, atten_db=80.0, max_nel=1):
+        self._lib = load()
+        self._p = self._lib.atkdsp_osx_create(float(fs), int(nfft), float(atten_db), int(max_nel))
+        if not self._p:
+            raise AtkDspError("osx_create: check fs>0, nfft a power of two >= 64 (and a GPU)")
+        self.max_nel = int(max_nel)
+
+    def __del__(self):
+        p, self._p = getattr(self, "_p", None), None
+        if p:
+            self._lib.atkdsp_osx_destroy(p)
+
+    def reset(self):
+        self._lib.atkdsp_osx_reset(self._p)
+
+    def add(self, offset_hz, bw_hz, out_rate):
+        ch = int(self._lib.atkdsp_osx_add(self._p, float(offset_hz), float(bw_hz), float(out_rate)))
+        return _check(ch, "osx_add")
+
+    def retune(self, ch, offset_hz):
+        _check(self._lib.atkdsp_osx_retune(self._p, int(ch), float(offset_hz)), "osx_retune")
+
+    def remove(self, ch):
+        _check(self._lib.atkdsp_osx_remove(self._p, int(ch)), "osx_remove")
+
+    def channels(self):
+        return int(self._lib.atkdsp_osx_channels(self._p))
+
+    def chan_rate(self, ch):
+        return float(self._lib.atkdsp_osx_chan_rate(self._p, int(ch)))
+
+    def out_max(self, ch, n):
+        return int(self._lib.atkdsp_osx_out_max(self._p, int(ch), int(n)))
+
+    def describe(self):
+        buf = C.create_string_buffer(256)
+        n = self._lib.atkdsp_osx_describe(self._p, buf, 256)
+        return buf.value.decode() if n > 0 else ""
+
+    def process_n(self, ins, nel):
+        """outs[ch] is [nel][count] complex64 (None for a free slot); counts[ch]."""
+        nel = int(nel)
+        arrs = [_cf32(np.asarray(ins[e]).ravel(), "in") for e in range(nel)]
+        n = arrs[0].size
+        ns = self.channels()
+        active = [self.chan_rate(ci) > 0 for ci in range(ns)]
+        cap_each = max([self.out_max(ci, n) for ci in range(ns) if active[ci]], default=0)
+        bufs = [[(np.zeros(cap_each, np.complex64) if active[ci] else None) for ci in range(ns)]
+                for _ in range(nel)]
+        ptrs = (C.POINTER(_Cf32) * max(1, nel * ns))()
+        for e in range(nel):
+            for ci in range(ns):
+                ptrs[e * ns + ci] = _cfp(bufs[e][ci]) if active[ci] else None
+        counts = (C.c_size_t * max(1, ns))()
+        inarr = (C.POINTER(_Cf32) * nel)(*[_cfp(a) for a in arrs])
+        _check(int(self._lib.atkdsp_osx_process_n(self._p, inarr, nel, n, ptrs, counts, cap_each)),
+               "osx_process_n")
+        cnts = [int(counts[ci]) for ci in range(ns)]
+        outs = [None] * ns
+        for ci in range(ns):
+            if active[ci]:
+                m = cnts[ci]
+                outs[ci] = (np.array([bufs[e][ci][:m] for e in range(nel)], np.complex64)
+                            if m else np.zeros((nel, 0), np.complex64))
+        return outs, cnts
+
+    def process(self, x):
+        outs, cnts = self.process_n([x], 1)
+        return [o[0] if o is not None else None for o in outs], cnts
+
+
 __all__ = [
     "ABI_VERSION", "AtkDspError", "load", "available", "lib_path", "version", "build_info",
     "set_threads", "get_threads", "FMT", "DET", "WIN", "bytes_per_sample",
     "unpack", "unpack_dc", "Fft", "window", "power_db", "spectrum_reduce", "spectrum_stats",
     "window_stats", "db_to_pixels", "decimate_max", "stitch_max",
     "Pfb", "pfb_cov_accumulate", "pfb_cov_reset", "pfb_channel_power",
+    "design_lowpass", "ArbResampler", "Osx",
 ]

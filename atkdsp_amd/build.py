@@ -49,6 +49,12 @@ IMPLEMENTED = {
     "atkdsp_pfb_describe", "atkdsp_pfb_cov_accumulate", "atkdsp_pfb_cov_reset",
     "atkdsp_pfb_channel_power",
     "atkdsp_nco_init", "atkdsp_nco_set_freq", "atkdsp_nco_mix",
+    "atkdsp_arb_resampler_create", "atkdsp_arb_resampler_destroy", "atkdsp_arb_resampler_reset",
+    "atkdsp_arb_resampler_set_ratio", "atkdsp_arb_resampler_ratio", "atkdsp_arb_resampler_out_max",
+    "atkdsp_arb_resampler_process", "atkdsp_design_lowpass",
+    "atkdsp_osx_create", "atkdsp_osx_destroy", "atkdsp_osx_reset", "atkdsp_osx_add", "atkdsp_osx_retune",
+    "atkdsp_osx_remove", "atkdsp_osx_channels", "atkdsp_osx_chan_rate", "atkdsp_osx_out_max",
+    "atkdsp_osx_process", "atkdsp_osx_process_n", "atkdsp_osx_describe",
 }
 
 _STUB_BODY = {

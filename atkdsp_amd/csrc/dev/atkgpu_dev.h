@@ -189,6 +189,23 @@ typedef struct {
     float scale;               /* 1/frames-per-line for the average detector, else 1 */
     int pad_;
 } atkgpu_line_blk;
+/* ---- the overlap-save extractor and the arbitrary resampler (osx.hip) ----
+ * fold: for nt blocks t (forward transforms at X + t*nfft),
+ *   Yf[t*M + k] = sum_j X[t][(k + j*M + sh) & (nfft-1)] * H[k + j*M], j < decim;
+ * mix: valid[e*vpitch + i*(M-ovd) + k] = ys[(e*m+i)*M + ovd + k] / decim times
+ *   exp(-2 pi i (((sh*start) mod nfft)/nfft + delta*nabs/fs)) in FP64, start =
+ *   (blk0+i)*hop - ov, nabs = (blk0+i)*hop + k*decim;
+ * rs_run: outputs K0 .. K0+count-1 of a call over n inputs per element
+ *   (element e at in + e*ipitch), history hist_in ([nel][Q]) -> hist_out. */
+int atkgpu_osx_fold(void *s, const void *X, const void *H, void *Yf, size_t nt, unsigned nfft, unsigned M,
+                    unsigned decim, unsigned sh);
+int atkgpu_osx_mix(void *s, const void *ys, void *valid, size_t vpitch, unsigned nel, size_t m, unsigned M,
+                   unsigned ovd, unsigned decim, unsigned hop, unsigned ov, unsigned nfft, unsigned sh,
+                   long long blk0, double delta, double fs);
+int atkgpu_rs_run(void *s, const void *in, size_t ipitch, size_t n, unsigned nel, const void *hist_in,
+                  void *hist_out, const float *p, const float *d, unsigned P, unsigned Q, double base,
+                  double step, unsigned long long K0, unsigned long long in_off, void *out, size_t opitch,
+                  size_t count);
 int atkgpu_disp_teams(unsigned N);          /* frames one block of the frames launch transforms */
 int atkgpu_disp_frames(void *s, const void *blks, int nblk, unsigned N, const void *tw);
 int atkgpu_disp_lines(void *s, const void *blks, int nblk, unsigned N, int det);

@@ -18,7 +18,7 @@
  *   - NaN means "unmeasured" (no fast-math anywhere).
  *
  * The library also exports every other name of the reference ABI (334 in
- * all; 38 are implemented here) as a documented stub: *_create returns NULL, signed counts return
+ * all; 58 are implemented here) as a documented stub: *_create returns NULL, signed counts return
  * ATKDSP_E_ARG, unsigned counts 0, floating results NaN, void functions do
  * nothing. Those are outside the wideband front end.
  */
@@ -126,6 +126,29 @@ ATKDSP_API void atkdsp_nco_init(atkdsp_nco *nco, double freq_hz, double sample_r
 ATKDSP_API void atkdsp_nco_set_freq(atkdsp_nco *nco, double freq_hz, double sample_rate);
 ATKDSP_API void atkdsp_nco_mix(atkdsp_nco *nco, const atkdsp_cf32 *in, atkdsp_cf32 *out, size_t n);
 
+/* ---- arbitrary-ratio resampler: reference include/atkdsp.h:368-377,
+ * src/resample.c:100-252. A 64-phase (nphase) bank of the Kaiser lowpass below
+ * with linear interpolation toward the next phase; output K sits at input
+ * position base + K*in/out, carried across calls, so block boundaries are
+ * invisible. The FIR runs on the GPU; a call whose outputs exceed out_cap
+ * returns ATKDSP_E_CAP with the state unchanged. */
+typedef struct atkdsp_arb_resampler atkdsp_arb_resampler;
+ATKDSP_API atkdsp_arb_resampler *atkdsp_arb_resampler_create(double in_rate, double out_rate,
+                                                             double atten_db, unsigned nphase);
+ATKDSP_API void      atkdsp_arb_resampler_destroy(atkdsp_arb_resampler *r);
+ATKDSP_API void      atkdsp_arb_resampler_reset(atkdsp_arb_resampler *r);
+ATKDSP_API void      atkdsp_arb_resampler_set_ratio(atkdsp_arb_resampler *r, double ratio);
+ATKDSP_API double    atkdsp_arb_resampler_ratio(const atkdsp_arb_resampler *r);
+ATKDSP_API size_t    atkdsp_arb_resampler_out_max(const atkdsp_arb_resampler *r, size_t n_in);
+ATKDSP_API ptrdiff_t atkdsp_arb_resampler_process(atkdsp_arb_resampler *r, const atkdsp_cf32 *in,
+                                                  size_t n, atkdsp_cf32 *out, size_t out_cap);
+
+/* ---- Kaiser lowpass design: reference include/atkdsp.h:505-506,
+ * src/design.c:5-62. Odd length, unit DC gain, FP64 design cast once to
+ * float; out == NULL returns the length. Host code (a design-time helper). */
+ATKDSP_API ptrdiff_t atkdsp_design_lowpass(double fp_hz, double fs_hz, double atten_db,
+                                           double rate, float *out, size_t out_cap);
+
 /* ---- polyphase filterbank: reference include/atkdsp.h:3191-3250,
  * src/channelizer.c:79-471. Output layouts, bookkeeping, cal and W1/W2 are
  * the reference's; the streaming history (ov samples per element) stays in
@@ -157,6 +180,33 @@ ATKDSP_API void        atkdsp_pfb_cov_reset(atkdsp_cf32 *R, uint32_t *counts, un
 ATKDSP_API void        atkdsp_pfb_channel_power(const atkdsp_cf32 *out, unsigned nel,
                                                 unsigned nchan, size_t m,
                                                 float *pwr, float *mean_pwr);
+
+/* ---- overlap-save extractor: reference include/atkdsp.h:3252-3289,
+ * src/channelizer.c:473-788. One nfft-point forward FFT per block (overlap
+ * nfft/2); per channel the roll to its integer bin, its lowpass's frequency
+ * response, the fold to M = nfft/decim and the M-point inverse FFT (the
+ * decimation), the ov/decim transient dropped, the mix to absolute time in
+ * FP64, then the arbitrary resampler to out_rate when the channel rate
+ * differs. All of it runs on the GPU; the history stays in device memory.
+ * Channels are independent: add, retune and remove leave the others
+ * bit-identical. outs is [nel][nslots] (NULL for a free slot); a call whose
+ * outputs exceed cap_each returns ATKDSP_E_CAP with the state unchanged. */
+typedef struct atkdsp_osx atkdsp_osx;
+ATKDSP_API atkdsp_osx *atkdsp_osx_create(double fs, size_t nfft, double atten_db, unsigned max_nel);
+ATKDSP_API void        atkdsp_osx_destroy(atkdsp_osx *x);
+ATKDSP_API void        atkdsp_osx_reset(atkdsp_osx *x);
+ATKDSP_API int         atkdsp_osx_add(atkdsp_osx *x, double offset_hz, double bw_hz, double out_rate);
+ATKDSP_API int         atkdsp_osx_retune(atkdsp_osx *x, int ch, double offset_hz);
+ATKDSP_API int         atkdsp_osx_remove(atkdsp_osx *x, int ch);
+ATKDSP_API int         atkdsp_osx_channels(const atkdsp_osx *x);
+ATKDSP_API double      atkdsp_osx_chan_rate(const atkdsp_osx *x, int ch);
+ATKDSP_API size_t      atkdsp_osx_out_max(const atkdsp_osx *x, int ch, size_t n);
+ATKDSP_API ptrdiff_t   atkdsp_osx_process(atkdsp_osx *x, const atkdsp_cf32 *in, size_t n,
+                                          atkdsp_cf32 *const *outs, size_t *out_counts, size_t cap_each);
+ATKDSP_API ptrdiff_t   atkdsp_osx_process_n(atkdsp_osx *x, const atkdsp_cf32 *const *ins, unsigned nel,
+                                            size_t n, atkdsp_cf32 *const *outs, size_t *out_counts,
+                                            size_t cap_each);
+ATKDSP_API int         atkdsp_osx_describe(const atkdsp_osx *x, char *buf, size_t cap);
 
 #ifdef __cplusplus
 }

@@ -446,6 +446,214 @@ def nco_exact_phase(n: int, p: int, q: int) -> float:
     return wrap(-2.0 * math.pi * ((n * p) % q) / q)
 
 
+# ---- Kaiser lowpass: src/design.c:24-62 ------------------------------------------------
+def kaiser_length(atten_db: float, transition_hz: float, rate: float) -> int:
+    """src/design.c:24-32."""
+    if transition_hz <= 0.0 or rate <= 0.0:
+        return 0
+    dw = 2.0 * math.pi * transition_hz / rate
+    n = (atten_db - 8.0 if atten_db > 8.0 else 1.0) / (2.285 * dw)
+    N = int(math.ceil(n)) + 1
+    N = max(N, 3)
+    return N + 1 if N % 2 == 0 else N
+
+
+def design_lowpass(fp_hz: float, fs_hz: float, atten_db: float, rate: float) -> np.ndarray:
+    """src/design.c:34-62: FP64 Kaiser-windowed sinc, normalised by the
+    sequential FP64 sum of the taps, each tap cast once to float32."""
+    if rate <= 0.0 or fp_hz < 0.0 or fs_hz <= fp_hz or fs_hz > rate * 0.5 + 1e-9:
+        raise ValueError("design_lowpass: bad edges")
+    N = kaiser_length(atten_db, fs_hz - fp_hz, rate)
+    beta = kaiser_beta(atten_db)
+    i0b = _i0_series(np.array([beta]))[0]
+    fc = 0.5 * (fp_hz + fs_hz) / rate
+    M = (N - 1) * 0.5
+    t = np.arange(N, dtype=np.float64) - M
+    x = 2.0 * fc * t
+    with np.errstate(invalid="ignore", divide="ignore"):
+        sinc = np.where(t == 0.0, 1.0, np.sin(math.pi * x) / (math.pi * x))
+    r = t / M
+    rr = r * r
+    w = _i0_series(beta * np.sqrt(np.where(rr < 1.0, 1.0 - rr, 0.0))) / i0b
+    h = 2.0 * fc * sinc * w
+    total = float(np.cumsum(h)[-1])               # the reference's sequential sum
+    if total == 0.0:
+        total = 1.0
+    return (h / total).astype(F32)
+
+
+# ---- arbitrary-ratio resampler: src/resample.c:100-252 -------------------------------
+class ArbResampler:
+    """The FP64 twin of the reference resampler: the same prototype (gain P,
+    forward differences), positions base + K*step - in_off carried across
+    calls, the tap c = p + mu*d formed in float32 as the reference does, and
+    the 2Q-tap FIR accumulated in FP64."""
+
+    def __init__(self, in_rate: float, out_rate: float, atten_db: float = 60.0, nphase: int = 64):
+        if atten_db <= 0.0:
+            atten_db = 60.0
+        P = nphase or 64
+        nyq = 0.5 * min(in_rate, out_rate)
+        taps = design_lowpass(0.8 * nyq, nyq, atten_db, in_rate * P)
+        Q = (taps.size + P - 1) // P
+        p = np.zeros(P * Q, F32)
+        p[: taps.size] = taps
+        p = (p * F32(P)).astype(F32)
+        d = np.empty_like(p)
+        d[:-1] = p[1:] - p[:-1]
+        d[-1] = -p[-1]
+        self.P, self.Q, self.p, self.d = P, Q, p, d
+        self.step = in_rate / out_rate
+        self.base, self.K, self.in_off = 0.0, 0, 0
+        self.hist = np.zeros(Q, np.complex128)
+
+    def set_ratio(self, ratio: float) -> None:
+        """src/resample.c:185-192: re-anchor so the next output stays put."""
+        self.base = self.base + float(self.K) * self.step
+        self.K = 0
+        self.step = 1.0 / ratio
+
+    def _local(self, K: int) -> float:
+        return self.base + float(K) * self.step - float(self.in_off)
+
+    def process(self, x) -> np.ndarray:
+        x = np.asarray(x, np.complex64).astype(np.complex128)
+        n, P, Q = x.size, self.P, self.Q
+        ext = np.concatenate([self.hist, x])       # input m at ext[Q + m]
+        q = np.arange(Q)
+        out = []
+        K = self.K
+        local = self._local(K)
+        while local < n:
+            i = math.floor(local)
+            frac = local - i
+            ph = frac * P
+            b = min(int(ph), P - 1)
+            mu = F32(ph - b)
+            idx = b + q * P
+            c = self.p[idx] + F32(mu) * self.d[idx]             # float32, as src/resample.c:237
+            m = i - q
+            ok = Q + m >= 0
+            out.append(np.sum(c[ok].astype(np.float64) * ext[Q + m[ok]]))
+            K += 1
+            local = self._local(K)
+        self.K = K
+        self.in_off += n
+        self.hist = ext[-Q:].copy()
+        return np.array(out, np.complex64)
+
+
+# ---- overlap-save extractor: src/channelizer.c:473-788 -------------------------------------
+class Osx:
+    """The FP64 twin of the reference extractor: the block FFT, the fold with
+    the channel's float32 frequency response, the M-point inverse FFT and the
+    absolute-time mix in FP64, rounded to float32 where the reference stores
+    (the pre-resampler samples), then the resampler twin."""
+
+    def __init__(self, fs: float, nfft: int = 4096, atten_db: float = 80.0, max_nel: int = 1):
+        self.fs, self.nfft, self.atten = float(fs), int(nfft), max(float(atten_db), 20.0)
+        self.ov = self.nfft // 2
+        self.hop = self.nfft - self.ov
+        self.max_nel = max_nel
+        self.hist = np.zeros((max_nel, self.ov), np.complex64)
+        self.left = np.zeros((max_nel, 0), np.complex64)
+        self.blk = 0
+        self.chans = []
+
+    def _decim(self, bw, out_rate):
+        target = out_rate if out_rate > 1.25 * bw else 1.25 * bw
+        decim = 1
+        while True:
+            nd = decim * 2
+            if self.fs / nd < target or self.ov % nd or self.nfft % nd:
+                return decim
+            decim = nd
+
+    def _tune(self, c, off):
+        s = math.floor(off / self.fs * self.nfft + 0.5)
+        c["shift"] = s % self.nfft
+        c["delta"] = off - s * self.fs / self.nfft
+        c["offset"] = off
+
+    def add(self, off, bw, out_rate) -> int:
+        decim = self._decim(bw, out_rate)
+        chan_rate = self.fs / decim
+        fp, fstop = 0.5 * bw, min(chan_rate - 0.5 * bw, 0.5 * self.fs)
+        taps = design_lowpass(fp, fstop, self.atten, self.fs)
+        if taps.size > self.ov + 1:
+            taps = taps[: self.ov + 1]
+            tot = float(np.cumsum(taps.astype(np.float64))[-1]) or 1.0
+            taps = (taps.astype(np.float64) / tot).astype(F32)
+        hin = np.zeros(self.nfft, np.complex128)
+        hin[: taps.size] = taps
+        M = self.nfft // decim
+        c = dict(decim=decim, M=M, ovd=self.ov // decim, chan_rate=chan_rate, out_rate=out_rate,
+                 H=np.fft.fft(hin).astype(np.complex64).astype(np.complex128),
+                 rs=[ArbResampler(chan_rate, out_rate, self.atten, 64) for _ in range(self.max_nel)]
+                 if abs(chan_rate - out_rate) > 1e-6 * out_rate else None)
+        self._tune(c, off)
+        for i, old in enumerate(self.chans):
+            if old is None:
+                self.chans[i] = c
+                return i
+        self.chans.append(c)
+        return len(self.chans) - 1
+
+    def retune(self, ch, off):
+        self._tune(self.chans[ch], off)
+
+    def remove(self, ch):
+        self.chans[ch] = None
+
+    def process_n(self, ins, nel):
+        nfft, ov, hop = self.nfft, self.ov, self.hop
+        xs = [np.concatenate([self.hist[e], self.left[e], np.asarray(ins[e], np.complex64)]) for e in range(nel)]
+        avail = self.left.shape[1] + np.asarray(ins[0]).size
+        m = avail // hop
+        outs = [[] if c is not None else None for c in self.chans]
+        for i in range(m):
+            bi = self.blk + i
+            X = [np.fft.fft(xs[e][i * hop: i * hop + nfft].astype(np.complex128)) for e in range(nel)]
+            for ci, c in enumerate(self.chans):
+                if c is None:
+                    continue
+                M, decim, sh = c["M"], c["decim"], c["shift"]
+                idx = np.arange(M)[:, None] + np.arange(decim)[None, :] * M
+                start = bi * hop - ov
+                a0 = ((sh * start) % nfft) / nfft
+                nvalid = M - c["ovd"]
+                nabs = bi * hop + np.arange(nvalid) * decim
+                ph = -2.0 * math.pi * (a0 + c["delta"] * nabs.astype(np.float64) / self.fs)
+                rot = np.exp(1j * ph)
+                row = []
+                for e in range(nel):
+                    Yf = np.sum(X[e][(idx + sh) % nfft] * c["H"][idx], axis=1)
+                    ys = np.fft.ifft(Yf)
+                    v = (ys[c["ovd"]:] / decim * rot).astype(np.complex64)
+                    row.append(c["rs"][e].process(v) if c["rs"] is not None else v)
+                outs[ci].append(row)
+        res = []
+        for ci, c in enumerate(self.chans):
+            if c is None:
+                res.append(None)
+            elif outs[ci]:
+                res.append(np.array([np.concatenate([blk[e] for blk in outs[ci]]) for e in range(nel)],
+                                    np.complex64))
+            else:
+                res.append(np.zeros((nel, 0), np.complex64))
+        # next history and leftover
+        for e in range(nel):
+            tail = xs[e][m * hop:]
+            self.hist[e] = tail[:ov]
+        self.left = np.array([xs[e][m * hop + ov:] for e in range(nel)], np.complex64).reshape(nel, -1)
+        self.blk += m
+        return res, [0 if r is None else r.shape[1] for r in res]
+
+    def process(self, x):
+        outs, cnts = self.process_n([x], 1)
+        return [o[0] if o is not None else None for o in outs], cnts
+
+
 def rel_err(a, b) -> float:
     """max |a - b| / max |b| over the whole output (the reference's measure,
     tests/test_channelizer.py:88)."""

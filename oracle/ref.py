@@ -55,6 +55,21 @@ class Ref:
             "atkdsp_pfb_channel_power": (None, [cf, u, u, sz, fp, fp]),
             "atkdsp_nco_init": (None, [P(NcoState), d, d]),
             "atkdsp_nco_mix": (None, [P(NcoState), cf, cf, sz]),
+            "atkdsp_design_lowpass": (C.c_ssize_t, [d, d, d, d, fp, sz]),
+            "atkdsp_arb_resampler_create": (vp, [d, d, d, u]),
+            "atkdsp_arb_resampler_destroy": (None, [vp]),
+            "atkdsp_arb_resampler_out_max": (sz, [vp, sz]),
+            "atkdsp_arb_resampler_set_ratio": (None, [vp, d]),
+            "atkdsp_arb_resampler_process": (C.c_ssize_t, [vp, cf, sz, cf, sz]),
+            "atkdsp_osx_create": (vp, [d, sz, d, u]),
+            "atkdsp_osx_destroy": (None, [vp]),
+            "atkdsp_osx_add": (C.c_int, [vp, d, d, d]),
+            "atkdsp_osx_retune": (C.c_int, [vp, C.c_int, d]),
+            "atkdsp_osx_remove": (C.c_int, [vp, C.c_int]),
+            "atkdsp_osx_channels": (C.c_int, [vp]),
+            "atkdsp_osx_chan_rate": (d, [vp, C.c_int]),
+            "atkdsp_osx_out_max": (sz, [vp, C.c_int, sz]),
+            "atkdsp_osx_process_n": (C.c_ssize_t, [vp, P(cf), u, sz, P(cf), P(sz), sz]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -214,3 +229,84 @@ class RefPfb:
         if getattr(self, "p", None):
             self.r.L.atkdsp_pfb_destroy(self.p)
             self.p = None
+
+
+class RefArb:
+    """The reference arbitrary resampler (src/resample.c:130-252)."""
+
+    def __init__(self, ref: Ref, in_rate, out_rate, atten=60.0, nphase=64):
+        self.L = ref.L
+        self.h = self.L.atkdsp_arb_resampler_create(float(in_rate), float(out_rate), float(atten), int(nphase))
+        assert self.h
+
+    def set_ratio(self, ratio):
+        self.L.atkdsp_arb_resampler_set_ratio(self.h, float(ratio))
+
+    def process(self, x):
+        x = np.ascontiguousarray(x, np.complex64)
+        out = np.empty(self.L.atkdsp_arb_resampler_out_max(self.h, x.size), np.complex64)
+        got = self.L.atkdsp_arb_resampler_process(self.h, Ref._cf(x), x.size, Ref._cf(out), out.size)
+        assert got >= 0, got
+        return out[:got]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.atkdsp_arb_resampler_destroy(self.h)
+            self.h = None
+
+
+class RefOsx:
+    """The reference overlap-save extractor (src/channelizer.c:473-788)."""
+
+    def __init__(self, ref: Ref, fs, nfft=4096, atten=80.0, max_nel=1):
+        self.L = ref.L
+        self.h = self.L.atkdsp_osx_create(float(fs), int(nfft), float(atten), int(max_nel))
+        assert self.h
+
+    def add(self, off, bw, rate):
+        ch = self.L.atkdsp_osx_add(self.h, float(off), float(bw), float(rate))
+        assert ch >= 0, ch
+        return ch
+
+    def retune(self, ch, off):
+        assert self.L.atkdsp_osx_retune(self.h, int(ch), float(off)) == 0
+
+    def remove(self, ch):
+        assert self.L.atkdsp_osx_remove(self.h, int(ch)) == 0
+
+    def process_n(self, ins, nel):
+        arrs = [np.ascontiguousarray(np.asarray(ins[e]).ravel(), np.complex64) for e in range(nel)]
+        n = arrs[0].size
+        ns = self.L.atkdsp_osx_channels(self.h)
+        active = [self.L.atkdsp_osx_chan_rate(self.h, ci) > 0 for ci in range(ns)]
+        cap = max([self.L.atkdsp_osx_out_max(self.h, ci, n) for ci in range(ns) if active[ci]], default=0)
+        bufs = [[np.zeros(cap, np.complex64) if active[ci] else None for ci in range(ns)] for _ in range(nel)]
+        ptrs = (C.POINTER(Cf) * max(1, nel * ns))()
+        for e in range(nel):
+            for ci in range(ns):
+                ptrs[e * ns + ci] = Ref._cf(bufs[e][ci]) if active[ci] else None
+        counts = (C.c_size_t * max(1, ns))()
+        inarr = (C.POINTER(Cf) * nel)(*[Ref._cf(a) for a in arrs])
+        got = self.L.atkdsp_osx_process_n(self.h, inarr, nel, n, ptrs, counts, cap)
+        assert got >= 0, got
+        outs = [None] * ns
+        for ci in range(ns):
+            if active[ci]:
+                outs[ci] = np.array([bufs[e][ci][:counts[ci]] for e in range(nel)], np.complex64)
+        return outs, [int(counts[ci]) for ci in range(ns)]
+
+    def process(self, x):
+        outs, cnts = self.process_n([x], 1)
+        return [o[0] if o is not None else None for o in outs], cnts
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.atkdsp_osx_destroy(self.h)
+            self.h = None
+
+
+def design_lowpass(ref: Ref, fp, fs_, atten, rate):
+    n = ref.L.atkdsp_design_lowpass(float(fp), float(fs_), float(atten), float(rate), None, 0)
+    out = np.zeros(n, np.float32)
+    assert ref.L.atkdsp_design_lowpass(float(fp), float(fs_), float(atten), float(rate), Ref._f(out), n) == n
+    return out

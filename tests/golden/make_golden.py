@@ -19,7 +19,7 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT))
-from oracle.ref import Ref  # noqa: E402
+from oracle.ref import Ref, RefArb, RefOsx, design_lowpass  # noqa: E402
 from atkdsp_amd import synth  # noqa: E402
 
 OUT = Path(__file__).resolve().parent
@@ -211,10 +211,73 @@ def nco_golden():
          tone_phase=np.array(pht), pq=np.array([p, q]), fs=np.array(fs))
 
 
+OSX_FS = 2.4e6
+# channels of the extractor case: (offset, bandwidth, out_rate); the first
+# three are resampled (75 kHz channel rate onto 50 kHz), the last is not
+OSX_CHANS = [(300e3, 50e3, 50e3), (513 * OSX_FS / 4096, 50e3, 50e3), (-411737.3, 50e3, 50e3),
+             (-150e3, 50e3, 75e3)]
+OSX_CHUNKS = [7000, 1, 12000]          # then the rest
+OSX_PHI = np.deg2rad(31.0)
+
+
+def osx_input(n=4096 * 8, seed=31):
+    """tones 7 kHz above each channel centre plus noise, complex64"""
+    k = np.arange(n)
+    x = sum(np.exp(2j * np.pi * (off + 7e3) / OSX_FS * k) for off, _, _ in OSX_CHANS)
+    rng = np.random.default_rng(seed)
+    x = x + 0.01 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    return x.astype(np.complex64)
+
+
+def osx_elements(x, nel=3):
+    return [(x * np.exp(1j * e * OSX_PHI)).astype(np.complex64) for e in range(nel)]
+
+
+def osx_golden():
+    """The reference lowpass design, arbitrary resampler and overlap-save
+    extractor (src/design.c, src/resample.c, src/channelizer.c:473-788)."""
+    r = Ref()
+    designs = [(25e3, 50e3, 80.0, 2.4e6), (30e3, 37.5e3, 80.0, 4.8e6), (30e3, 40e3, 40.0, 1e6),
+               (1e3, 3e3, 95.0, 48e3)]
+    taps = [design_lowpass(r, *a) for a in designs]
+    rng = np.random.default_rng(20261205)
+    rx = noise(rng, 5000)
+    rcases = [(75e3, 50e3, 80.0, 64), (1.0, 1.37, 60.0, 64), (48e3, 44.1e3, 80.0, 32)]
+    ry = []
+    for ri, ro, at, ph in rcases:
+        a = RefArb(r, ri, ro, at, ph)
+        ry.append(np.concatenate([a.process(rx[:1000]), a.process(rx[1000:1001]), a.process(rx[1001:])]))
+    x = osx_input()
+    o = RefOsx(r, OSX_FS, 4096)
+    chans = [o.add(*c) for c in OSX_CHANS]
+    outs = [[] for _ in chans]
+    a = 0
+    for c in OSX_CHUNKS + [x.size - sum(OSX_CHUNKS)]:
+        y, _ = o.process(x[a:a + c])
+        for i, ch in enumerate(chans):
+            outs[i].append(y[ch])
+        a += c
+    ys = [np.concatenate(v) for v in outs]
+    # three phase-coherent elements, two channels, one call
+    oe = RefOsx(r, OSX_FS, 4096, 80.0, 3)
+    c0 = oe.add(*OSX_CHANS[0])
+    c1 = oe.add(*OSX_CHANS[3])
+    ye, _ = oe.process_n(osx_elements(x), 3)
+    save("osx", design_args=np.array(designs), design_len=np.array([t.size for t in taps]),
+         design_taps=np.concatenate(taps), rs_args=np.array(rcases), rs_x=rx,
+         rs_len=np.array([y.size for y in ry]), rs_y=np.concatenate(ry), osx_x=x,
+         osx_chans=np.array(OSX_CHANS), osx_chunks=np.array(OSX_CHUNKS),
+         osx_len=np.array([y.size for y in ys]), osx_y=np.concatenate(ys),
+         osx_el_y0=ye[c0], osx_el_y1=ye[c1])
+
+
 if __name__ == "__main__":
     import sys
     if sys.argv[1:] == ["nco"]:
         nco_golden()
+    elif sys.argv[1:] == ["osx"]:
+        osx_golden()
     else:
         main()
         nco_golden()
+        osx_golden()

@@ -61,7 +61,7 @@ def test_stubs_return_failure_values():
     f = lib.atkdsp_fir_create
     f.restype = C.c_void_p
     assert f() is None                                  # a *_create stub: NULL
-    g = lib.atkdsp_arb_resampler_reset
+    g = lib.atkdsp_fir_reset
     g.restype = None
     g()                                                 # void: nothing
     h = lib.atkdsp_median
@@ -70,6 +70,21 @@ def test_stubs_return_failure_values():
     k = lib.atkdsp_cfar_process
     k.restype = C.c_ssize_t
     assert k() == -1                                    # signed count: ATKDSP_E_ARG
+
+
+def test_lowpass_design_is_host_code_and_bit_exact():
+    """atkdsp_design_lowpass runs on the host (a design-time helper) and
+    reproduces the reference's taps bit for bit (src/design.c:34-62)."""
+    import numpy as np
+    import atkdsp_amd as A
+    from conftest import golden
+    g = golden("osx")
+    taps = np.split(g["design_taps"], np.cumsum(g["design_len"])[:-1])
+    for args, t in zip(g["design_args"], taps):
+        h = A.design_lowpass(*args)
+        assert np.array_equal(h.view(np.uint32), t.view(np.uint32)), args
+    with pytest.raises(A.AtkDspError):
+        A.design_lowpass(50e3, 25e3, 80.0, 2.4e6)              # stopband below the passband
 
 
 def test_window_and_window_stats_are_host_design_helpers():
@@ -95,6 +110,10 @@ def test_no_device_fails_loudly():
         A.Pfb(2.4e6, 64)
     with pytest.raises(A.AtkDspError):
         A.unpack(b"\x00\x01\x02\x03", "cu8")
+    with pytest.raises(A.AtkDspError):
+        A.Osx(2.4e6, 4096)
+    with pytest.raises(A.AtkDspError):
+        A.ArbResampler(75e3, 50e3)
 
 
 def test_reference_binding_would_load_it():

@@ -331,3 +331,157 @@ def test_nco_known_answers():
     g = golden("nco")
     o = O.Nco(-173_125.0, 2.4e6)
     assert np.max(np.abs(o.mix(g["x"]) - g["y1"])) > 0.1
+
+
+# ---- the lowpass design, the arbitrary resampler and the overlap-save extractor --------
+OSX_FS = 2.4e6
+
+
+def _osx_gold():
+    g = golden("osx")
+    lens = g["osx_len"]
+    ys = np.split(g["osx_y"], np.cumsum(lens)[:-1])
+    return g, ys
+
+
+def test_lowpass_design_is_bit_exact_with_the_reference():
+    g = golden("osx")
+    taps = np.split(g["design_taps"], np.cumsum(g["design_len"])[:-1])
+    for args, t in zip(g["design_args"], taps):
+        o = O.design_lowpass(*args)
+        assert np.array_equal(o.view(np.uint32), t.view(np.uint32)), args
+
+
+def test_lowpass_design_closed_form_properties():
+    h = O.design_lowpass(25e3, 50e3, 80.0, 2.4e6).astype(np.float64)
+    assert h.size % 2 == 1 and np.allclose(h, h[::-1])            # odd, symmetric
+    assert abs(h.sum() - 1.0) < 1e-6                              # unit DC gain
+    H = np.abs(np.fft.rfft(h, 1 << 16))
+    f = np.fft.rfftfreq(1 << 16, 1.0 / 2.4e6)
+    assert 20 * np.log10(H[f >= 50e3].max()) < -79.0               # the stopband edge holds 80 dB
+    assert np.max(np.abs(20 * np.log10(H[f <= 20e3]))) < 0.01      # flat passband
+
+
+def test_resampler_oracle_matches_the_reference():
+    g = golden("osx")
+    ys = np.split(g["rs_y"], np.cumsum(g["rs_len"])[:-1])
+    x = g["rs_x"]
+    for (ri, ro, at, ph), y in zip(g["rs_args"], ys):
+        r = O.ArbResampler(ri, ro, at, int(ph))
+        o = np.concatenate([r.process(x[:1000]), r.process(x[1000:1001]), r.process(x[1001:])])
+        assert o.size == y.size, (ri, ro)
+        assert O.rel_err(o, y) < 1e-6, (ri, ro, O.rel_err(o, y))
+
+
+def test_resampler_oracle_float32_form_is_bit_exact():
+    """The reference's float32 operation order (c = p + mu*d, acc += c*x,
+    q ascending) restated per output reproduces the reference's values."""
+    g = golden("osx")
+    ri, ro, at, ph = g["rs_args"][0]
+    ref = g["rs_y"][:150]
+    r = O.ArbResampler(ri, ro, at, int(ph))
+    F = np.float32
+    x = g["rs_x"]
+    ext = np.concatenate([np.zeros(r.Q, np.complex64), x])
+    for K in range(ref.size):
+        local = float(K) * r.step
+        i = math.floor(local)
+        b = min(int((local - i) * r.P), r.P - 1)
+        mu = F((local - i) * r.P - b)
+        ar = ai = F(0.0)
+        for q in range(r.Q):
+            if r.Q + i - q < 0:
+                continue
+            xv = ext[r.Q + i - q]
+            c = F(r.p[b + q * r.P] + F(mu * r.d[b + q * r.P]))
+            ar = F(ar + F(c * F(xv.real)))
+            ai = F(ai + F(c * F(xv.imag)))
+        assert F(ar) == ref[K].real and F(ai) == ref[K].imag, K
+
+
+def test_resampler_oracle_tone_and_set_ratio():
+    """A tone through a 3:2 down-resampler comes out at the same frequency
+    (phase advancing by 2 pi f/out per output); after set_ratio the next
+    output keeps its position and the rate changes."""
+    fi, fo, f = 75e3, 50e3, 7e3
+    n = 30000
+    x = np.exp(2j * np.pi * f / fi * np.arange(n)).astype(np.complex64)
+    r = O.ArbResampler(fi, fo, 80.0)
+    y = r.process(x)
+    k = np.arange(y.size)
+    # output K sits at input K*fi/fo; the FIR's group delay is (Q*P-1)/2/P inputs
+    d = np.angle(y[200:] * np.exp(-2j * np.pi * f / fo * k[200:]))
+    assert np.std(np.unwrap(d)) < 1e-3
+    assert abs(np.abs(y[200:]).mean() - 1.0) < 1e-3
+    r2 = O.ArbResampler(fi, fo, 80.0)
+    a = r2.process(x[:10000])
+    r2.set_ratio(fo / fi * 1.001)
+    b = r2.process(x[10000:])
+    assert abs((a.size + b.size) - (10000 * fo / fi + 20000 * fo / fi * 1.001)) < 3
+
+
+def test_osx_oracle_matches_the_reference():
+    g, ys = _osx_gold()
+    x = g["osx_x"]
+    o = O.Osx(OSX_FS, 4096)
+    chans = [o.add(*c) for c in g["osx_chans"]]
+    outs = [[] for _ in chans]
+    a = 0
+    for c in list(g["osx_chunks"]) + [x.size - int(g["osx_chunks"].sum())]:
+        y, _ = o.process(x[a:a + c])
+        for i, ch in enumerate(chans):
+            outs[i].append(y[ch])
+        a += c
+    for i, ch in enumerate(chans):
+        got = np.concatenate(outs[i])
+        assert got.size == ys[i].size, i
+        assert O.rel_err(got, ys[i]) < 2e-6, (i, O.rel_err(got, ys[i]))
+
+
+def test_osx_oracle_elements_match_the_reference():
+    g, _ = _osx_gold()
+    x = g["osx_x"]
+    phi = np.deg2rad(31.0)
+    els = [(x * np.exp(1j * e * phi)).astype(np.complex64) for e in range(3)]
+    o = O.Osx(OSX_FS, 4096, 80.0, 3)
+    c0 = o.add(*g["osx_chans"][0])
+    c1 = o.add(*g["osx_chans"][3])
+    y, _ = o.process_n(els, 3)
+    assert O.rel_err(y[c0], g["osx_el_y0"]) < 2e-6
+    assert O.rel_err(y[c1], g["osx_el_y1"]) < 2e-6
+
+
+def test_osx_oracle_tone_lands_on_centre_with_unit_gain_and_stopband():
+    """the reference's own properties (tests/test_channelizer.py:203-219, :319-330)"""
+    n = 4096 * 30
+    for off in (300e3, 513 * OSX_FS / 4096, -411737.3):
+        x = np.exp(2j * np.pi * (off + 7e3) / OSX_FS * np.arange(n)).astype(np.complex64)
+        o = O.Osx(OSX_FS, 4096)
+        c = o.add(off, 50e3, 50e3)
+        y = o.process(x)[0][c]
+        assert abs(np.abs(y[300:800]).mean() - 1.0) < 1e-3
+        k = np.arange(y.size)
+        d = np.unwrap(np.angle(y * np.exp(-2j * np.pi * 7e3 / 50e3 * k)))[300:-50]
+        assert np.std(d) < 1e-3, off
+    x = np.exp(2j * np.pi * (300e3 + 150e3) / OSX_FS * np.arange(n)).astype(np.complex64)
+    o = O.Osx(OSX_FS, 4096)
+    c = o.add(300e3, 50e3, 50e3)
+    y = o.process(x)[0][c]
+    assert 20 * np.log10(np.abs(y[300:800]).mean() + 1e-20) < -80.0
+
+
+def test_osx_oracle_catches_planted_mistakes():
+    """a roll one bin off and a residual with the wrong sign each fail the
+    comparison with the reference"""
+    g, ys = _osx_gold()
+    x = g["osx_x"]
+    for plant in ("shift", "delta"):
+        o = O.Osx(OSX_FS, 4096)
+        c = o.add(*g["osx_chans"][2])
+        if plant == "shift":
+            o.chans[c]["shift"] = (o.chans[c]["shift"] + 1) % 4096
+        else:
+            o.chans[c]["delta"] = -o.chans[c]["delta"]
+        y = o.process(x)[0][c]
+        m = min(y.size, ys[2].size)
+        assert O.rel_err(y[:m], ys[2][:m]) > 1e-2, plant
