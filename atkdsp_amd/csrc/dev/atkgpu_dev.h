@@ -197,15 +197,30 @@ typedef struct {
  *   (blk0+i)*hop - ov, nabs = (blk0+i)*hop + k*decim;
  * rs_run: outputs K0 .. K0+count-1 of a call over n inputs per element
  *   (element e at in + e*ipitch), history hist_in ([nel][Q]) -> hist_out. */
-int atkgpu_osx_fold(void *s, const void *X, const void *H, void *Yf, size_t nt, unsigned nfft, unsigned M,
-                    unsigned decim, unsigned sh);
-int atkgpu_osx_mix(void *s, const void *ys, void *valid, size_t vpitch, unsigned nel, size_t m, unsigned M,
-                   unsigned ovd, unsigned decim, unsigned hop, unsigned ov, unsigned nfft, unsigned sh,
-                   long long blk0, double delta, double fs);
 int atkgpu_rs_run(void *s, const void *in, size_t ipitch, size_t n, unsigned nel, const void *hist_in,
                   void *hist_out, const float *p, const float *d, unsigned P, unsigned Q, double base,
                   double step, unsigned long long K0, unsigned long long in_off, void *out, size_t opitch,
                   size_t count);
+/* every channel of one extractor sub-call in one launch per stage; the
+ * descriptors (one per active channel) sit in device-visible memory */
+typedef struct {
+    const void *H;             /* nfft: the channel filter's frequency response */
+    void *Y;                   /* folds, then their inverse transforms: row t = e*m + i at Y + t*M */
+    void *V;                   /* pre-resampler samples: element e, block i at V + e*vpitch + i*nvalid */
+    void *out;                 /* resampled outputs: element e at out + e*ocap */
+    const float *p, *d;        /* resampler prototype and differences (rs) */
+    const void *hist_in;       /* [nel][Q] history before the sub-call (rs) */
+    void *hist_out;            /* [nel][Q] history after it (rs) */
+    double delta, base, step, inoff;
+    unsigned long long K0;
+    int M, decim, ovd, nvalid, sh, rs, P, Q, count, pad_;
+    long long vpitch, ocap;
+} atkgpu_osx_chan;
+int atkgpu_osx_fold_all(void *s, const void *X, const void *chans, int nch, unsigned nel, size_t m, size_t xpitch,
+                        unsigned nfft);
+int atkgpu_osx_mix_all(void *s, const void *chans, int nch, unsigned nel, size_t m, int max_nvalid, unsigned hop,
+                       unsigned ov, unsigned nfft, long long blk0, double fs);
+int atkgpu_osx_rs_all(void *s, const void *chans, int nch, unsigned nel, size_t nin, int max_count, int max_q);
 int atkgpu_disp_teams(unsigned N);          /* frames one block of the frames launch transforms */
 int atkgpu_disp_frames(void *s, const void *blks, int nblk, unsigned N, const void *tw);
 int atkgpu_disp_lines(void *s, const void *blks, int nblk, unsigned N, int det);

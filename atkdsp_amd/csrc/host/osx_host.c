@@ -281,13 +281,17 @@ typedef struct {
     atkdsp_cf32 *d_H;          /* nfft: the channel filter's frequency response */
     atkgpu_fftplan *iplan;     /* M-point */
     atkdsp_cf32 *d_iscr;       /* its scratch */
+    atkdsp_cf32 *d_Y;          /* folds, inverse-transformed in place: [max_nel*mcap][M] */
+    atkdsp_cf32 *d_V;          /* pre-resampler samples: [max_nel][mcap*nvalid] */
     int rs_on;
     gpu_rs rs;
     atkdsp_cf32 *d_out;        /* resampled outputs, [max_nel][ocap] */
     size_t ocap;
 } osx_chan;
 
-#define OSX_SLAB_MIN ((size_t)1 << 18)     /* input samples per sub-call (at least) */
+/* input samples per element per sub-call: about 2M samples over all elements */
+#define OSX_SLAB_TOTAL ((size_t)1 << 21)
+#define OSX_SLAB_MIN ((size_t)1 << 16)
 
 struct atkdsp_osx {
     double fs, atten;
@@ -301,17 +305,23 @@ struct atkdsp_osx {
     atkdsp_cf32 *d_s[2];       /* rows, max_nel * spitch */
     int cur;
     atkdsp_cf32 *d_X;          /* max_nel * mcap * nfft forward transforms */
-    atkdsp_cf32 *d_Y;          /* max_nel * mcap * nfft folds, inverse-transformed in place */
-    atkdsp_cf32 *d_V;          /* max_nel * vpitch valid samples, vpitch = mcap * hop */
     atkdsp_cf32 *d_scr;        /* forward plan scratch */
     atkdsp_cf32 *lo;           /* host mirror of the leftover, max_nel * hop */
     osx_chan *chans;
     int nslots, cap_slots;
+    /* the channel descriptors of a sub-call, device-visible, two sets used in
+     * turn; a set is rewritten only after the launches that read it are done */
+    atkgpu_osx_chan *desc_h[2];
+    void *desc_d[2];
+    void *desc_ev[2];
+    int desc_used[2], desc_cap, dcur;
 };
 
 static void osx_free_chan(osx_chan *c) {
     if (!c->active) return;
     atkgpu_dfree(c->d_H);
+    atkgpu_dfree(c->d_Y);
+    atkgpu_dfree(c->d_V);
     atkgpu_fftplan_destroy(c->iplan);
     atkgpu_dfree(c->d_iscr);
     if (c->rs_on) rs_free(&c->rs);
@@ -328,10 +338,12 @@ void atkdsp_osx_destroy(atkdsp_osx *x) {
     atkgpu_dfree(x->d_s[0]);
     atkgpu_dfree(x->d_s[1]);
     atkgpu_dfree(x->d_X);
-    atkgpu_dfree(x->d_Y);
-    atkgpu_dfree(x->d_V);
     atkgpu_dfree(x->d_scr);
     free(x->lo);
+    for (int i = 0; i < 2; ++i) {
+        atkgpu_hfree(x->desc_h[i]);
+        atkgpu_event_destroy(x->desc_ev[i]);
+    }
     if (x->stream) atkgpu_stream_destroy(x->stream);
     free(x);
 }
@@ -348,8 +360,10 @@ atkdsp_osx *atkdsp_osx_create(double fs, size_t nfft, double atten_db, unsigned 
     x->max_nel = max_nel;
     x->ov = nfft / 2;
     x->hop = nfft - x->ov;
-    /* a sub-call takes a whole number of hops, at least OSX_SLAB_MIN samples */
-    x->slab = ((OSX_SLAB_MIN + x->hop - 1) / x->hop) * x->hop;
+    /* a sub-call takes a whole number of hops */
+    size_t slab = OSX_SLAB_TOTAL / max_nel;
+    if (slab < OSX_SLAB_MIN) slab = OSX_SLAB_MIN;
+    x->slab = ((slab + x->hop - 1) / x->hop) * x->hop;
     x->mcap = x->slab / x->hop + 1;
     x->spitch = x->ov + x->hop + x->slab;
     const size_t nx = (size_t)max_nel * x->mcap * nfft;
@@ -358,13 +372,16 @@ atkdsp_osx *atkdsp_osx_create(double fs, size_t nfft, double atten_db, unsigned 
         x->d_s[0] = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)max_nel * x->spitch * sizeof(atkdsp_cf32));
         x->d_s[1] = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)max_nel * x->spitch * sizeof(atkdsp_cf32));
         x->d_X = (atkdsp_cf32 *)atkgpu_dmalloc(nx * sizeof(atkdsp_cf32));
-        x->d_Y = (atkdsp_cf32 *)atkgpu_dmalloc(nx * sizeof(atkdsp_cf32));
-        x->d_V = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)max_nel * x->mcap * x->hop * sizeof(atkdsp_cf32));
         const size_t scr = atkgpu_fftplan_scratch(x->plan, x->mcap);
         x->d_scr = scr ? (atkdsp_cf32 *)atkgpu_dmalloc(scr * sizeof(atkdsp_cf32)) : NULL;
         x->lo = (atkdsp_cf32 *)calloc((size_t)max_nel * x->hop, sizeof(atkdsp_cf32));
-        ok = x->d_s[0] && x->d_s[1] && x->d_X && x->d_Y && x->d_V && (!scr || x->d_scr) && x->lo;
+        ok = x->d_s[0] && x->d_s[1] && x->d_X && (!scr || x->d_scr) && x->lo;
     }
+    x->desc_cap = 64;
+    for (int i = 0; ok && i < 2; ++i)
+        ok = !atkgpu_event_create_order(&x->desc_ev[i]) &&
+             (x->desc_h[i] = (atkgpu_osx_chan *)atkgpu_hmalloc_mapped(x->desc_cap * sizeof(atkgpu_osx_chan),
+                                                                      &x->desc_d[i])) != NULL;
     /* the history starts at zero (the reference callocs it) */
     if (ok) ok = !atkgpu_fill_u32(x->stream, x->d_s[0], 0, (size_t)max_nel * x->spitch * 2) && !atkgpu_sync(x->stream);
     if (!ok) { atkdsp_osx_destroy(x); return NULL; }
@@ -502,6 +519,25 @@ int atkdsp_osx_add(atkdsp_osx *x, double offset_hz, double bw_hz, double out_rat
     if (!rc) {
         const size_t scr = atkgpu_fftplan_scratch(c->iplan, (size_t)x->max_nel * x->mcap);
         if (scr && !(c->d_iscr = (atkdsp_cf32 *)atkgpu_dmalloc(scr * sizeof(atkdsp_cf32)))) rc = ATKDSP_E_NOMEM;
+        c->d_Y = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)x->max_nel * x->mcap * c->M * sizeof(atkdsp_cf32));
+        c->d_V = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)x->max_nel * x->mcap * c->nvalid * sizeof(atkdsp_cf32));
+        if (!c->d_Y || !c->d_V) rc = ATKDSP_E_NOMEM;
+    }
+    /* room for every slot's descriptor (grown here, a configuration call) */
+    if (!rc && x->nslots > x->desc_cap) {
+        int cap = x->desc_cap;
+        while (cap < x->nslots) cap *= 2;
+        if (atkgpu_sync(x->stream)) rc = ATKDSP_E_FFT;
+        for (int i = 0; i < 2 && !rc; ++i) {
+            void *dd = NULL;
+            atkgpu_osx_chan *h = (atkgpu_osx_chan *)atkgpu_hmalloc_mapped(cap * sizeof(atkgpu_osx_chan), &dd);
+            if (!h) { rc = ATKDSP_E_NOMEM; break; }
+            atkgpu_hfree(x->desc_h[i]);
+            x->desc_h[i] = h;
+            x->desc_d[i] = dd;
+            x->desc_used[i] = 0;
+        }
+        if (!rc) x->desc_cap = cap;
     }
     if (!rc) rc = atkgpu_h2d(x->stream, c->d_H, H, x->nfft * sizeof(atkdsp_cf32)) ? ATKDSP_E_FFT : 0;
     if (!rc && fabs(chan_rate - out_rate) > 1e-6 * out_rate) {
@@ -510,6 +546,7 @@ int atkdsp_osx_add(atkdsp_osx *x, double offset_hz, double bw_hz, double out_rat
         if (rc) c->rs_on = 0;
         if (!rc) {
             c->ocap = (size_t)((double)(x->mcap * c->nvalid) * out_rate / chan_rate) + 64;
+            c->ocap = (c->ocap + 63) & ~(size_t)63;
             c->d_out = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)x->max_nel * c->ocap * sizeof(atkdsp_cf32));
             if (!c->d_out) rc = ATKDSP_E_NOMEM;
         }
@@ -549,7 +586,8 @@ size_t atkdsp_osx_out_max(const atkdsp_osx *x, int ch, size_t n) {
     return (size_t)((double)n * x->chans[ch].out_rate / x->fs) + 64;
 }
 
-/* one sub-call over ins[e][pos, pos+c) */
+/* one sub-call over ins[e][pos, pos+c): the forward FFT of every block, then
+ * every channel's fold, mix and resampler in one launch each */
 static int osx_sub(atkdsp_osx *x, const atkdsp_cf32 *const *ins, unsigned nel, size_t pos, size_t c,
                    atkdsp_cf32 *const *outs, size_t *out_counts) {
     const size_t hop = x->hop, ov = x->ov, nfft = x->nfft;
@@ -572,37 +610,81 @@ static int osx_sub(atkdsp_osx *x, const atkdsp_cf32 *const *ins, unsigned nel, s
         if (!rc) rc = atkgpu_fft_batch(x->stream, x->plan, se, hop, NULL, ms, 0, x->d_X + (size_t)e * x->mcap * nfft,
                                        x->d_scr);
     }
-    const size_t vpitch = x->mcap * hop;
+    /* this sub-call's channel descriptors, in the set not in flight */
+    const int b = x->dcur;
+    x->dcur ^= 1;
+    if (!rc && x->desc_used[b] && atkgpu_event_sync(x->desc_ev[b])) rc = ATKDSP_E_FFT;
+    atkgpu_osx_chan *dh = x->desc_h[b];
+    int nd = 0, max_nvalid = 0, max_count = 0, max_q = 0;
+    size_t got[x->nslots > 0 ? x->nslots : 1];
+    int slot_of[x->nslots > 0 ? x->nslots : 1];
     for (int ci = 0; ci < ns && !rc; ++ci) {
         osx_chan *ch = &x->chans[ci];
         if (!ch->active) continue;
-        /* the element's blocks sit mcap*nfft apart in X; fold them into
-         * consecutive M-point rows of Y */
-        for (unsigned e = 0; e < nel && !rc; ++e)
-            rc = atkgpu_osx_fold(x->stream, x->d_X + (size_t)e * x->mcap * nfft, ch->d_H,
-                                 x->d_Y + (size_t)e * ms * ch->M, ms, (unsigned)nfft, (unsigned)ch->M, ch->decim,
-                                 ch->shift);
-        if (!rc)
-            rc = atkgpu_fft_batch(x->stream, ch->iplan, x->d_Y, ch->M, NULL, (size_t)nel * ms, 1, x->d_Y, ch->d_iscr);
-        if (!rc)
-            rc = atkgpu_osx_mix(x->stream, x->d_Y, x->d_V, vpitch, nel, ms, (unsigned)ch->M, (unsigned)ch->ovd,
-                                ch->decim, (unsigned)hop, (unsigned)ov, (unsigned)nfft, ch->shift, (long long)x->blk,
-                                ch->delta, x->fs);
+        atkgpu_osx_chan *d = &dh[nd];
+        memset(d, 0, sizeof *d);
+        d->H = ch->d_H;
+        d->Y = ch->d_Y;
+        d->V = ch->d_V;
+        d->delta = ch->delta;
+        d->M = (int)ch->M;
+        d->decim = (int)ch->decim;
+        d->ovd = (int)ch->ovd;
+        d->nvalid = (int)ch->nvalid;
+        d->sh = (int)ch->shift;
+        d->vpitch = (long long)(x->mcap * ch->nvalid);
         const size_t nv = ms * ch->nvalid;
-        size_t got = nv;
-        const atkdsp_cf32 *src = x->d_V;
-        size_t spitch = vpitch;
-        if (!rc && ch->rs_on) {
-            got = rs_count(&ch->rs, nv);
-            if (got > ch->ocap) rc = ATKDSP_E_FFT;
-            if (!rc) rc = rs_run(&ch->rs, x->stream, x->d_V, vpitch, nv, nel, ch->d_out, ch->ocap, got);
-            src = ch->d_out;
-            spitch = ch->ocap;
+        got[nd] = nv;
+        if (ch->rs_on) {
+            const size_t k = rs_count(&ch->rs, nv);
+            if (k > ch->ocap) { rc = ATKDSP_E_FFT; break; }
+            d->rs = 1;
+            d->out = ch->d_out;
+            d->ocap = (long long)ch->ocap;
+            d->p = ch->rs.d_p;
+            d->d = ch->rs.d_d;
+            d->hist_in = ch->rs.d_hist[ch->rs.hcur];
+            d->hist_out = ch->rs.d_hist[ch->rs.hcur ^ 1];
+            d->base = ch->rs.base;
+            d->step = ch->rs.step;
+            d->inoff = (double)ch->rs.in_off;
+            d->K0 = (unsigned long long)ch->rs.K;
+            d->P = (int)ch->rs.P;
+            d->Q = (int)ch->rs.Q;
+            d->count = (int)k;
+            got[nd] = k;
+            if ((int)k > max_count) max_count = (int)k;
+            if ((int)ch->rs.Q > max_q) max_q = (int)ch->rs.Q;
+            /* the position and the history advance with this sub-call */
+            ch->rs.hcur ^= 1;
+            ch->rs.K += k;
+            ch->rs.in_off += nv;
         }
-        for (unsigned e = 0; e < nel && !rc && got; ++e)
+        if (d->nvalid > max_nvalid) max_nvalid = d->nvalid;
+        slot_of[nd] = ci;
+        ++nd;
+    }
+    const void *dd = x->desc_d[b];
+    if (!rc && nd) rc = atkgpu_osx_fold_all(x->stream, x->d_X, dd, nd, nel, ms, x->mcap * nfft, (unsigned)nfft);
+    for (int i = 0; i < nd && !rc; ++i) {
+        osx_chan *ch = &x->chans[slot_of[i]];
+        rc = atkgpu_fft_batch(x->stream, ch->iplan, ch->d_Y, ch->M, NULL, (size_t)nel * ms, 1, ch->d_Y, ch->d_iscr);
+    }
+    if (!rc && nd)
+        rc = atkgpu_osx_mix_all(x->stream, dd, nd, nel, ms, max_nvalid, (unsigned)hop, (unsigned)ov, (unsigned)nfft,
+                                (long long)x->blk, x->fs);
+    if (!rc && nd && max_q) rc = atkgpu_osx_rs_all(x->stream, dd, nd, nel, ms, max_count, max_q);
+    if (!rc) rc = atkgpu_event_record(x->desc_ev[b], x->stream);
+    x->desc_used[b] = 1;
+    for (int i = 0; i < nd && !rc; ++i) {
+        const int ci = slot_of[i];
+        const osx_chan *ch = &x->chans[ci];
+        const atkdsp_cf32 *src = ch->rs_on ? ch->d_out : ch->d_V;
+        const size_t spitch = ch->rs_on ? ch->ocap : x->mcap * ch->nvalid;
+        for (unsigned e = 0; e < nel && !rc && got[i]; ++e)
             rc = atkgpu_d2h(x->stream, outs[(size_t)e * ns + ci] + out_counts[ci], src + (size_t)e * spitch,
-                            got * sizeof(atkdsp_cf32));
-        out_counts[ci] += got;
+                            got[i] * sizeof(atkdsp_cf32));
+        out_counts[ci] += got[i];
     }
     /* next history: s[ms hop, ms hop + ov) into the other row */
     atkdsp_cf32 *T = x->d_s[x->cur ^ 1];

@@ -129,6 +129,53 @@ def coherent_array(n: int = 1_000_000, nel: int = 8, steps: int = 10):
             "realtime_x": round((n / 20e6) / (wall * 1e-3), 1)}
 
 
+def osx_extract(n: int = 2_000_000, nchan: int = 32, calls: int = 10, cpu_seconds: float = 3.0):
+    """The overlap-save extractor through the drop-in ABI: 40 MSPS cf32 in
+    50 ms calls (host buffers in and out, as the reference's callers use it),
+    8192-point blocks, nchan channels of 100 kHz landed on 125 kHz (through the
+    arbitrary resampler from the 156.25 kHz channel rate), spread over
+    +-15 MHz. Beside it, the reference C on the same calls (one host thread:
+    the reference's extractor is serial)."""
+    import atkdsp_amd as A
+    fs = 40e6
+    rng = np.random.default_rng(7)
+    offs = np.linspace(-15e6, 15e6, nchan)
+    k = np.arange(n)
+    x = (0.01 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))).astype(np.complex64)
+    for f in offs[::4]:
+        x += (0.1 * np.exp(2j * np.pi * (f + 20e3) / fs * k)).astype(np.complex64)
+    o = A.Osx(fs, 8192)
+    for f in offs:
+        o.add(float(f), 100e3, 125e3)
+    o.process(x)                                   # warm-up
+    t0 = time.perf_counter()
+    outs = 0
+    for _ in range(calls):
+        y, cnt = o.process(x)
+        outs += sum(cnt)
+    wall = (time.perf_counter() - t0) / calls
+    res = {"workload": f"40 MSPS cf32, 50 ms calls, 8192-pt blocks, {nchan} channels of 100 kHz onto 125 kHz "
+                       "(resampled from 156.25 kHz); host buffers in and out",
+           "ms_per_call": round(wall * 1e3, 3), "Msamples_per_s": round(n / wall / 1e6, 1),
+           "realtime_x": round((n / fs) / wall, 1), "outputs_per_call": int(outs // calls)}
+    try:
+        from oracle.ref import Ref, RefOsx, available
+        if available():
+            r = RefOsx(Ref(), fs, 8192)
+            for f in offs:
+                r.add(float(f), 100e3, 125e3)
+            t0, c = time.perf_counter(), 0
+            while time.perf_counter() - t0 < cpu_seconds:
+                r.process(x)
+                c += 1
+            cw = (time.perf_counter() - t0) / c
+            res["cpu_reference"] = {"ms_per_call": round(cw * 1e3, 2), "Msamples_per_s": round(n / cw / 1e6, 1),
+                                    "threads": 1, "calls": c}
+    except Exception as e:                          # noqa: BLE001
+        res["cpu_reference"] = f"unavailable: {e}"
+    return res
+
+
 def nco_phase_run(seconds: float = 10.0, fs: float = 40e6, p: int = 1234, q: int = 10007, chunk: int = 65536):
     """10 s of 40 MSPS through the device-resident NCO in 65,536-sample runs,
     f = fs * p / q: the FP64 phase after every run against the exact phase
@@ -320,6 +367,7 @@ def main() -> int:
         extras["precision_form"] = precision_forms()
         extras["nco_phase_run"] = nco_phase_run()
         extras["coherent_array"] = coherent_array()
+        extras["osx_extract"] = osx_extract()
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, streams[0])
     uni = serve.get("uniform", {})
     out = {
