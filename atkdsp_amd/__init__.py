@@ -40,6 +40,20 @@ class _Cf32(C.Structure):
     _fields_ = [("re", C.c_float), ("im", C.c_float)]
 
 
+class _RingMeta(C.Structure):
+    _fields_ = [("seq", C.c_uint64), ("first_sample", C.c_uint64), ("t_utc_ns", C.c_int64),
+                ("fc_hz", C.c_double), ("fs_hz", C.c_double), ("gain_db", C.c_float), ("format", C.c_int32),
+                ("flags", C.c_uint32), ("nbytes", C.c_uint32)]
+
+
+class _RingCounters(C.Structure):
+    _fields_ = [("committed", C.c_uint64), ("oldest", C.c_uint64), ("block_bytes", C.c_uint64),
+                ("nblocks", C.c_uint64), ("total_bytes", C.c_uint64), ("torn", C.c_uint64),
+                ("max_consumers", C.c_uint32), ("active", C.c_uint32), ("active_mask", C.c_uint32),
+                ("reserved", C.c_uint32), ("lost", C.c_uint64 * 16), ("read", C.c_uint64 * 16),
+                ("lag", C.c_uint64 * 16)]
+
+
 class _Nco(C.Structure):
     _fields_ = [("phase", C.c_double), ("step", C.c_double)]
 
@@ -120,6 +134,19 @@ def _bind(lib) -> None:
         "atkdsp_osx_process": (C.c_ssize_t, [vp, cf, sz, P(cf), P(sz), sz]),
         "atkdsp_osx_process_n": (C.c_ssize_t, [vp, P(cf), u, sz, P(cf), P(sz), sz]),
         "atkdsp_osx_describe": (C.c_int, [vp, C.c_char_p, sz]),
+        "atkdsp_ring_bytes": (sz, [sz, sz, C.c_int]),
+        "atkdsp_ring_init": (vp, [vp, sz, sz, sz, C.c_int]),
+        "atkdsp_ring_attach": (vp, [vp, sz]),
+        "atkdsp_ring_claim": (vp, [vp]),
+        "atkdsp_ring_commit": (C.c_int, [vp, P(_RingMeta)]),
+        "atkdsp_ring_open": (C.c_int, [vp, C.c_int]),
+        "atkdsp_ring_close": (C.c_int, [vp, C.c_int]),
+        "atkdsp_ring_read": (C.c_ssize_t, [vp, C.c_int, vp, sz, P(_RingMeta), P(C.c_uint64)]),
+        "atkdsp_ring_peek": (vp, [vp, C.c_int, P(_RingMeta), P(C.c_uint64)]),
+        "atkdsp_ring_release": (C.c_int, [vp, C.c_int]),
+        "atkdsp_ring_copy_range": (C.c_ssize_t, [vp, C.c_uint64, C.c_uint64, vp, sz, P(_RingMeta), sz]),
+        "atkdsp_ring_stats": (C.c_int, [vp, P(_RingCounters)]),
+        "atkdsp_ring_describe": (C.c_int, [vp, C.c_char_p, sz]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -669,11 +696,129 @@ class Osx:
         return [o[0] if o is not None else None for o in outs], cnts
 
 
+def _meta_dict(m) -> dict:
+    return {k: (float(getattr(m, k)) if isinstance(getattr(m, k), float) else int(getattr(m, k)))
+            for k, _ in _RingMeta._fields_}
+
+
+def ring_bytes(block_bytes: int, nblocks: int, max_consumers: int = 4) -> int:
+    """Bytes of caller memory a ring of this shape needs; 0 if impossible."""
+    return int(load().atkdsp_ring_bytes(int(block_bytes), int(nblocks), int(max_consumers)))
+
+
+class Ring:
+    """The block ring (atkdsp_ring, reference include/atkdsp.h:2020-2107) in
+    caller memory: one producer that never waits, up to 16 consumers, a
+    seqlock per block. `mem` (a writable uint8 array or any object with a
+    buffer) attaches to an existing ring, made by this library or another;
+    otherwise a new ring is formatted in a fresh 64-byte-aligned buffer."""
+
+    def __init__(self, block_bytes: int = 0, nblocks: int = 0, max_consumers: int = 4, mem=None):
+        self._lib = load()
+        if mem is not None:
+            self._buf = np.frombuffer(mem, np.uint8)
+            self._r = self._lib.atkdsp_ring_attach(self._buf.ctypes.data, self._buf.size)
+            if not self._r:
+                raise AtkDspError("ring_attach: not a ring of this version")
+        else:
+            need = ring_bytes(block_bytes, nblocks, max_consumers)
+            if not need:
+                raise AtkDspError("ring: block_bytes >= 1, nblocks >= 2, max_consumers 1..16")
+            raw = np.zeros(need + 64, np.uint8)
+            off = (-raw.ctypes.data) % 64
+            self._buf = raw[off:off + need]
+            self._r = self._lib.atkdsp_ring_init(self._buf.ctypes.data, need, int(block_bytes), int(nblocks),
+                                                 int(max_consumers))
+            if not self._r:
+                raise AtkDspError("ring_init failed")
+        st = self.stats()
+        self.block_bytes, self.nblocks = st["block_bytes"], st["nblocks"]
+        self._rbuf = np.empty(self.block_bytes, np.uint8)
+
+    @property
+    def handle(self) -> int:
+        """the atkdsp_ring pointer (for atkgpu_server_submit_ring)"""
+        return int(self._r)
+
+    @property
+    def memory(self) -> np.ndarray:
+        return self._buf
+
+    def claim(self) -> np.ndarray:
+        p = self._lib.atkdsp_ring_claim(self._r)
+        return np.ctypeslib.as_array((C.c_uint8 * self.block_bytes).from_address(p))
+
+    def commit(self, nbytes: int, **meta) -> None:
+        m = _RingMeta(**{k: v for k, v in meta.items() if k != "seq"})
+        m.nbytes = int(nbytes)
+        _check(self._lib.atkdsp_ring_commit(self._r, C.byref(m)), "ring_commit")
+
+    def write(self, data, **meta) -> None:
+        b = np.frombuffer(data, np.uint8) if not isinstance(data, np.ndarray) else data.view(np.uint8).ravel()
+        self.claim()[: b.size] = b
+        self.commit(b.size, **meta)
+
+    def open(self, from_oldest: bool = False) -> int:
+        return _check(int(self._lib.atkdsp_ring_open(self._r, int(bool(from_oldest)))), "ring_open")
+
+    def close_consumer(self, cid: int) -> None:
+        _check(self._lib.atkdsp_ring_close(self._r, int(cid)), "ring_close")
+
+    def read(self, cid: int):
+        """(bytes, meta, lost), or (None, None, lost) when there is nothing new"""
+        m, lost = _RingMeta(), C.c_uint64(0)
+        n = _check(int(self._lib.atkdsp_ring_read(self._r, int(cid), self._rbuf.ctypes.data, self._rbuf.size,
+                                                  C.byref(m), C.byref(lost))), "ring_read")
+        if n == 0:
+            return None, None, int(lost.value)
+        return self._rbuf[:n].copy(), _meta_dict(m), int(lost.value)
+
+    def peek(self, cid: int):
+        """(view into the ring, meta, lost), or (None, None, lost)"""
+        m, lost = _RingMeta(), C.c_uint64(0)
+        p = self._lib.atkdsp_ring_peek(self._r, int(cid), C.byref(m), C.byref(lost))
+        if not p:
+            return None, None, int(lost.value)
+        return (np.ctypeslib.as_array((C.c_uint8 * m.nbytes).from_address(p)), _meta_dict(m), int(lost.value))
+
+    def release(self, cid: int) -> bool:
+        """True if the peeked block survived the use; False if it was overwritten"""
+        rc = int(self._lib.atkdsp_ring_release(self._r, int(cid)))
+        if rc == -6:
+            return False
+        _check(rc, "ring_release")
+        return True
+
+    def copy_range(self, seq0: int, seq1: int, cap: int | None = None, mcap: int | None = None):
+        """whole blocks [seq0, seq1) still held, packed: (bytes, [meta])"""
+        cap = self.block_bytes * max(0, seq1 - seq0) if cap is None else cap
+        mcap = max(0, seq1 - seq0) if mcap is None else mcap
+        out = np.empty(max(cap, 1), np.uint8)
+        metas = (_RingMeta * max(mcap, 1))()
+        n = _check(int(self._lib.atkdsp_ring_copy_range(self._r, int(seq0), int(seq1), out.ctypes.data, cap,
+                                                        metas, mcap)), "ring_copy_range")
+        ms = [_meta_dict(metas[i]) for i in range(n)]
+        return out[: sum(m["nbytes"] for m in ms)], ms
+
+    def stats(self) -> dict:
+        st = _RingCounters()
+        _check(self._lib.atkdsp_ring_stats(self._r, C.byref(st)), "ring_stats")
+        d = {k: getattr(st, k) for k, _ in _RingCounters._fields_ if k not in ("lost", "read", "lag")}
+        for k in ("lost", "read", "lag"):
+            d[k] = list(getattr(st, k))
+        return d
+
+    def describe(self) -> str:
+        buf = C.create_string_buffer(256)
+        self._lib.atkdsp_ring_describe(self._r, buf, 256)
+        return buf.value.decode()
+
+
 __all__ = [
     "ABI_VERSION", "AtkDspError", "load", "available", "lib_path", "version", "build_info",
     "set_threads", "get_threads", "FMT", "DET", "WIN", "bytes_per_sample",
     "unpack", "unpack_dc", "Fft", "window", "power_db", "spectrum_reduce", "spectrum_stats",
     "window_stats", "db_to_pixels", "decimate_max", "stitch_max",
     "Pfb", "pfb_cov_accumulate", "pfb_cov_reset", "pfb_channel_power",
-    "design_lowpass", "ArbResampler", "Osx",
+    "design_lowpass", "ArbResampler", "Osx", "Ring", "ring_bytes",
 ]

@@ -55,6 +55,9 @@ IMPLEMENTED = {
     "atkdsp_osx_create", "atkdsp_osx_destroy", "atkdsp_osx_reset", "atkdsp_osx_add", "atkdsp_osx_retune",
     "atkdsp_osx_remove", "atkdsp_osx_channels", "atkdsp_osx_chan_rate", "atkdsp_osx_out_max",
     "atkdsp_osx_process", "atkdsp_osx_process_n", "atkdsp_osx_describe",
+    "atkdsp_ring_bytes", "atkdsp_ring_init", "atkdsp_ring_attach", "atkdsp_ring_claim", "atkdsp_ring_commit",
+    "atkdsp_ring_open", "atkdsp_ring_close", "atkdsp_ring_read", "atkdsp_ring_peek", "atkdsp_ring_release",
+    "atkdsp_ring_copy_range", "atkdsp_ring_stats", "atkdsp_ring_describe",
 }
 
 _STUB_BODY = {

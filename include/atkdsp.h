@@ -18,7 +18,7 @@
  *   - NaN means "unmeasured" (no fast-math anywhere).
  *
  * The library also exports every other name of the reference ABI (334 in
- * all; 58 are implemented here) as a documented stub: *_create returns NULL, signed counts return
+ * all; 71 are implemented here) as a documented stub: *_create returns NULL, signed counts return
  * ATKDSP_E_ARG, unsigned counts 0, floating results NaN, void functions do
  * nothing. Those are outside the wideband front end.
  */
@@ -43,6 +43,7 @@ extern "C" {
 #define ATKDSP_E_NOMEM -2   /* also: device memory exhausted at create */
 #define ATKDSP_E_CAP   -3
 #define ATKDSP_E_FFT   -4   /* also: a HIP launch or copy failed       */
+#define ATKDSP_E_OVERRUN -6 /* a ring reader was lapped: the data is gone */
 
 /* complex float32, numpy complex64 layout: reference include/atkdsp.h:212 */
 typedef struct { float re, im; } atkdsp_cf32;
@@ -125,6 +126,54 @@ typedef struct { double phase; double step; } atkdsp_nco;
 ATKDSP_API void atkdsp_nco_init(atkdsp_nco *nco, double freq_hz, double sample_rate);
 ATKDSP_API void atkdsp_nco_set_freq(atkdsp_nco *nco, double freq_hz, double sample_rate);
 ATKDSP_API void atkdsp_nco_mix(atkdsp_nco *nco, const atkdsp_cf32 *in, atkdsp_cf32 *out, size_t n);
+
+/* ---- the block ring: reference include/atkdsp.h:2020-2107, src/ring.c.
+ * A block ring in caller memory (a private buffer, or a shared mapping another
+ * process attaches to): one producer that never waits, up to 16 consumers
+ * with their own cursors, fixed-size blocks each with a header, a seqlock per
+ * block (lock word 2 seq + 1 while written, 2 seq + 2 once committed). The
+ * memory layout is the reference's, offset for offset, so a ring made by
+ * either library is attached by the other. Host code: it allocates nothing
+ * and starts no thread. atkgpu_server_submit_ring (include/atkgpu.h) feeds
+ * a ring's blocks into the serving loop. */
+#define ATKDSP_RING_MAX_CONSUMERS 16
+#define ATKDSP_RING_RETUNE   1u
+#define ATKDSP_RING_OVERFLOW 2u
+#define ATKDSP_RING_CAL      4u
+typedef struct {
+    uint64_t seq, first_sample;
+    int64_t  t_utc_ns;
+    double   fc_hz, fs_hz;
+    float    gain_db;
+    int32_t  format;
+    uint32_t flags, nbytes;
+} atkdsp_ring_meta;                /* 56 bytes */
+typedef struct {
+    uint64_t committed, oldest;
+    uint64_t block_bytes, nblocks, total_bytes;
+    uint64_t torn;
+    uint32_t max_consumers, active, active_mask, reserved;
+    uint64_t lost[ATKDSP_RING_MAX_CONSUMERS];
+    uint64_t read[ATKDSP_RING_MAX_CONSUMERS];
+    uint64_t lag[ATKDSP_RING_MAX_CONSUMERS];
+} atkdsp_ring_counters;            /* 448 bytes */
+typedef struct atkdsp_ring atkdsp_ring;
+ATKDSP_API size_t       atkdsp_ring_bytes(size_t block_bytes, size_t nblocks, int max_consumers);
+ATKDSP_API atkdsp_ring *atkdsp_ring_init(void *mem, size_t mem_bytes, size_t block_bytes, size_t nblocks,
+                                         int max_consumers);
+ATKDSP_API atkdsp_ring *atkdsp_ring_attach(void *mem, size_t mem_bytes);
+ATKDSP_API void        *atkdsp_ring_claim(atkdsp_ring *r);
+ATKDSP_API int          atkdsp_ring_commit(atkdsp_ring *r, const atkdsp_ring_meta *m);
+ATKDSP_API int          atkdsp_ring_open(atkdsp_ring *r, int from_oldest);
+ATKDSP_API int          atkdsp_ring_close(atkdsp_ring *r, int id);
+ATKDSP_API ptrdiff_t    atkdsp_ring_read(atkdsp_ring *r, int id, void *out, size_t cap, atkdsp_ring_meta *m,
+                                         uint64_t *lost);
+ATKDSP_API const void  *atkdsp_ring_peek(atkdsp_ring *r, int id, atkdsp_ring_meta *m, uint64_t *lost);
+ATKDSP_API int          atkdsp_ring_release(atkdsp_ring *r, int id);
+ATKDSP_API ptrdiff_t    atkdsp_ring_copy_range(const atkdsp_ring *r, uint64_t seq0, uint64_t seq1, void *out,
+                                               size_t cap, atkdsp_ring_meta *metas, size_t mcap);
+ATKDSP_API int          atkdsp_ring_stats(const atkdsp_ring *r, atkdsp_ring_counters *st);
+ATKDSP_API int          atkdsp_ring_describe(const atkdsp_ring *r, char *buf, size_t cap);
 
 /* ---- arbitrary-ratio resampler: reference include/atkdsp.h:368-377,
  * src/resample.c:100-252. A 64-phase (nphase) bank of the Kaiser lowpass below
