@@ -21,9 +21,28 @@
  * decimated to 1024 columns; a chunk's latency includes the lines it
  * completes.
  *
+ * With --ring 1 the chunks go through a block ring per stream (atkdsp_ring_*,
+ * --ring-blocks B blocks of the largest chunk, default 64, about 50 ms of a
+ * stream's arrivals, in ordinary host memory), as from a
+ * receiver process: the arrival threads become producers that claim, copy and
+ * commit each chunk as a block (t_utc_ns = the commit time on the monotonic
+ * clock), and --consumers C threads (default 4) poll their streams' rings with
+ * atkgpu_server_submit_ring, sleeping RING_POLL_US when all of them are
+ * empty. Producers, consumers and the server's two threads together stay
+ * within the 16 cores the serving host gives the process: with more runnable
+ * threads than cores, the scheduler parks one for a whole time slice (about
+ * 10 ms measured), and a parked consumer or dispatcher is a latency spike. Latency is then timed from the submit_ring call; the JSON adds the
+ * blocks lost (a producer that lapped its consumer), the delay from commit to
+ * the call that took the block, and the time a submit_ring call took.
+ *
+ * Before the clock starts every stream gets one chunk of CHUNK_MAX samples and
+ * the server is drained and its statistics reset: the figures are the steady
+ * state, without the first launch of each kernel (code object loading) or the
+ * first touch of ring pages.
+ *
  * usage: serve_load [--seconds S] [--streams N] [--speed X] [--budget-gib G]
  *                   [--max-batch B] [--seed K] [--mix c4|uniform] [--arrays A]
- *                   [--display 0|1]
+ *                   [--display 0|1] [--ring 0|1] [--consumers C] [--ring-blocks B]
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -31,12 +50,14 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/resource.h>
 #include <time.h>
 #include "atkgpu.h"
 
 #define NTHREADS 8
 #define CHUNK_MAX 65536
 #define SRC_SAMPLES (1u << 18)
+#define RING_POLL_US 20
 
 typedef struct {
     int sid, fmt, bps;
@@ -48,12 +69,20 @@ typedef struct {
     unsigned long long rng;
     double next_t;
     long long submitted;
+    void *ring_mem;            /* --ring: the stream's block ring and its consumer */
+    atkdsp_ring *ring;
+    int cons;
 } lstream;
 
 static atkgpu_server *g_srv;
 static lstream *g_st;
 static int g_ns;
 static double g_t0, g_seconds, g_speed;
+static int g_ring, g_consumers = 4, g_ring_blocks = 64;
+static volatile int g_producers_done;
+static pthread_mutex_t g_pick_mu = PTHREAD_MUTEX_INITIALIZER;
+static long long g_picked, g_lost;
+static double g_pick_sum, g_pick_max, g_call_sum, g_call_max, g_pass_max, g_late_max;
 
 static double now_s(void) {
     struct timespec ts;
@@ -127,6 +156,7 @@ static void make_source(lstream *s) {
 static void *submitter(void *arg) {
     const int tid = (int)(long)arg;
     unsigned char *tmp = (unsigned char *)malloc((size_t)CHUNK_MAX * 8 * 8);
+    double late = 0;               /* the most this thread started a chunk after its arrival time */
     for (;;) {
         /* the earliest next arrival among this thread's streams */
         int best = -1;
@@ -142,6 +172,8 @@ static void *submitter(void *arg) {
         if (d > 0) {
             struct timespec ts = {(time_t)d, (long)((d - (double)(time_t)d) * 1e9)};
             nanosleep(&ts, NULL);
+        } else if (-d > late) {
+            late = -d;
         }
         const size_t n = 1 + (size_t)(splitmix(&s->rng) % CHUNK_MAX);
         const size_t bytes = n * (size_t)s->bps * s->nel;
@@ -159,14 +191,80 @@ static void *submitter(void *arg) {
             }
             p = tmp;
         }
+        const size_t first = s->pos;
         s->pos = (s->pos + n) % SRC_SAMPLES;
-        if (atkgpu_server_submit(g_srv, s->sid, p, bytes) < 0) break;
+        if (g_ring) {
+            void *blk = atkdsp_ring_claim(s->ring);
+            if (!blk) break;
+            memcpy(blk, p, bytes);
+            atkdsp_ring_meta m;
+            memset(&m, 0, sizeof m);
+            m.first_sample = first;
+            m.t_utc_ns = (int64_t)(now_s() * 1e9);
+            m.fs_hz = s->fs;
+            m.format = s->fmt;
+            m.nbytes = (uint32_t)bytes;
+            if (atkdsp_ring_commit(s->ring, &m) != ATKDSP_OK) break;
+        } else if (atkgpu_server_submit(g_srv, s->sid, p, bytes) < 0) {
+            break;
+        }
         s->submitted++;
         /* Poisson arrivals at the stream's rate: exponential gaps, mean chunk / fs */
         const double gap = -log(urand(&s->rng)) * ((CHUNK_MAX + 1) / 2.0) / (s->fs * g_speed);
         s->next_t += gap;
     }
     free(tmp);
+    pthread_mutex_lock(&g_pick_mu);
+    if (late > g_late_max) g_late_max = late;
+    pthread_mutex_unlock(&g_pick_mu);
+    return NULL;
+}
+
+/* --ring: take every committed block of this thread's streams into the
+ * server until the producers are done and the rings are empty */
+static void *ring_consumer(void *arg) {
+    const int tid = (int)(long)arg;
+    long long picked = 0, lost_n = 0;
+    double sum = 0, mx = 0, csum = 0, cmx = 0, pmx = 0;
+    for (;;) {
+        const int done = g_producers_done;
+        int took = 0;
+        const double p0 = now_s();
+        for (int i = tid; i < g_ns; i += g_consumers) {
+            lstream *s = &g_st[i];
+            for (;;) {
+                atkdsp_ring_meta m;
+                uint64_t lost = 0;
+                const double c0 = now_s();
+                const long long t = atkgpu_server_submit_ring(g_srv, s->sid, s->ring, s->cons, &m, &lost);
+                const double c1 = now_s();
+                lost_n += (long long)lost;
+                if (t < 0) break;
+                csum += c1 - c0;
+                if (c1 - c0 > cmx) cmx = c1 - c0;
+                const double d = c0 - 1e-9 * (double)m.t_utc_ns;
+                sum += d;
+                if (d > mx) mx = d;
+                ++picked;
+                took = 1;
+            }
+        }
+        if (now_s() - p0 > pmx) pmx = now_s() - p0;
+        if (!took) {
+            if (done) break;
+            struct timespec ts = {0, RING_POLL_US * 1000L};
+            nanosleep(&ts, NULL);
+        }
+    }
+    pthread_mutex_lock(&g_pick_mu);
+    g_picked += picked;
+    g_lost += lost_n;
+    g_pick_sum += sum;
+    if (mx > g_pick_max) g_pick_max = mx;
+    g_call_sum += csum;
+    if (cmx > g_call_max) g_call_max = cmx;
+    if (pmx > g_pass_max) g_pass_max = pmx;
+    pthread_mutex_unlock(&g_pick_mu);
     return NULL;
 }
 
@@ -185,7 +283,11 @@ int main(int argc, char **argv) {
         else if (!strcmp(argv[i], "--mix")) uniform = !strcmp(argv[i + 1], "uniform");
         else if (!strcmp(argv[i], "--arrays")) arrays = atoi(argv[i + 1]);
         else if (!strcmp(argv[i], "--display")) display = atoi(argv[i + 1]);
+        else if (!strcmp(argv[i], "--ring")) g_ring = atoi(argv[i + 1]);
+        else if (!strcmp(argv[i], "--consumers")) g_consumers = atoi(argv[i + 1]);
+        else if (!strcmp(argv[i], "--ring-blocks")) g_ring_blocks = atoi(argv[i + 1]);
     }
+    if (g_consumers < 1 || g_consumers > NTHREADS) { fprintf(stderr, "serve_load: --consumers 1..%d\n", NTHREADS); return 2; }
     g_srv = atkgpu_server_create((size_t)(budget_gib * (1u << 30)), (unsigned)(streams + arrays),
                                  (unsigned)max_batch, (size_t)1024 << 20);
     if (!g_srv) { fprintf(stderr, "serve_load: no server (no GPU?)\n"); return 2; }
@@ -235,6 +337,15 @@ int main(int argc, char **argv) {
         if (sid < 0) { ++refused; continue; }
         ls->sid = sid;
         make_source(ls);
+        if (g_ring) {
+            const size_t bb = (size_t)CHUNK_MAX * ls->bps * ls->nel;
+            const size_t rb = atkdsp_ring_bytes(bb, (size_t)g_ring_blocks, 1);
+            if (!rb || posix_memalign(&ls->ring_mem, 64, rb)) { fprintf(stderr, "serve_load: ring\n"); return 2; }
+            memset(ls->ring_mem, 0, rb);
+            ls->ring = atkdsp_ring_init(ls->ring_mem, rb, bb, (size_t)g_ring_blocks, 1);
+            ls->cons = ls->ring ? atkdsp_ring_open(ls->ring, 0) : -1;
+            if (ls->cons < 0) { fprintf(stderr, "serve_load: ring\n"); return 2; }
+        }
         ls->next_t = -log(urand(&ls->rng)) * ((CHUNK_MAX + 1) / 2.0) / (ls->fs * speed);
         total_rate += ls->fs * ls->nel;
         ++admitted;
@@ -244,11 +355,39 @@ int main(int argc, char **argv) {
     g_speed = speed;
     fprintf(stderr, "serve_load: %d streams admitted (%d refused), %.1f MS/s offered x%.1f, %.1f s\n", admitted,
             refused, total_rate / 1e6, speed, seconds);
-    pthread_t th[NTHREADS];
+    /* warm-up: one full chunk per stream, then a clean slate */
+    for (int s = 0; s < admitted; ++s) {
+        lstream *ls = &g_st[s];
+        const size_t n = ls->nel == 1 ? CHUNK_MAX : CHUNK_MAX / 8;
+        if (atkgpu_server_submit(g_srv, ls->sid, ls->src, n * (size_t)ls->bps * ls->nel) < 0) {
+            fprintf(stderr, "serve_load: warm-up submit failed\n");
+            return 2;
+        }
+    }
+    atkgpu_server_drain(g_srv);
+    {
+        atkgpu_serve_stats w;
+        atkgpu_server_stats(g_srv, &w, 1);
+    }
+    pthread_t th[NTHREADS], rc[NTHREADS];
+    struct rusage ru0, ru1;
+    getrusage(RUSAGE_SELF, &ru0);
+    const double w0 = now_s();
     g_t0 = now_s() + 0.05;
     for (long t = 0; t < NTHREADS; ++t) pthread_create(&th[t], NULL, submitter, (void *)t);
+    if (g_ring)
+        for (long t = 0; t < g_consumers; ++t) pthread_create(&rc[t], NULL, ring_consumer, (void *)t);
     for (int t = 0; t < NTHREADS; ++t) pthread_join(th[t], NULL);
+    g_producers_done = 1;
+    if (g_ring)
+        for (int t = 0; t < g_consumers; ++t) pthread_join(rc[t], NULL);
     atkgpu_server_drain(g_srv);
+    getrusage(RUSAGE_SELF, &ru1);
+    const double cpu_s = (double)(ru1.ru_utime.tv_sec - ru0.ru_utime.tv_sec) +
+                         1e-6 * (double)(ru1.ru_utime.tv_usec - ru0.ru_utime.tv_usec) +
+                         (double)(ru1.ru_stime.tv_sec - ru0.ru_stime.tv_sec) +
+                         1e-6 * (double)(ru1.ru_stime.tv_usec - ru0.ru_stime.tv_usec);
+    const double cpus = cpu_s / (now_s() - w0);
     atkgpu_serve_stats st;
     atkgpu_server_stats(g_srv, &st, 0);
     long long lines = 0;
@@ -259,12 +398,20 @@ int main(int argc, char **argv) {
     printf("{\"mix\": \"%s\", \"arrays\": %d, \"streams\": %d, \"refused\": %d, \"offered_msps\": %.2f, \"speed\": %.2f, \"seconds\": %.2f, "
            "\"chunks\": %lld, \"samples\": %lld, \"batches\": %lld, \"msamples_per_s\": %.2f, "
            "\"p50_ms\": %.4f, \"p99_ms\": %.4f, \"p999_ms\": %.4f, \"max_ms\": %.4f, \"mean_ms\": %.4f, "
-           "\"hbm_used_gib\": %.3f, \"hbm_budget_gib\": %.1f, \"display\": %d, \"lines\": %lld}\n",
+           "\"hbm_used_gib\": %.3f, \"hbm_budget_gib\": %.1f, \"display\": %d, \"lines\": %lld, "
+           "\"ring\": %d, \"ring_lost\": %lld, \"ring_pickup_mean_ms\": %.4f, \"ring_pickup_max_ms\": %.4f, "
+           "\"ring_call_mean_us\": %.2f, \"ring_call_max_ms\": %.4f, \"consumers\": %d, \"ring_blocks\": %d, \"ring_pass_max_ms\": %.3f, \"arrival_late_max_ms\": %.3f, "
+           "\"host_cpus_busy\": %.2f}\n",
            uniform ? "uniform" : "c4", arrays, admitted, refused, total_rate / 1e6, speed, seconds, st.chunks, st.samples, st.batches,
            st.busy_s > 0 ? st.samples / st.busy_s / 1e6 : 0.0, st.p50_ms, st.p99_ms, st.p999_ms, st.max_ms,
-           st.mean_ms, st.hbm_used / 1073741824.0, st.hbm_budget / 1073741824.0, display, lines);
+           st.mean_ms, st.hbm_used / 1073741824.0, st.hbm_budget / 1073741824.0, display, lines, g_ring, g_lost,
+           g_picked ? 1e3 * g_pick_sum / (double)g_picked : 0.0, 1e3 * g_pick_max,
+           g_picked ? 1e6 * g_call_sum / (double)g_picked : 0.0, 1e3 * g_call_max, g_ring ? g_consumers : 0, g_ring ? g_ring_blocks : 0, 1e3 * g_pass_max, 1e3 * g_late_max, cpus);
     atkgpu_server_destroy(g_srv);
-    for (int s = 0; s < admitted; ++s) free(g_st[s].src);
+    for (int s = 0; s < admitted; ++s) {
+        free(g_st[s].src);
+        free(g_st[s].ring_mem);
+    }
     free(g_st);
     return 0;
 }

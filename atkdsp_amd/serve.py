@@ -4,6 +4,7 @@
     sid = srv.add_stream(fmt="ci16", fs=40e6, nchan=1024, taps_per_poly=16, chunk_max=65536)
     t = srv.submit(sid, raw_bytes)        # returns a ticket at once
     y = srv.read(sid, t)                  # [nchan][m] once the chunk is done
+    t, meta, lost = srv.submit_ring(sid, ring, consumer)   # or from a block ring
     srv.stats()                           # chunks, samples, p50/p99/p99.9 latency
 
 A stream may carry a display (display=dict(nfft=4096, hop=2048, frames=977,
@@ -16,7 +17,9 @@ import ctypes as C
 
 import numpy as np
 
-from . import AtkDspError, _Cf32, _check, _fmt, load
+from . import AtkDspError, _Cf32, _RingMeta, _check, _fmt, _meta_dict, load
+
+RING_EMPTY = -64               # atkgpu.h ATKGPU_RING_EMPTY: the ring consumer has nothing new
 
 
 class _StreamCfg(C.Structure):
@@ -48,6 +51,8 @@ def _bind(lib) -> None:
         "atkgpu_server_add_stream": (C.c_int, [vp, P(_StreamCfg)]),
         "atkgpu_server_remove_stream": (C.c_int, [vp, C.c_int]),
         "atkgpu_server_submit": (C.c_longlong, [vp, C.c_int, vp, sz]),
+        "atkgpu_server_submit_ring": (C.c_longlong, [vp, C.c_int, vp, C.c_int, P(_RingMeta),
+                                                     P(C.c_uint64)]),
         "atkgpu_server_wait": (C.c_int, [vp, C.c_int, C.c_longlong]),
         "atkgpu_server_read": (C.c_ssize_t, [vp, C.c_int, C.c_longlong, P(_Cf32), sz]),
         "atkgpu_server_drain": (C.c_int, [vp]),
@@ -122,6 +127,17 @@ class Server:
         b = np.frombuffer(raw, np.uint8) if not isinstance(raw, np.ndarray) else raw.view(np.uint8).ravel()
         t = self._lib.atkgpu_server_submit(self._h, int(sid), b.ctypes.data, b.size)
         return _check(int(t), "server_submit")
+
+    def submit_ring(self, sid: int, ring, consumer: int):
+        """The next intact block of a ring consumer (atkdsp_amd.Ring) as one
+        chunk: (ticket, block meta, blocks lost before it), or (None, None,
+        lost) when the consumer has nothing new."""
+        m, lost = _RingMeta(), C.c_uint64()
+        t = int(self._lib.atkgpu_server_submit_ring(self._h, int(sid), ring.handle, int(consumer), C.byref(m),
+                                                    C.byref(lost)))
+        if t == RING_EMPTY:
+            return None, None, lost.value
+        return _check(t, "server_submit_ring"), _meta_dict(m), lost.value
 
     def wait(self, sid: int, ticket: int) -> None:
         _check(self._lib.atkgpu_server_wait(self._h, int(sid), int(ticket)), "server_wait")

@@ -141,6 +141,18 @@ ATKGPU_API int       atkgpu_server_remove_stream(atkgpu_server *v, int sid);
 /* copies the chunk (whole samples of nbytes) and returns its ticket (>= 0);
  * blocks while the stream's queue or the staging ring is full */
 ATKGPU_API long long atkgpu_server_submit(atkgpu_server *v, int sid, const void *raw, size_t nbytes);
+/* submit from a block ring (atkdsp_ring_*): the next intact block of the
+ * ring consumer becomes one chunk of the stream, copied once, from the ring
+ * slot straight into staging. Returns the ticket, or ATKGPU_RING_EMPTY when
+ * the consumer has nothing new; *meta gets the block's header (first_sample
+ * and time tag, for the caller's continuity checks) and *lost the blocks the
+ * consumer lost before it (lapped by the producer, or rewritten while being
+ * copied; those are skipped, as atkdsp_ring_read skips them). A block whose
+ * format is not the stream's, or that is larger than chunk_max, is not
+ * consumed: ATKDSP_E_ARG. meta and lost may be NULL. */
+#define ATKGPU_RING_EMPTY (-64)
+ATKGPU_API long long atkgpu_server_submit_ring(atkgpu_server *v, int sid, atkdsp_ring *ring, int consumer,
+                                               atkdsp_ring_meta *meta, uint64_t *lost);
 ATKGPU_API int       atkgpu_server_wait(atkgpu_server *v, int sid, long long ticket);
 /* the chunk's filterbank output, [nchan][m] channel-major; returns m. Valid
  * until out_slots later chunks of the stream have been dispatched. */
