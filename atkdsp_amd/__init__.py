@@ -749,6 +749,10 @@ class Ring:
         return np.ctypeslib.as_array((C.c_uint8 * self.block_bytes).from_address(p))
 
     def commit(self, nbytes: int, **meta) -> None:
+        """publish the claimed block; meta: first_sample, t_utc_ns, fc_hz, fs_hz,
+        gain_db, format (an enum value or a name such as "ci16"), flags"""
+        if isinstance(meta.get("format"), str):
+            meta["format"] = _fmt(meta["format"])
         m = _RingMeta(**{k: v for k, v in meta.items() if k != "seq"})
         m.nbytes = int(nbytes)
         _check(self._lib.atkdsp_ring_commit(self._r, C.byref(m)), "ring_commit")

@@ -24,3 +24,16 @@ gcc -std=c99 -O3 -fno-math-errno -mavx2 -mfma -fopenmp -fPIC -shared \
     -o "$OUT/atkdsp_ref.so.tmp" -lm -Wl,--no-undefined 2>"$OUT/build.log"
 mv "$OUT/atkdsp_ref.so.tmp" "$OUT/atkdsp_ref.so"
 echo "build_ref: $OUT/atkdsp_ref.so" >&2
+
+# The reference's own ring stress test (tests/test_ring_stress.c), compiled
+# unmodified with the reference header, twice: against the reference library
+# and against this repository's drop-in (atkdsp_amd/atkdsp.so, when built).
+gcc -std=c99 -D_POSIX_C_SOURCE=200809L -O2 -Iinclude tests/test_ring_stress.c -o "$OUT/ring_stress_ref" \
+    "$OUT/atkdsp_ref.so" -Wl,-rpath,"$OUT" -lpthread 2>>"$OUT/build.log"
+OURS="$HERE/../atkdsp_amd/atkdsp.so"
+if [ -f "$OURS" ]; then
+    OURS=$(cd "$(dirname "$OURS")" && pwd)/atkdsp.so
+    gcc -std=c99 -D_POSIX_C_SOURCE=200809L -O2 -Iinclude tests/test_ring_stress.c -o "$OUT/ring_stress_ours" \
+        "$OURS" -Wl,-rpath,"$(dirname "$OURS")" -lpthread 2>>"$OUT/build.log"
+fi
+echo "build_ref: $OUT/ring_stress_ref, ring_stress_ours" >&2

@@ -1,12 +1,16 @@
 """The block ring (reference include/atkdsp.h:2020-2107, src/ring.c), host code:
-layout parity with the reference build (a ring made by either library, after
-the same calls, is the same bytes, and each attaches to the other's), the
+the reference's own stress test (tests/test_ring_stress.c, compiled unmodified
+with the reference header) linked against this library; layout parity with
+the reference build (a ring made by either library, after the same calls, is
+the same bytes, and each attaches to the other's), the
 producer that never waits (a lapped consumer is told exactly how many blocks
 it lost), E_CAP on a short buffer, zero-copy peek/release with overrun
 detection, copy_range, and a threaded stress run in which no torn block is
 ever returned and every block is either read or counted lost."""
 import ctypes as C
+import subprocess
 import threading
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -15,6 +19,21 @@ import atkdsp_amd as A
 from oracle import ref as R
 
 needs_ref = pytest.mark.skipif(not R.available(), reason="oracle/_ref not built")
+ROOT = Path(__file__).resolve().parents[1]
+STRESS = ROOT / "oracle" / "_ref" / "ring_stress_ours"
+
+
+@pytest.mark.skipif(not STRESS.exists(), reason="the reference stress test is not built (oracle/build_ref.sh)")
+@pytest.mark.parametrize("args", [[], ["3", "160", "1024", "128"]], ids=["full_speed_2s", "160MBps_1MiB_blocks"])
+def test_reference_stress_test_passes_on_this_library(args):
+    """one producer, a fast, a slow (3x work) and a stalled consumer: every
+    block read is whole and in order, and read + lost is exact per consumer
+    (reference tests/test_ring_stress.c:1-18; the 160 MB/s case is the shape
+    of bench/ring_bench_result.txt, 3 s instead of 600 s)"""
+    ldd = subprocess.run(["ldd", str(STRESS)], capture_output=True, text=True, check=True).stdout
+    assert str(ROOT / "atkdsp_amd" / "atkdsp.so") in ldd           # this library, not the reference's
+    r = subprocess.run([str(STRESS), *args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "RING STRESS: ok" in r.stdout, r.stdout + r.stderr
 
 
 def _ref_lib():
