@@ -40,6 +40,27 @@ class _Cf32(C.Structure):
     _fields_ = [("re", C.c_float), ("im", C.c_float)]
 
 
+class _ChannelC(C.Structure):
+    _fields_ = [("start_bin", C.c_int), ("end_bin", C.c_int), ("centroid_bin", C.c_float),
+                ("peak_db", C.c_float), ("snr_db", C.c_float)]
+
+
+class Channel:
+    """One detected channel, in bins [start_bin, end_bin) (the caller knows the bin width)."""
+    __slots__ = ("start_bin", "end_bin", "centroid_bin", "peak_db", "snr_db")
+
+    def __init__(self, start_bin, end_bin, centroid_bin, peak_db, snr_db):
+        self.start_bin, self.end_bin = int(start_bin), int(end_bin)
+        self.centroid_bin, self.peak_db, self.snr_db = float(centroid_bin), float(peak_db), float(snr_db)
+
+    def as_tuple(self):
+        return (self.start_bin, self.end_bin, self.centroid_bin, self.peak_db, self.snr_db)
+
+    def __repr__(self) -> str:
+        return (f"Channel(bins {self.start_bin}..{self.end_bin}, centroid {self.centroid_bin:.1f}, "
+                f"peak {self.peak_db:.1f} dB, snr {self.snr_db:.1f} dB)")
+
+
 class _RingMeta(C.Structure):
     _fields_ = [("seq", C.c_uint64), ("first_sample", C.c_uint64), ("t_utc_ns", C.c_int64),
                 ("fc_hz", C.c_double), ("fs_hz", C.c_double), ("gain_db", C.c_float), ("format", C.c_int32),
@@ -96,6 +117,9 @@ def _bind(lib) -> None:
         "atkdsp_db_to_pixels": (None, [fp, sz, C.c_float, C.c_float, P(C.c_uint32), P(C.c_uint32)]),
         "atkdsp_decimate_max": (None, [fp, sz, fp, sz]),
         "atkdsp_stitch_max": (None, [fp, sz, d, d, d, d, d, d, fp, sz]),
+        "atkdsp_median": (C.c_float, [fp, sz, fp]),
+        "atkdsp_detect_channels": (C.c_ssize_t, [fp, sz, C.c_float, C.c_float, C.c_int, C.c_int, fp,
+                                                 C.POINTER(_ChannelC), sz]),
         "atkdsp_pfb_create": (vp, [d, u, u, u, d, u]),
         "atkdsp_pfb_destroy": (None, [vp]),
         "atkdsp_pfb_reset": (None, [vp]),
@@ -420,6 +444,32 @@ def decimate_max(x, m: int, out: np.ndarray | None = None) -> np.ndarray:
         out = np.empty(m, dtype=np.float32)
     load().atkdsp_decimate_max(_fp(x), x.size, _fp(out), m)
     return out
+
+
+def median(x, scratch: np.ndarray | None = None) -> float:
+    """The median of x (a robust noise floor; numpy's rule for even n)."""
+    x = _f32(x, "x")
+    if scratch is None or scratch.size < x.size:
+        scratch = np.empty(max(x.size, 1), dtype=np.float32)
+    return float(load().atkdsp_median(_fp(x), x.size, _fp(scratch)))
+
+
+def _channels(buf, n) -> list:
+    return [Channel(c.start_bin, c.end_bin, c.centroid_bin, c.peak_db, c.snr_db) for c in buf[:n]]
+
+
+def detect_channels(line, floor_db: float, threshold_db: float, gap_bins: int, min_run: int,
+                    max_channels: int = 256, scratch: np.ndarray | None = None) -> list:
+    """Runs of the 5-bin-smoothed line above floor + threshold (gaps under
+    gap_bins bridged, runs under min_run dropped), strongest first."""
+    line = _f32(line, "line")
+    if scratch is None or scratch.size < line.size:
+        scratch = np.empty(max(line.size, 1), dtype=np.float32)
+    buf = (_ChannelC * max(int(max_channels), 1))()
+    got = _check(int(load().atkdsp_detect_channels(_fp(line), line.size, float(floor_db), float(threshold_db),
+                                                   int(gap_bins), int(min_run), _fp(scratch), buf,
+                                                   int(max_channels))), "detect_channels")
+    return _channels(buf, got)
 
 
 def stitch_max(seg, seg_lo_hz: float, seg_hz_per_bin: float, keep_lo_hz: float,
@@ -822,7 +872,7 @@ __all__ = [
     "ABI_VERSION", "AtkDspError", "load", "available", "lib_path", "version", "build_info",
     "set_threads", "get_threads", "FMT", "DET", "WIN", "bytes_per_sample",
     "unpack", "unpack_dc", "Fft", "window", "power_db", "spectrum_reduce", "spectrum_stats",
-    "window_stats", "db_to_pixels", "decimate_max", "stitch_max",
+    "window_stats", "db_to_pixels", "decimate_max", "stitch_max", "median", "detect_channels", "Channel",
     "Pfb", "pfb_cov_accumulate", "pfb_cov_reset", "pfb_channel_power",
     "design_lowpass", "ArbResampler", "Osx", "Ring", "ring_bytes",
 ]

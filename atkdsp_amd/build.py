@@ -58,6 +58,7 @@ IMPLEMENTED = {
     "atkdsp_ring_bytes", "atkdsp_ring_init", "atkdsp_ring_attach", "atkdsp_ring_claim", "atkdsp_ring_commit",
     "atkdsp_ring_open", "atkdsp_ring_close", "atkdsp_ring_read", "atkdsp_ring_peek", "atkdsp_ring_release",
     "atkdsp_ring_copy_range", "atkdsp_ring_stats", "atkdsp_ring_describe",
+    "atkdsp_median", "atkdsp_detect_channels",
 }
 
 _STUB_BODY = {

@@ -86,6 +86,21 @@ int atkgpu_spec_stats_db(void *s, size_t n, long long frames, const float *mx,
 int atkgpu_power_db(void *s, const void *X, size_t n, float *out_db);
 
 /* ---- display ------------------------------------------------------------------ */
+/* detection on spectrum lines (reference src/detect.c:4-114); a record is
+ * atkdsp_channel's layout */
+typedef struct {
+    int start_bin, end_bin;
+    float centroid_bin, peak_db, snr_db;
+} atkgpu_chan;
+/* nlines lines of n bins, line l at lines + l*lstride; scratch n floats per
+ * line; out cap records per line (bin order), found[l] the runs found,
+ * floors[l] the floor used (the median of the smoothed line when
+ * compute_floor, else floor_db); gap >= 2, min_run >= 2 */
+int atkgpu_detect_lines(void *s, const float *lines, size_t lstride, size_t n, unsigned nlines, float floor_db,
+                        int compute_floor, float threshold_db, int gap, int min_run, float *scratch,
+                        atkgpu_chan *out, size_t cap, unsigned long long *found, float *floors);
+/* the median of x[0..n) into *out (device) */
+int atkgpu_median(void *s, const float *x, size_t n, float *out);
 int atkgpu_decimate_max(void *s, const float *in, size_t n, float *out, size_t m);
 int atkgpu_db_to_pixels(void *s, const float *db, size_t n, float lo, float hi,
                         const uint32_t *lut, uint32_t *out);

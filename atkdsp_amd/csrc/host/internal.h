@@ -38,4 +38,7 @@ typedef struct {
 } atk_pool;
 atk_pool *atk_pool_get(void);
 
+/* channel records strongest first, with the reference's comparator and qsort */
+void atk_sort_channels(atkdsp_channel *c, size_t n);
+
 #endif

@@ -87,6 +87,8 @@ int atkgpu_pfb_prototype(double *h, unsigned nchan, unsigned K, double atten_db)
 #define PFLOATS_CAP (32u << 20) /* display power spectra per batch, floats (128 MiB) */
 #define UBLK_CAP 16384         /* ingest blocks per batch */
 #define PBLK_CAP 16384         /* filterbank blocks per batch */
+#define QREC_CAP 1088          /* channel records of one detect query (>= 4096 / 4 + 1) */
+#define DISP_COLS_MAX 4096
 
 typedef struct {
     long long ticket;
@@ -178,6 +180,11 @@ struct atkgpu_server {
     stage_rec *recs;
     size_t rec_cap, rec_head, rec_tail;
     void *stream, *ustream, *dstream;            /* compute, ingest, display */
+    /* line queries (detect): their own stream, scratch and mapped results */
+    pthread_mutex_t qmu;
+    void *qstream;
+    float *q_scratch;                            /* device, DISP_COLS_MAX floats */
+    void *q_h, *q_d;                             /* mapped: records, count, floor */
     atkgpu_fftplan *plans[NPLANS];               /* twiddles for 64..4096 points */
     batch_t batch[NINFLIGHT];
     unsigned b_submit, b_done;                   /* batches launched / completed */
@@ -249,7 +256,10 @@ atkgpu_server *atkgpu_server_create(size_t hbm_budget, unsigned max_streams, uns
     v->lat = (double *)malloc(LAT_CAP * sizeof(double));
     v->lat_sort = (double *)malloc(LAT_CAP * sizeof(double));
     int ok = v->st && v->stage_h && v->recs && v->lat && v->lat_sort && !atkgpu_stream_create(&v->stream) &&
-             !atkgpu_stream_create(&v->ustream) && !atkgpu_stream_create(&v->dstream);
+             !atkgpu_stream_create(&v->ustream) && !atkgpu_stream_create(&v->dstream) &&
+             !atkgpu_stream_create(&v->qstream) &&
+             (v->q_scratch = (float *)atkgpu_dmalloc(DISP_COLS_MAX * sizeof(float))) &&
+             (v->q_h = atkgpu_hmalloc_mapped(QREC_CAP * sizeof(atkgpu_chan) + 64, &v->q_d));
     for (int i = 0; ok && i < NPLANS; ++i) ok = (v->plans[i] = atkgpu_fftplan_create(64u << i)) != NULL;
     for (int i = 0; ok && i < NINFLIGHT; ++i) {
         batch_t *b = &v->batch[i];
@@ -266,6 +276,7 @@ atkgpu_server *atkgpu_server_create(size_t hbm_budget, unsigned max_streams, uns
      * 11 us per 128 KiB, measured) */
     memset(v->stage_h, 0, v->stage_cap);
     pthread_mutex_init(&v->mu, NULL);
+    pthread_mutex_init(&v->qmu, NULL);
     pthread_cond_init(&v->cv_work, NULL);
     pthread_cond_init(&v->cv_launch, NULL);
     pthread_cond_init(&v->cv_done, NULL);
@@ -323,6 +334,9 @@ void atkgpu_server_destroy(atkgpu_server *v) {
     if (v->stream) atkgpu_stream_destroy(v->stream);
     if (v->ustream) atkgpu_stream_destroy(v->ustream);
     if (v->dstream) atkgpu_stream_destroy(v->dstream);
+    if (v->qstream) atkgpu_stream_destroy(v->qstream);
+    atkgpu_dfree(v->q_scratch);
+    atkgpu_hfree(v->q_h);
     atkgpu_hfree(v->stage_h);
     free(v->recs);
     free(v->lat);
@@ -1031,6 +1045,52 @@ int atkgpu_server_read_line(atkgpu_server *v, int sid, long long line, float *ou
     if (gone) return ATKDSP_E_ARG;
     if (cap < len) return ATKDSP_E_CAP;
     return atkgpu_d2h_sync(out, src, len * sizeof(float)) ? ATKDSP_E_FFT : ATKDSP_OK;
+}
+
+/* channels of a served line, found on the device where the line lives
+ * (atkdsp_detect_channels' policy; the floor, when NaN is passed, is the
+ * median of the 5-bin-smoothed line as signal_id computes it): only the
+ * records cross PCIe, strongest first */
+ptrdiff_t atkgpu_server_detect(atkgpu_server *v, int sid, long long line, unsigned element, float floor_db,
+                               float threshold_db, int gap_bins, int min_run, atkdsp_channel *out, size_t out_cap,
+                               float *floor_used) {
+    if (!v || (!out && out_cap) || sid < 0 || (unsigned)sid >= v->max_streams || line < 0) return ATKDSP_E_ARG;
+    pthread_mutex_lock(&v->mu);
+    sstream *s = &v->st[sid];
+    if (!s->used || !s->dN || element >= s->nel || line >= disp_frames_at(s->g_sub, s->dN, s->dhop) / s->dF) {
+        pthread_mutex_unlock(&v->mu);
+        return ATKDSP_E_ARG;
+    }
+    while (s->lines_done <= line && !v->failed) pthread_cond_wait(&v->cv_done, &v->mu);
+    const int failed = v->failed;
+    const int gone = line + (long long)s->dslots < s->lines_disp;
+    const size_t cols = s->dcols;
+    const float *src = s->dring + (size_t)(line % s->dslots) * s->nel * cols + (size_t)element * cols;
+    pthread_mutex_unlock(&v->mu);
+    if (failed) return ATKDSP_E_FFT;
+    if (gone) return ATKDSP_E_ARG;
+    if (gap_bins < 2) gap_bins = 2;
+    if (min_run < 2) min_run = 2;
+    const size_t cap = out_cap < QREC_CAP ? out_cap : QREC_CAP;
+    atkgpu_chan *rec_h = (atkgpu_chan *)v->q_h, *rec_d = (atkgpu_chan *)v->q_d;
+    unsigned long long *found_h = (unsigned long long *)(rec_h + QREC_CAP);
+    unsigned long long *found_d = (unsigned long long *)(rec_d + QREC_CAP);
+    float *floor_h = (float *)(found_h + 1), *floor_d = (float *)(found_d + 1);
+    pthread_mutex_lock(&v->qmu);
+    int rc = atkgpu_detect_lines(v->qstream, src, cols, cols, 1, floor_db, floor_db != floor_db, threshold_db,
+                                 gap_bins, min_run, v->q_scratch, rec_d, cap, found_d, floor_d) ||
+             atkgpu_sync(v->qstream);
+    const unsigned long long found = rc ? 0 : *found_h;
+    const size_t kept = found < cap ? (size_t)found : cap;
+    if (!rc) {
+        memcpy(out, rec_h, kept * sizeof(atkgpu_chan));
+        if (floor_used) *floor_used = *floor_h;
+    }
+    pthread_mutex_unlock(&v->qmu);
+    if (rc) return ATKDSP_E_FFT;
+    if (found > out_cap) return ATKDSP_E_CAP;
+    atk_sort_channels(out, kept);
+    return (ptrdiff_t)kept;
 }
 
 int atkgpu_server_drain(atkgpu_server *v) {

@@ -17,7 +17,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import AtkDspError, _Cf32, _RingMeta, _check, _fmt, _meta_dict, load
+from . import AtkDspError, _ChannelC, _Cf32, _RingMeta, _channels, _check, _fmt, _meta_dict, load
 
 RING_EMPTY = -64               # atkgpu.h ATKGPU_RING_EMPTY: the ring consumer has nothing new
 
@@ -61,6 +61,8 @@ def _bind(lib) -> None:
         "atkgpu_server_stats": (C.c_int, [vp, P(_Stats), C.c_int]),
         "atkgpu_server_lines": (C.c_longlong, [vp, C.c_int]),
         "atkgpu_server_read_line": (C.c_int, [vp, C.c_int, C.c_longlong, P(C.c_float), sz]),
+        "atkgpu_server_detect": (C.c_ssize_t, [vp, C.c_int, C.c_longlong, C.c_uint, C.c_float, C.c_float,
+                                               C.c_int, C.c_int, P(_ChannelC), sz, P(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -183,6 +185,17 @@ class Server:
                "server_read_line")
         return out if nel == 1 else out.reshape(nel, cols)
 
+    def detect(self, sid: int, line: int, element: int = 0, floor_db: float | None = None,
+               threshold_db: float = 6.0, gap_bins: int = 4, min_run: int = 2, max_channels: int = 256):
+        """detect_channels over a served line on the device: (channels strongest
+        first, the floor used). floor_db None = the median of the smoothed line."""
+        buf = (_ChannelC * max(int(max_channels), 1))()
+        fl = C.c_float()
+        n = _check(int(self._lib.atkgpu_server_detect(self._h, int(sid), int(line), int(element),
+                                                      float("nan") if floor_db is None else float(floor_db),
+                                                      float(threshold_db), int(gap_bins), int(min_run), buf,
+                                                      int(max_channels), C.byref(fl))), "server_detect")
+        return _channels(buf, n), float(fl.value)
     def drain(self) -> None:
         _check(self._lib.atkgpu_server_drain(self._h), "server_drain")
 

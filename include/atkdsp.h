@@ -18,7 +18,7 @@
  *   - NaN means "unmeasured" (no fast-math anywhere).
  *
  * The library also exports every other name of the reference ABI (334 in
- * all; 71 are implemented here) as a documented stub: *_create returns NULL, signed counts return
+ * all; 73 are implemented here) as a documented stub: *_create returns NULL, signed counts return
  * ATKDSP_E_ARG, unsigned counts 0, floating results NaN, void functions do
  * nothing. Those are outside the wideband front end.
  */
@@ -117,6 +117,26 @@ ATKDSP_API void atkdsp_stitch_max(const float *seg, size_t nseg,
                                   double keep_lo_hz, double keep_hi_hz,
                                   double out_lo_hz, double out_hz_per_bin,
                                   float *out, size_t nout);
+
+/* ---- detection on a spectrum line: reference include/atkdsp.h:465-486,
+ * src/detect.c:4-114. Both run on the GPU (one block per line): the median is
+ * an exact radix select, so it returns the reference's value (at most the
+ * sign of a zero median may differ); detect_channels smooths, thresholds,
+ * bridges gaps and measures runs on the device and orders the records on the
+ * host with the reference's qsort comparator. smooth_scratch must be given,
+ * as in the reference, but is not written. Lines longer than 2^21 bins (and
+ * median inputs longer than 2^23) are refused: ATKDSP_E_ARG / NaN. ---- */
+ATKDSP_API float atkdsp_median(const float *x, size_t n, float *scratch);
+typedef struct {
+    int   start_bin, end_bin;   /* [start, end) */
+    float centroid_bin;         /* power-weighted centre */
+    float peak_db;
+    float snr_db;               /* peak - floor */
+} atkdsp_channel;
+ATKDSP_API ptrdiff_t atkdsp_detect_channels(const float *line, size_t n, float floor_db,
+                                            float threshold_db, int gap_bins, int min_run,
+                                            float *smooth_scratch,
+                                            atkdsp_channel *out, size_t out_cap);
 
 /* ---- NCO: reference include/atkdsp.h:309-319, src/nco.c:27-98.
  * out[n] = in[n] * exp(j*(phase + n*step)); the phase advances by n*step,

@@ -170,6 +170,15 @@ ATKGPU_API int       atkgpu_server_stats(atkgpu_server *v, atkgpu_serve_stats *s
  * have been dispatched. */
 ATKGPU_API long long atkgpu_server_lines(atkgpu_server *v, int sid);
 ATKGPU_API int       atkgpu_server_read_line(atkgpu_server *v, int sid, long long line, float *out, size_t cap);
+/* the line's consumer on the device: atkdsp_detect_channels over element
+ * `element` of line L (waits as read_line does), only the records returned,
+ * strongest first. floor_db NaN = the median of the 5-bin-smoothed line
+ * (signal_id's floor), computed on the device; *floor_used (may be NULL)
+ * gets the floor applied. ATKDSP_E_CAP when more than out_cap are found (out
+ * then holds the first out_cap in bin order). */
+ATKGPU_API ptrdiff_t atkgpu_server_detect(atkgpu_server *v, int sid, long long line, unsigned element,
+                                          float floor_db, float threshold_db, int gap_bins, int min_run,
+                                          atkdsp_channel *out, size_t out_cap, float *floor_used);
 
 /* ---- the filterbank in matrix-core form: S streams of one shape ----------------
  * The polyphase fold as a matrix product on the MFMA units, from raw 8-bit

@@ -7,7 +7,7 @@ CPU-baseline leg may import this module, and only as the checker. The product
 Each function restates the reference C (file:line given) in numpy:
   - byte/integer work and the display helpers are restated exactly in float32
     (bit-identical to the reference: unpack, decimate_max, db_to_pixels,
-    stitch_max);
+    stitch_max, median, detect_channels);
   - the filterbank, FFT and W1/W2 are restated in FP64, the "FP64 twin" the
     reference's own tests measure against (tests/test_channelizer.py:397-398);
   - the spectrum line restates the reference's mixed arithmetic: window and
@@ -277,6 +277,86 @@ def stitch_max(seg, seg_lo, seg_step, keep_lo, keep_hi, out_lo, out_step, out: n
         v, cur = seg[i], out[k]
         if cur != cur or v > cur:
             out[k] = v
+    return out
+
+
+# ---- detection on a spectrum line: src/detect.c:4-114 ---------------------------------------
+def median(x) -> float:
+    """src/detect.c:25-34: the middle order statistic, or for even n the
+    float32 0.5 * (lower middle + upper middle); -120 for an empty input."""
+    x = np.asarray(x, F32)
+    n = x.size
+    if n == 0:
+        return -120.0
+    s = np.sort(x, kind="stable")
+    if n & 1:
+        return float(s[n // 2])
+    return float(F32(0.5) * (s[n // 2 - 1] + s[n // 2]))
+
+
+def smooth5(x) -> np.ndarray:
+    """src/detect.c:38-50: the 5-bin moving average with edge padding, summed
+    from 0.0f in index order, then times 0.2f, all in float32."""
+    x = np.asarray(x, F32)
+    n = x.size
+    if n < 5:
+        return x.copy()
+    idx = np.arange(n)
+    s = np.zeros(n, F32)
+    for d in (-2, -1, 0, 1, 2):
+        s = s + x[np.clip(idx + d, 0, n - 1)]
+    return s * F32(0.2)
+
+
+def detect_channels(line, floor_db: float, threshold_db: float, gap_bins: int, min_run: int):
+    """src/detect.c:58-114: runs of the smoothed line above floor + threshold
+    (float32 sum), interior gaps shorter than gap_bins bridged, runs of at
+    least min_run kept; each run's peak (scan from its first bin), the FP64
+    power-weighted centroid summed in bin order with the C library's pow,
+    snr = peak - floor. Records strongest first by a stable sort (the
+    reference's qsort is glibc's merge sort here). Returns a list of
+    (start, end, centroid, peak, snr) with float32 fields."""
+    line = np.asarray(line, F32)
+    n = line.size
+    if n == 0:
+        return []
+    fl = F32(floor_db)
+    thr = fl + F32(threshold_db)
+    gap = max(2, int(gap_bins))
+    mr = max(2, int(min_run))
+    m = (smooth5(line) > thr).copy()
+    i = 0
+    while i < n:
+        if not m[i]:
+            j = i
+            while j < n and not m[j]:
+                j += 1
+            if i > 0 and j < n and j - i < gap:
+                m[i:j] = True
+            i = j
+        else:
+            i += 1
+    out = []
+    i = 0
+    while i < n:
+        if not m[i]:
+            i += 1
+            continue
+        j = i
+        while j < n and m[j]:
+            j += 1
+        if j - i >= mr:
+            wsum = csum = 0.0
+            peak = line[i]
+            for k in range(i, j):
+                w = math.pow(10.0, float(line[k]) / 10.0)
+                wsum += w
+                csum += float(k - i) * w
+                if line[k] > peak:
+                    peak = line[k]
+            out.append((i, j, float(F32(float(i) + csum / (wsum + 1e-12))), float(peak), float(F32(peak - fl))))
+        i = j
+    out.sort(key=lambda c: -c[4])
     return out
 
 

@@ -14,6 +14,11 @@ import numpy as np
 REF_SO = Path(__file__).resolve().parent / "_ref" / "atkdsp_ref.so"
 
 
+class ChanC(C.Structure):
+    _fields_ = [("start_bin", C.c_int), ("end_bin", C.c_int), ("centroid_bin", C.c_float),
+                ("peak_db", C.c_float), ("snr_db", C.c_float)]
+
+
 class Cf(C.Structure):
     _fields_ = [("re", C.c_float), ("im", C.c_float)]
 
@@ -70,6 +75,9 @@ class Ref:
             "atkdsp_osx_chan_rate": (d, [vp, C.c_int]),
             "atkdsp_osx_out_max": (sz, [vp, C.c_int, sz]),
             "atkdsp_osx_process_n": (C.c_ssize_t, [vp, P(cf), u, sz, P(cf), P(sz), sz]),
+            "atkdsp_median": (C.c_float, [fp, sz, fp]),
+            "atkdsp_detect_channels": (C.c_ssize_t, [fp, sz, C.c_float, C.c_float, C.c_int, C.c_int, fp,
+                                                     P(ChanC), sz]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -186,6 +194,22 @@ class Ref:
         seg = np.ascontiguousarray(seg, np.float32)
         self.L.atkdsp_stitch_max(self._f(seg), seg.size, a, b, c, d, e, f, self._f(out), out.size)
         return out
+
+    def median(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        scratch = np.empty(max(x.size, 1), np.float32)
+        return float(self.L.atkdsp_median(self._f(x), x.size, self._f(scratch)))
+
+    def detect_channels(self, line, floor_db, threshold_db, gap_bins, min_run, cap=4096):
+        """(records as (start, end, centroid, peak, snr) tuples, return code)"""
+        line = np.ascontiguousarray(line, np.float32)
+        scratch = np.empty(max(line.size, 1), np.float32)
+        buf = (ChanC * max(cap, 1))()
+        n = self.L.atkdsp_detect_channels(self._f(line), line.size, floor_db, threshold_db, gap_bins, min_run,
+                                          self._f(scratch), buf, cap)
+        kept = cap if n == -3 else max(n, 0)                 # E_CAP: the first cap found, bin order
+        recs = [(c.start_bin, c.end_bin, c.centroid_bin, c.peak_db, c.snr_db) for c in buf[:kept]]
+        return recs, int(n)
 
     def pfb(self, fs, nchan, os_, K, atten, max_nel=1):
         return RefPfb(self, fs, nchan, os_, K, atten, max_nel)

@@ -19,6 +19,7 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT))
+from oracle import oracle  # noqa: E402
 from oracle.ref import Ref, RefArb, RefOsx, design_lowpass  # noqa: E402
 from atkdsp_amd import synth  # noqa: E402
 
@@ -271,13 +272,73 @@ def osx_golden():
          osx_el_y0=ye[c0], osx_el_y1=ye[c1])
 
 
+def detect_lines(seed=41):
+    """Spectrum lines in dB for detect_channels and median: each is
+    (line, threshold_db, gap_bins, min_run, cap); the floor is the reference
+    median of the reference-smoothed line, as signal_id computes it."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    # a busy band (the reference's own test shape, tests/test_cross_check.py:366-383)
+    n = 16384
+    line = (-100.0 + 10 * np.log10(rng.exponential(1.0, n) + 1e-12)).astype(np.float32)
+    for c, w, lvl in [(2000, 12, -60), (2030, 6, -70), (9000, 80, -50), (15000, 3, -65), (16380, 4, -55)]:
+        line[c:c + w] = lvl + rng.standard_normal(min(w, n - c)).astype(np.float32)
+    cases.append((line, 6.0, 4, 2, 4096))
+    # noise only at 12 dB: nothing
+    cases.append(((-100 + 10 * np.log10(rng.exponential(1.0, 8192))).astype(np.float32), 12.0, 4, 3, 256))
+    # a display-sized line with plateaus of equal level (SNR ties), runs at both ends,
+    # gaps of 1..6 bins between plateaus, and gap / min_run below their minimum
+    line = np.full(4096, -110.0, np.float32)
+    pos = 0
+    for k in range(60):
+        w = 2 + (k * 7) % 13
+        line[pos:pos + w] = -70.0 if k % 3 else -60.0
+        pos += w + 1 + (k % 6)
+        if pos > 4000:
+            break
+    line[:5] = -65.0
+    line[-4:] = -65.0
+    cases.append((line, 10.0, 3, 2, 4096))
+    cases.append((line, 10.0, 0, 0, 4096))
+    # out_cap too small: ATKDSP_E_CAP with the first records in bin order
+    cases.append((line, 10.0, 3, 2, 5))
+    # fewer than five bins: no smoothing
+    cases.append((np.array([-50.0, -51.0, -120.0], np.float32), -1.5, 2, 2, 16))
+    return cases
+
+
+def detect_golden():
+    r = Ref()
+    lines, params, recs, nrec, rcs, floors = [], [], [], [], [], []
+    for line, thr, gap, mr, cap in detect_lines():
+        fl = r.median(oracle.smooth5(line))             # an input: the floor as signal_id makes it
+        got, rc = r.detect_channels(line, fl, thr, gap, mr, cap)
+        lines.append(line)
+        params.append((line.size, thr, gap, mr, cap))
+        floors.append(fl)
+        recs += got
+        nrec.append(len(got))
+        rcs.append(rc)
+    rng = np.random.default_rng(43)
+    med_x = [rng.standard_normal(n).astype(np.float32) for n in (5, 6, 1023, 1024, 131072)]
+    save("detect", d_line=np.concatenate(lines), d_params=np.array(params, np.float64),
+         d_floor=np.array(floors, np.float32), d_rc=np.array(rcs, np.int64), d_nrec=np.array(nrec),
+         d_rec=np.array([rc_[:2] for rc_ in recs], np.int64).reshape(-1, 2),
+         d_recf=np.array([rc_[2:] for rc_ in recs], np.float32).reshape(-1, 3),
+         m_len=np.array([x.size for x in med_x]), m_x=np.concatenate(med_x),
+         m_y=np.array([r.median(x) for x in med_x], np.float32))
+
+
 if __name__ == "__main__":
     import sys
     if sys.argv[1:] == ["nco"]:
         nco_golden()
     elif sys.argv[1:] == ["osx"]:
         osx_golden()
+    elif sys.argv[1:] == ["detect"]:
+        detect_golden()
     else:
         main()
         nco_golden()
         osx_golden()
+        detect_golden()

@@ -2,6 +2,7 @@
 every name of the reference ABI and every function include/atkdsp.h declares,
 and its stubs return the documented failure values. No compute calls."""
 import ctypes as C
+import math
 import re
 import subprocess
 
@@ -64,9 +65,9 @@ def test_stubs_return_failure_values():
     g = lib.atkdsp_fir_reset
     g.restype = None
     g()                                                 # void: nothing
-    h = lib.atkdsp_median
-    h.restype = C.c_float
-    assert h() != h()                                   # float: NaN
+    h = lib.atkdsp_ddc_out_rate
+    h.restype = C.c_double
+    assert h() != h()                                   # floating: NaN
     k = lib.atkdsp_cfar_process
     k.restype = C.c_ssize_t
     assert k() == -1                                    # signed count: ATKDSP_E_ARG
@@ -114,6 +115,16 @@ def test_no_device_fails_loudly():
         A.Osx(2.4e6, 4096)
     with pytest.raises(A.AtkDspError):
         A.ArbResampler(75e3, 50e3)
+    # detection runs on the device: no GPU is E_FFT / NaN ("unmeasured"), after
+    # the reference's own argument checks (a NaN floor is E_ARG, n = 0 is 0)
+    line = np.full(64, -100.0, np.float32)
+    with pytest.raises(A.AtkDspError):
+        A.detect_channels(line, -100.0, 6.0, 2, 2)
+    assert A.detect_channels(line[:0], -100.0, 6.0, 2, 2) == []
+    lib = A.load()
+    sc = np.empty(64, np.float32)
+    assert lib.atkdsp_detect_channels(A._fp(line), 64, float("nan"), 6.0, 2, 2, A._fp(sc), None, 0) == -1
+    assert math.isnan(A.median(line)) and A.median(line[:0]) == -120.0
 
 
 def test_reference_binding_would_load_it():

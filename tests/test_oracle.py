@@ -485,3 +485,81 @@ def test_osx_oracle_catches_planted_mistakes():
         y = o.process(x)[0][c]
         m = min(y.size, ys[2].size)
         assert O.rel_err(y[:m], ys[2][:m]) > 1e-2, plant
+
+
+# ---- detection on a spectrum line ----------------------------------------------------------
+def _detect_cases():
+    g = golden("detect")
+    p = g["d_params"]
+    lines = np.split(g["d_line"], np.cumsum(p[:, 0].astype(int))[:-1])
+    idx = np.split(np.arange(len(g["d_rec"])), np.cumsum(g["d_nrec"])[:-1])
+    out = []
+    for line, pp, fl, rc, ri in zip(lines, p, g["d_floor"], g["d_rc"], idx):
+        recs = [(int(a), int(b), float(c), float(d), float(e))
+                for (a, b), (c, d, e) in zip(g["d_rec"][ri], g["d_recf"][ri])]
+        out.append((line, float(fl), float(pp[1]), int(pp[2]), int(pp[3]), int(pp[4]), int(rc), recs))
+    return out
+
+
+def test_detect_channels_oracle_is_the_reference():
+    """every golden line (a busy band, noise only, SNR ties, runs at both
+    ends, gap / min_run below their minimum, fewer than five bins) gives the
+    reference's records exactly, in the reference's order; when out_cap is
+    short the reference returns E_CAP with the first out_cap in bin order"""
+    for line, fl, thr, gap, mr, cap, rc, recs in _detect_cases():
+        got = O.detect_channels(line, fl, thr, gap, mr)
+        if rc == -3:
+            assert len(got) > cap and sorted(got)[:cap] == recs
+        else:
+            assert got == recs and rc == len(recs)
+
+
+def test_median_oracle_is_the_reference_and_numpy():
+    g = golden("detect")
+    for x, y in zip(np.split(g["m_x"], np.cumsum(g["m_len"])[:-1]), g["m_y"]):
+        assert O.median(x) == y
+        assert O.median(x) == np.float32(np.median(x))      # the reference's own check, exact here
+    assert O.median([3.0, 1.0, 2.0]) == 2.0 and O.median([4.0, 1.0, 3.0, 2.0]) == 2.5
+    assert O.median([]) == -120.0
+
+
+def _runs_brute(line, floor, thr, gap, mr):
+    """an independent statement on a short line: regular expressions over the
+    0/1 mask string (interior zero runs shorter than gap become ones, then
+    runs of ones at least min_run wide)"""
+    import re
+    sm = O.smooth5(line)
+    mask = "".join("1" if v > np.float32(floor) + np.float32(thr) else "0" for v in sm)
+    gap, mr = max(gap, 2), max(mr, 2)
+    mask = re.sub(r"(?<=1)0+(?=1)", lambda m: "1" * len(m.group()) if len(m.group()) < gap else m.group(), mask)
+    return [(m.start(), m.end()) for m in re.finditer(r"1+", mask) if m.end() - m.start() >= mr]
+
+
+def test_detect_channels_oracle_runs_match_a_brute_force_on_random_masks():
+    rng = np.random.default_rng(5)
+    for trial in range(300):
+        n = int(rng.integers(1, 60))
+        line = rng.choice(np.array([-100.0, -90.0, -40.0, -30.0], np.float32), n)
+        gap, mr = int(rng.integers(0, 6)), int(rng.integers(0, 5))
+        got = sorted((c[0], c[1]) for c in O.detect_channels(line, -100.0, 20.0, gap, mr))
+        assert got == _runs_brute(line, -100.0, 20.0, gap, mr), (trial, line, gap, mr)
+
+
+def test_detect_channels_oracle_record_fields_closed_form():
+    """a flat run: peak = the level, snr = level - floor, centroid = the
+    run's middle (equal weights); a run with one loud bin: the centroid sits
+    at that bin to within its power share"""
+    line = np.full(64, -120.0, np.float32)
+    line[10:20] = -50.0
+    # threshold 60 dB over a -120 floor: the smoothed ramps (-78 and -64 dB at
+    # the plateau's first two and last two bins) fall below it, so the run is
+    # the plateau less two bins each side
+    (c,) = O.detect_channels(line, -120.0, 60.0, 2, 2)
+    assert c[:2] == (12, 18) and c[3] == -50.0 and c[4] == np.float32(70.0)
+    assert abs(c[2] - 14.5) < 1e-4
+    # at 10 dB the 5-bin smoothing widens the run by two bins each side
+    assert O.detect_channels(line, -120.0, 10.0, 2, 2)[0][:2] == (8, 22)
+    line[15] = -20.0
+    (c,) = O.detect_channels(line, -120.0, 60.0, 2, 2)
+    # weights 1e-5 at five bins, 1e-2 at bin 15: 12 + 0.03012 / 0.01005
+    assert abs(c[2] - 14.99701) < 1e-4 and c[3] == -20.0
