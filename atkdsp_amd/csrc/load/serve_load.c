@@ -35,6 +35,12 @@
  * blocks lost (a producer that lapped its consumer), the delay from commit to
  * the call that took the block, and the time a submit_ring call took.
  *
+ * With --detect 1 a consumer thread runs atkgpu_server_detect on every display
+ * line as it completes (each stream's lines in turn; the floor is the median
+ * of the smoothed line, found on the device; threshold 6 dB, gap 4, min run
+ * 2), as a signal-finding consumer of the lines would; the JSON adds the
+ * lines it covered, the channels found and the call latency.
+ *
  * Before the clock starts every stream gets one chunk of CHUNK_MAX samples and
  * the server is drained and its statistics reset: the figures are the steady
  * state, without the first launch of each kernel (code object loading) or the
@@ -43,6 +49,7 @@
  * usage: serve_load [--seconds S] [--streams N] [--speed X] [--budget-gib G]
  *                   [--max-batch B] [--seed K] [--mix c4|uniform] [--arrays A]
  *                   [--display 0|1] [--ring 0|1] [--consumers C] [--ring-blocks B]
+ *                   [--detect 0|1]
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -78,7 +85,10 @@ static atkgpu_server *g_srv;
 static lstream *g_st;
 static int g_ns;
 static double g_t0, g_seconds, g_speed;
-static int g_ring, g_consumers = 4, g_ring_blocks = 64;
+static int g_ring, g_consumers = 4, g_ring_blocks = 64, g_detect;
+static long long g_det_lines, g_det_chans, g_det_missed;
+static double *g_det_lat;
+static size_t g_det_n, g_det_cap;
 static volatile int g_producers_done;
 static pthread_mutex_t g_pick_mu = PTHREAD_MUTEX_INITIALIZER;
 static long long g_picked, g_lost;
@@ -268,6 +278,48 @@ static void *ring_consumer(void *arg) {
     return NULL;
 }
 
+static int cmp_dbl(const void *a, const void *b) {
+    const double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+
+/* --detect: every completed line of every display stream, in turn */
+static void *line_detector(void *arg) {
+    (void)arg;
+    long long *next = (long long *)calloc((size_t)g_ns, sizeof(long long));
+    atkdsp_channel ch[512];
+    for (;;) {
+        const int done = g_producers_done;
+        int took = 0;
+        for (int i = 0; i < g_ns; ++i) {
+            const long long nl = atkgpu_server_lines(g_srv, g_st[i].sid);
+            if (nl <= 0) continue;
+            if (nl - next[i] > 3) {                   /* fell behind the line ring: skip to the newest */
+                g_det_missed += nl - 1 - next[i];
+                next[i] = nl - 1;
+            }
+            for (; next[i] < nl; ++next[i]) {
+                float fl;
+                const double t0 = now_s();
+                const ptrdiff_t n = atkgpu_server_detect(g_srv, g_st[i].sid, next[i], 0, NAN, 6.0f, 4, 2, ch, 512, &fl);
+                const double t1 = now_s();
+                if (n < 0 && n != ATKDSP_E_CAP) continue;
+                if (g_det_n < g_det_cap) g_det_lat[g_det_n++] = (t1 - t0) * 1e3;
+                g_det_lines++;
+                g_det_chans += n < 0 ? 512 : n;
+                took = 1;
+            }
+        }
+        if (!took) {
+            if (done) break;
+            struct timespec ts = {0, 200000L};
+            nanosleep(&ts, NULL);
+        }
+    }
+    free(next);
+    return NULL;
+}
+
 int main(int argc, char **argv) {
     double seconds = 5.0, speed = 1.0, budget_gib = 32.0;
     int streams = 128, max_batch = 256;
@@ -286,6 +338,7 @@ int main(int argc, char **argv) {
         else if (!strcmp(argv[i], "--ring")) g_ring = atoi(argv[i + 1]);
         else if (!strcmp(argv[i], "--consumers")) g_consumers = atoi(argv[i + 1]);
         else if (!strcmp(argv[i], "--ring-blocks")) g_ring_blocks = atoi(argv[i + 1]);
+        else if (!strcmp(argv[i], "--detect")) g_detect = atoi(argv[i + 1]);
     }
     if (g_consumers < 1 || g_consumers > NTHREADS) { fprintf(stderr, "serve_load: --consumers 1..%d\n", NTHREADS); return 2; }
     g_srv = atkgpu_server_create((size_t)(budget_gib * (1u << 30)), (unsigned)(streams + arrays),
@@ -369,7 +422,11 @@ int main(int argc, char **argv) {
         atkgpu_serve_stats w;
         atkgpu_server_stats(g_srv, &w, 1);
     }
-    pthread_t th[NTHREADS], rc[NTHREADS];
+    pthread_t th[NTHREADS], rc[NTHREADS], dth;
+    if (g_detect && display) {
+        g_det_cap = 1u << 22;
+        g_det_lat = (double *)malloc(g_det_cap * sizeof(double));
+    }
     struct rusage ru0, ru1;
     getrusage(RUSAGE_SELF, &ru0);
     const double w0 = now_s();
@@ -377,17 +434,26 @@ int main(int argc, char **argv) {
     for (long t = 0; t < NTHREADS; ++t) pthread_create(&th[t], NULL, submitter, (void *)t);
     if (g_ring)
         for (long t = 0; t < g_consumers; ++t) pthread_create(&rc[t], NULL, ring_consumer, (void *)t);
+    if (g_det_lat) pthread_create(&dth, NULL, line_detector, NULL);
     for (int t = 0; t < NTHREADS; ++t) pthread_join(th[t], NULL);
     g_producers_done = 1;
     if (g_ring)
         for (int t = 0; t < g_consumers; ++t) pthread_join(rc[t], NULL);
     atkgpu_server_drain(g_srv);
+    if (g_det_lat) pthread_join(dth, NULL);
     getrusage(RUSAGE_SELF, &ru1);
     const double cpu_s = (double)(ru1.ru_utime.tv_sec - ru0.ru_utime.tv_sec) +
                          1e-6 * (double)(ru1.ru_utime.tv_usec - ru0.ru_utime.tv_usec) +
                          (double)(ru1.ru_stime.tv_sec - ru0.ru_stime.tv_sec) +
                          1e-6 * (double)(ru1.ru_stime.tv_usec - ru0.ru_stime.tv_usec);
     const double cpus = cpu_s / (now_s() - w0);
+    double dp50 = 0, dp99 = 0, dmax = 0;
+    if (g_det_n) {
+        qsort(g_det_lat, g_det_n, sizeof(double), cmp_dbl);
+        dp50 = g_det_lat[(size_t)(0.50 * (double)(g_det_n - 1))];
+        dp99 = g_det_lat[(size_t)(0.99 * (double)(g_det_n - 1))];
+        dmax = g_det_lat[g_det_n - 1];
+    }
     atkgpu_serve_stats st;
     atkgpu_server_stats(g_srv, &st, 0);
     long long lines = 0;
@@ -401,17 +467,20 @@ int main(int argc, char **argv) {
            "\"hbm_used_gib\": %.3f, \"hbm_budget_gib\": %.1f, \"display\": %d, \"lines\": %lld, "
            "\"ring\": %d, \"ring_lost\": %lld, \"ring_pickup_mean_ms\": %.4f, \"ring_pickup_max_ms\": %.4f, "
            "\"ring_call_mean_us\": %.2f, \"ring_call_max_ms\": %.4f, \"consumers\": %d, \"ring_blocks\": %d, \"ring_pass_max_ms\": %.3f, \"arrival_late_max_ms\": %.3f, "
-           "\"host_cpus_busy\": %.2f}\n",
+           "\"detect\": %d, \"detect_lines\": %lld, \"detect_missed\": %lld, \"detect_channels\": %lld, "
+           "\"detect_p50_ms\": %.4f, \"detect_p99_ms\": %.4f, \"detect_max_ms\": %.4f, \"host_cpus_busy\": %.2f}\n",
            uniform ? "uniform" : "c4", arrays, admitted, refused, total_rate / 1e6, speed, seconds, st.chunks, st.samples, st.batches,
            st.busy_s > 0 ? st.samples / st.busy_s / 1e6 : 0.0, st.p50_ms, st.p99_ms, st.p999_ms, st.max_ms,
            st.mean_ms, st.hbm_used / 1073741824.0, st.hbm_budget / 1073741824.0, display, lines, g_ring, g_lost,
            g_picked ? 1e3 * g_pick_sum / (double)g_picked : 0.0, 1e3 * g_pick_max,
-           g_picked ? 1e6 * g_call_sum / (double)g_picked : 0.0, 1e3 * g_call_max, g_ring ? g_consumers : 0, g_ring ? g_ring_blocks : 0, 1e3 * g_pass_max, 1e3 * g_late_max, cpus);
+           g_picked ? 1e6 * g_call_sum / (double)g_picked : 0.0, 1e3 * g_call_max, g_ring ? g_consumers : 0, g_ring ? g_ring_blocks : 0, 1e3 * g_pass_max, 1e3 * g_late_max,
+           g_det_lat != NULL, g_det_lines, g_det_missed, g_det_chans, dp50, dp99, dmax, cpus);
     atkgpu_server_destroy(g_srv);
     for (int s = 0; s < admitted; ++s) {
         free(g_st[s].src);
         free(g_st[s].ring_mem);
     }
+    free(g_det_lat);
     free(g_st);
     return 0;
 }
