@@ -2,8 +2,9 @@
 // the median (a robust noise floor) and detect_channels, for lines in device
 // memory (the served display lines) and for the drop-in.
 //
-// One block of DT threads per line, the line in global memory (it is L2
-// resident at display sizes), a float scratch of n per line:
+// One block of DT threads per line; a line of up to 4096 bins and its mask
+// are held in LDS (longer ones use a float scratch of n per line in global
+// memory):
 //   smooth   the 5-bin moving average with edge padding, summed in the
 //            reference's order (x[i-2] + ... + x[i+2], from 0.0f) times 0.2f;
 //   floor    optionally the median of the smoothed line (signal_id's floor),
@@ -14,7 +15,8 @@
 //            it only ever turns a run's own zeros to ones, so each run's
 //            fate depends on the original mask alone: here each thread owns
 //            a contiguous segment and learns the nearest one on either side
-//            of it from two block scans;
+//            of it from two block scans (wavefront shuffles, then the
+//            sixteen wave totals);
 //   runs     the ones runs at least min_run wide, numbered in bin order by a
 //            block scan; the thread that owns a run's first bin writes its
 //            record: peak (the reference's scan from line[start]), the
@@ -46,10 +48,79 @@ struct SelectLds {
     uint32_t prefix;
 };
 
+// inclusive scans within a wavefront (64 lanes)
+template <class T, class Op>
+__device__ __forceinline__ T wave_scan_up(T v, Op op) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const T w = __shfl_up(v, o, 64);
+        if (lane >= o) v = op(v, w);
+    }
+    return v;
+}
+template <class T, class Op>
+__device__ __forceinline__ T wave_scan_down(T v, Op op) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const T w = __shfl_down(v, o, 64);
+        if (lane + o < 64) v = op(v, w);
+    }
+    return v;
+}
+
+// exclusive block scans over the DT threads' values, in thread order
+// (prefix) or reverse thread order (suffix); `part` is 16 slots of LDS; the
+// caller must not reuse `part` before a barrier
+constexpr int NW = DT / 64;
+template <class T, class Op>
+__device__ T block_excl_prefix(T v, T ident, Op op, T *part) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const T incl = wave_scan_up(v, op);
+    T ex = __shfl_up(incl, 1, 64);
+    if (lane == 0) ex = ident;
+    if (lane == 63) part[w] = incl;
+    __syncthreads();
+    if (w == 0) {
+        T t = lane < NW ? part[lane] : ident;
+        t = wave_scan_up(t, op);
+        if (lane < NW) part[lane] = t;
+    }
+    __syncthreads();
+    return w > 0 ? op(part[w - 1], ex) : ex;
+}
+template <class T, class Op>
+__device__ T block_excl_suffix(T v, T ident, Op op, T *part) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const T incl = wave_scan_down(v, op);
+    T ex = __shfl_down(incl, 1, 64);
+    if (lane == 63) ex = ident;
+    if (lane == 0) part[w] = incl;
+    __syncthreads();
+    if (w == 0) {
+        T t = lane < NW ? part[lane] : ident;
+        // suffix over the NW wave totals
+#pragma unroll
+        for (int o = 1; o < NW; o <<= 1) {
+            const T u = __shfl_down(t, o, 64);
+            if (lane + o < NW) t = op(t, u);
+        }
+        if (lane < NW) part[lane] = t;
+    }
+    __syncthreads();
+    return w + 1 < NW ? op(part[w + 1], ex) : ex;
+}
+
+struct OpMax { __device__ long long operator()(long long a, long long b) const { return a > b ? a : b; } };
+struct OpMin { __device__ long long operator()(long long a, long long b) const { return a < b ? a : b; } };
+struct OpAdd { __device__ unsigned long long operator()(unsigned long long a, unsigned long long b) const { return a + b; } };
+
 // the k-th smallest (0-based) of x[0..n): four 8-bit passes over the order
-// key, most significant first; every thread of the block calls it and gets
-// the value
-__device__ float block_select(const float *x, size_t n, size_t k, SelectLds &s) {
+// key, most significant first; one wavefront finds each pass's digit from the
+// 256-bin histogram (a scan of 4-bin lane sums and a ballot); every thread
+// of the block calls it and gets the value
+__device__ __forceinline__ float block_select(const float *x, size_t n, size_t k, SelectLds &s) {
     uint32_t prefix = 0, pmask = 0;
     if (threadIdx.x == 0) s.k = k;
     for (int shift = 24; shift >= 0; shift -= 8) {
@@ -60,69 +131,44 @@ __device__ float block_select(const float *x, size_t n, size_t k, SelectLds &s) 
             if ((key & pmask) == prefix) atomicAdd(&s.hist[(key >> shift) & 255u], 1u);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long kk = s.k, c = 0;
-            uint32_t b = 0;
-            for (; b < 255; ++b) {
-                if (c + s.hist[b] > kk) break;
-                c += s.hist[b];
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            const uint32_t c0 = s.hist[4 * lane], c1 = s.hist[4 * lane + 1], c2 = s.hist[4 * lane + 2],
+                           c3 = s.hist[4 * lane + 3];
+            const unsigned long long mine = (unsigned long long)c0 + c1 + c2 + c3;
+            const unsigned long long incl = wave_scan_up(mine, OpAdd());
+            const unsigned long long kk = s.k;
+            const unsigned long long hit = __ballot(incl > kk);
+            const int L = __ffsll((long long)hit) - 1;
+            if (lane == L) {
+                unsigned long long rem = kk - (incl - mine);
+                uint32_t b = 4u * (uint32_t)lane;
+                if (rem >= c0) {
+                    rem -= c0; ++b;
+                    if (rem >= c1) {
+                        rem -= c1; ++b;
+                        if (rem >= c2) { rem -= c2; ++b; }
+                    }
+                }
+                s.k = rem;
+                s.prefix = prefix | (b << shift);
             }
-            s.k = kk - c;
-            s.prefix = prefix | (b << shift);
         }
         __syncthreads();
         prefix = s.prefix;
         pmask |= 255u << shift;
-        __syncthreads();
     }
+    __syncthreads();
     return key_float(prefix);
 }
 
 // the median as the reference defines it: the middle value, or for even n
 // 0.5f * (lower middle + upper middle)
-__device__ float block_median(const float *x, size_t n, SelectLds &s) {
+__device__ __forceinline__ float block_median(const float *x, size_t n, SelectLds &s) {
     if (n & 1) return block_select(x, n, n / 2, s);
     const float hi = block_select(x, n, n / 2, s);
     const float lo = block_select(x, n, n / 2 - 1, s);
     return 0.5f * (lo + hi);
-}
-
-// inclusive scans over the block's DT values (Hillis-Steele in LDS)
-__device__ long long scan_max(long long v, long long *sh) {
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < DT; o <<= 1) {
-        const long long w = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : -1;
-        __syncthreads();
-        if (w > v) v = w;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-    }
-    return v;
-}
-__device__ long long scan_min_rev(long long v, long long *sh) {
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < DT; o <<= 1) {
-        const long long w = threadIdx.x + o < (unsigned)DT ? sh[threadIdx.x + o] : 0x7fffffffffffffffLL;
-        __syncthreads();
-        if (w < v) v = w;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-    }
-    return v;
-}
-__device__ long long scan_sum(long long v, long long *sh) {
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < DT; o <<= 1) {
-        const long long w = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0;
-        __syncthreads();
-        v += w;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-    }
-    return v;
 }
 
 // run [s, e) of the bridged mask starting in this thread's segment
@@ -132,18 +178,36 @@ __device__ __forceinline__ size_t run_end(const float *m, size_t s, size_t n) {
     return e;
 }
 
+// lines up to this many bins (every display line) are held in LDS, line and
+// mask, 32 KiB in all; the instantiation for them indexes the LDS arrays by
+// name, so no access goes through a generic (flat) pointer into LDS
+constexpr int LDS_BINS = 4096;
+
+template <bool IN_LDS>
 __global__ __launch_bounds__(DT) void k_detect(const float *__restrict__ lines, size_t lstride, size_t n,
                                                float floor_db, int compute_floor, float threshold_db, int gap,
                                                int min_run, float *__restrict__ scratch,
                                                atkgpu_chan *__restrict__ out, size_t cap,
                                                unsigned long long *__restrict__ found, float *__restrict__ floors) {
     __shared__ SelectLds sel;
-    __shared__ long long sh[DT];
+    __shared__ long long part[NW];
+    __shared__ unsigned long long upart[NW];
     __shared__ float s_floor;
-    const float *x = lines + blockIdx.x * lstride;
-    float *sm = scratch + blockIdx.x * n;
+    __shared__ float x_lds[IN_LDS ? LDS_BINS : 1], m_lds[IN_LDS ? LDS_BINS : 1];
+    const float *xg = lines + blockIdx.x * lstride;
+    const float *x;
+    float *sm;
+    if constexpr (IN_LDS) {
+        for (size_t i = threadIdx.x; i < n; i += DT) x_lds[i] = xg[i];
+        x = x_lds;
+        sm = m_lds;
+    } else {
+        x = xg;
+        sm = scratch + blockIdx.x * n;
+    }
     atkgpu_chan *o = out + blockIdx.x * cap;
     const int t = threadIdx.x;
+    __syncthreads();
 
     for (size_t i = t; i < n; i += DT) {
         float v;
@@ -185,12 +249,9 @@ __global__ __launch_bounds__(DT) void k_detect(const float *__restrict__ lines, 
             last = (long long)i;
         }
     // the last one strictly left of this segment, the first strictly right
-    scan_max(last, sh);
-    const long long left = t > 0 ? sh[t - 1] : -1;
+    const long long left = block_excl_prefix(last, -1LL, OpMax(), part);
     __syncthreads();
-    scan_min_rev(first, sh);
-    const long long right = t + 1 < DT ? sh[t + 1] : (long long)n;
-    __syncthreads();
+    const long long right = block_excl_suffix(first, (long long)n, OpMin(), part);
     {
         long long p = left;
         size_t i = a0;
@@ -207,22 +268,20 @@ __global__ __launch_bounds__(DT) void k_detect(const float *__restrict__ lines, 
     __syncthreads();
 
     // runs: count this segment's qualifying starts, number them, write
-    long long cnt = 0;
+    unsigned long long cnt = 0;
     for (size_t i = a0; i < a1;) {
         if (sm[i] == 0.0f || (i > 0 && sm[i - 1] != 0.0f)) { ++i; continue; }
         const size_t e = run_end(sm, i, n);
         if ((long long)(e - i) >= min_run) ++cnt;
         i = e;
     }
-    const long long incl = scan_sum(cnt, sh);
-    const long long total = sh[DT - 1];
-    long long idx = incl - cnt;
-    __syncthreads();
+    unsigned long long idx = block_excl_prefix(cnt, 0ULL, OpAdd(), upart);
+    const unsigned long long total = upart[NW - 1];
     for (size_t i = a0; i < a1;) {
         if (sm[i] == 0.0f || (i > 0 && sm[i - 1] != 0.0f)) { ++i; continue; }
         const size_t e = run_end(sm, i, n);
         if ((long long)(e - i) >= min_run) {
-            if ((size_t)idx < cap) {
+            if (idx < cap) {
                 double wsum = 0.0, csum = 0.0;
                 float peak = x[i];
                 for (size_t k = i; k < e; ++k) {
@@ -244,7 +303,7 @@ __global__ __launch_bounds__(DT) void k_detect(const float *__restrict__ lines, 
         i = e;
     }
     if (t == 0) {
-        found[blockIdx.x] = (unsigned long long)total;
+        found[blockIdx.x] = total;
         if (floors) floors[blockIdx.x] = fl;
     }
 }
@@ -262,8 +321,12 @@ extern "C" int atkgpu_detect_lines(void *s, const float *lines, size_t lstride, 
                                    float *scratch, atkgpu_chan *out, size_t cap, unsigned long long *found,
                                    float *floors) {
     if (!lines || !scratch || !found || n == 0 || nlines == 0 || (!out && cap)) return -1;
-    hipLaunchKernelGGL(k_detect, dim3(nlines), dim3(DT), 0, (hipStream_t)s, lines, lstride, n, floor_db,
-                       compute_floor, threshold_db, gap, min_run, scratch, out, cap, found, floors);
+    if (n <= (size_t)LDS_BINS)
+        hipLaunchKernelGGL(k_detect<true>, dim3(nlines), dim3(DT), 0, (hipStream_t)s, lines, lstride, n, floor_db,
+                           compute_floor, threshold_db, gap, min_run, scratch, out, cap, found, floors);
+    else
+        hipLaunchKernelGGL(k_detect<false>, dim3(nlines), dim3(DT), 0, (hipStream_t)s, lines, lstride, n,
+                           floor_db, compute_floor, threshold_db, gap, min_run, scratch, out, cap, found, floors);
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 

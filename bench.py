@@ -273,15 +273,18 @@ def cpu_baseline(seconds: float, base: np.ndarray):
                       f"carried tail), decimate_max, stitch_max: {el:.1f} s wall on {cores} threads"}
 
 
-def serving(mix: str, streams: int, seconds: float):
+def serving(mix: str, streams: int, seconds: float, extra=()):
     """The serving loop under Poisson chunk arrivals at the streams' real rates
-    (the native load generator beside the library), as one JSON object."""
+    (the native load generator beside the library), as one JSON object.
+    extra: more serve_load options (--detect 1: a consumer finding channels on
+    every display line on the device; --ring 1: chunks fed from per-stream
+    block rings)."""
     import subprocess
     exe = ROOT / "atkdsp_amd" / "serve_load"
     if not exe.exists():
         return {"error": "atkdsp_amd/serve_load not built"}
     print(f"[bench] serving {mix} x{streams} for {seconds:.0f} s", file=sys.stderr, flush=True)
-    r = subprocess.run([str(exe), "--mix", mix, "--streams", str(streams), "--seconds", str(seconds)],
+    r = subprocess.run([str(exe), "--mix", mix, "--streams", str(streams), "--seconds", str(seconds), *extra],
                        capture_output=True, text=True, timeout=seconds + 120)
     if r.returncode != 0:
         return {"error": f"serve_load exited {r.returncode}: {r.stderr.strip()[-300:]}"}
@@ -360,8 +363,9 @@ def main() -> int:
     del ch
     serve = {}
     if args.serve_seconds > 0:
-        serve["uniform"] = serving("uniform", S, args.serve_seconds)
+        serve["uniform"] = serving("uniform", S, args.serve_seconds, ("--detect", "1"))
         serve["c4"] = serving("c4", 128, args.serve_seconds)
+        serve["uniform_from_rings"] = serving("uniform", S, args.serve_seconds, ("--ring", "1"))
     extras = {}
     if not args.no_extras:
         extras["precision_form"] = precision_forms()
@@ -393,7 +397,8 @@ def main() -> int:
                          "4096-point display lines (hop 2048, Hann, max-hold over 50 ms, 1024 columns), chunks "
                          "of 1..65536 samples arriving as Poisson processes at the real rate; latency = host "
                          "submit call to the device completion of the chunk's batch (its filterbank output "
-                         "and every display line it completes), host monotonic clock"),
+                         "and every display line it completes), host monotonic clock; a consumer finds the "
+                         "channels of every completed line on the device meanwhile (serving.uniform.detect_*)"),
         "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                             "p99": round(float(np.percentile(lat, 99)), 3),
                             "note": "one chained step (a 50 ms block of every stream) submitted to completed"},
