@@ -99,6 +99,10 @@ typedef struct {
 int atkgpu_detect_lines(void *s, const float *lines, size_t lstride, size_t n, unsigned nlines, float floor_db,
                         int compute_floor, float threshold_db, int gap, int min_run, float *scratch,
                         atkgpu_chan *out, size_t cap, unsigned long long *found, float *floors);
+/* nlines lines of n <= 4096 bins at ptrs[l] (a device-visible table), each
+ * with its floor computed (the median of its smoothed line): one launch */
+int atkgpu_detect_line_set(void *s, const float *const *ptrs, size_t n, unsigned nlines, float threshold_db, int gap,
+                           int min_run, atkgpu_chan *out, size_t cap, unsigned long long *found, float *floors);
 /* the median of x[0..n) into *out (device) */
 int atkgpu_median(void *s, const float *x, size_t n, float *out);
 int atkgpu_decimate_max(void *s, const float *in, size_t n, float *out, size_t m);

@@ -184,7 +184,8 @@ __device__ __forceinline__ size_t run_end(const float *m, size_t s, size_t n) {
 constexpr int LDS_BINS = 4096;
 
 template <bool IN_LDS>
-__global__ __launch_bounds__(DT) void k_detect(const float *__restrict__ lines, size_t lstride, size_t n,
+__global__ __launch_bounds__(DT) void k_detect(const float *__restrict__ lines, const float *const *__restrict__ ptrs,
+                                               size_t lstride, size_t n,
                                                float floor_db, int compute_floor, float threshold_db, int gap,
                                                int min_run, float *__restrict__ scratch,
                                                atkgpu_chan *__restrict__ out, size_t cap,
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(DT) void k_detect(const float *__restrict__ lines, 
     __shared__ unsigned long long upart[NW];
     __shared__ float s_floor;
     __shared__ float x_lds[IN_LDS ? LDS_BINS : 1], m_lds[IN_LDS ? LDS_BINS : 1];
-    const float *xg = lines + blockIdx.x * lstride;
+    const float *xg = ptrs ? ptrs[blockIdx.x] : lines + blockIdx.x * lstride;
     const float *x;
     float *sm;
     if constexpr (IN_LDS) {
@@ -316,17 +317,34 @@ __global__ __launch_bounds__(DT) void k_median(const float *__restrict__ x, size
 
 }  // namespace
 
+static int detect_launch(void *s, const float *lines, const float *const *ptrs, size_t lstride, size_t n,
+                         unsigned nlines, float floor_db, int compute_floor, float threshold_db, int gap, int min_run,
+                         float *scratch, atkgpu_chan *out, size_t cap, unsigned long long *found, float *floors) {
+    if (n <= (size_t)LDS_BINS)
+        hipLaunchKernelGGL(k_detect<true>, dim3(nlines), dim3(DT), 0, (hipStream_t)s, lines, ptrs, lstride, n,
+                           floor_db, compute_floor, threshold_db, gap, min_run, scratch, out, cap, found, floors);
+    else
+        hipLaunchKernelGGL(k_detect<false>, dim3(nlines), dim3(DT), 0, (hipStream_t)s, lines, ptrs, lstride, n,
+                           floor_db, compute_floor, threshold_db, gap, min_run, scratch, out, cap, found, floors);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
 extern "C" int atkgpu_detect_lines(void *s, const float *lines, size_t lstride, size_t n, unsigned nlines,
                                    float floor_db, int compute_floor, float threshold_db, int gap, int min_run,
                                    float *scratch, atkgpu_chan *out, size_t cap, unsigned long long *found,
                                    float *floors) {
     if (!lines || !scratch || !found || n == 0 || nlines == 0 || (!out && cap)) return -1;
-    if (n <= (size_t)LDS_BINS)
-        hipLaunchKernelGGL(k_detect<true>, dim3(nlines), dim3(DT), 0, (hipStream_t)s, lines, lstride, n, floor_db,
-                           compute_floor, threshold_db, gap, min_run, scratch, out, cap, found, floors);
-    else
-        hipLaunchKernelGGL(k_detect<false>, dim3(nlines), dim3(DT), 0, (hipStream_t)s, lines, lstride, n,
-                           floor_db, compute_floor, threshold_db, gap, min_run, scratch, out, cap, found, floors);
+    return detect_launch(s, lines, nullptr, lstride, n, nlines, floor_db, compute_floor, threshold_db, gap, min_run,
+                         scratch, out, cap, found, floors);
+}
+
+extern "C" int atkgpu_detect_line_set(void *s, const float *const *ptrs, size_t n, unsigned nlines,
+                                      float threshold_db, int gap, int min_run, atkgpu_chan *out, size_t cap,
+                                      unsigned long long *found, float *floors) {
+    // held in LDS only: every line of the set is at most LDS_BINS bins
+    if (!ptrs || !found || n == 0 || n > (size_t)LDS_BINS || nlines == 0 || (!out && cap)) return -1;
+    hipLaunchKernelGGL(k_detect<true>, dim3(nlines), dim3(DT), 0, (hipStream_t)s, nullptr, ptrs, 0, n, 0.0f, 1,
+                       threshold_db, gap, min_run, nullptr, out, cap, found, floors);
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 

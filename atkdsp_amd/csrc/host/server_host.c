@@ -88,6 +88,7 @@ int atkgpu_pfb_prototype(double *h, unsigned nchan, unsigned K, double atten_db)
 #define UBLK_CAP 16384         /* ingest blocks per batch */
 #define PBLK_CAP 16384         /* filterbank blocks per batch */
 #define QREC_CAP 1088          /* channel records of one detect query (>= 4096 / 4 + 1) */
+#define QSET_CAP 256           /* channel records per line of a detect_latest query */
 #define DISP_COLS_MAX 4096
 
 typedef struct {
@@ -185,6 +186,7 @@ struct atkgpu_server {
     void *qstream;
     float *q_scratch;                            /* device, DISP_COLS_MAX floats */
     void *q_h, *q_d;                             /* mapped: records, count, floor */
+    void *qs_h, *qs_d;                           /* mapped, per stream: line pointers, records, counts, floors */
     atkgpu_fftplan *plans[NPLANS];               /* twiddles for 64..4096 points */
     batch_t batch[NINFLIGHT];
     unsigned b_submit, b_done;                   /* batches launched / completed */
@@ -239,6 +241,12 @@ static int disp_nseg(long long fa, int nf, unsigned F) {
 static void *disp_main(void *arg);
 static void *comp_main(void *arg);
 
+/* the detect_latest area: [n] line pointers, [n][QSET_CAP] records, [n] counts, [n] floors */
+static size_t qset_bytes(unsigned n) {
+    return (size_t)n * (sizeof(void *) + QSET_CAP * sizeof(atkgpu_chan) + sizeof(unsigned long long) + sizeof(float)) +
+           64;
+}
+
 atkgpu_server *atkgpu_server_create(size_t hbm_budget, unsigned max_streams, unsigned max_batch, size_t staging) {
     if (max_streams < 1 || max_batch < 1 || staging < ((size_t)1 << 20)) return NULL;
     if (atkgpu_init() != 0) return NULL;
@@ -259,7 +267,8 @@ atkgpu_server *atkgpu_server_create(size_t hbm_budget, unsigned max_streams, uns
              !atkgpu_stream_create(&v->ustream) && !atkgpu_stream_create(&v->dstream) &&
              !atkgpu_stream_create(&v->qstream) &&
              (v->q_scratch = (float *)atkgpu_dmalloc(DISP_COLS_MAX * sizeof(float))) &&
-             (v->q_h = atkgpu_hmalloc_mapped(QREC_CAP * sizeof(atkgpu_chan) + 64, &v->q_d));
+             (v->q_h = atkgpu_hmalloc_mapped(QREC_CAP * sizeof(atkgpu_chan) + 64, &v->q_d)) &&
+             (v->qs_h = atkgpu_hmalloc_mapped(qset_bytes(max_streams), &v->qs_d));
     for (int i = 0; ok && i < NPLANS; ++i) ok = (v->plans[i] = atkgpu_fftplan_create(64u << i)) != NULL;
     for (int i = 0; ok && i < NINFLIGHT; ++i) {
         batch_t *b = &v->batch[i];
@@ -337,6 +346,7 @@ void atkgpu_server_destroy(atkgpu_server *v) {
     if (v->qstream) atkgpu_stream_destroy(v->qstream);
     atkgpu_dfree(v->q_scratch);
     atkgpu_hfree(v->q_h);
+    atkgpu_hfree(v->qs_h);
     atkgpu_hfree(v->stage_h);
     free(v->recs);
     free(v->lat);
@@ -1091,6 +1101,72 @@ ptrdiff_t atkgpu_server_detect(atkgpu_server *v, int sid, long long line, unsign
     if (found > out_cap) return ATKDSP_E_CAP;
     atk_sort_channels(out, kept);
     return (ptrdiff_t)kept;
+}
+
+/* the newest completed line (element 0) of each listed stream, all in one
+ * launch: per stream i, out[i*per_cap ...] strongest first and counts[i]
+ * (ATKDSP_E_CAP past min(per_cap, QSET_CAP) channels, the first ones then in
+ * bin order; 0 with lines[i] = -1 when the stream has no line yet) */
+int atkgpu_server_detect_latest(atkgpu_server *v, const int *sids, unsigned n, float threshold_db, int gap_bins,
+                                int min_run, atkdsp_channel *out, size_t per_cap, ptrdiff_t *counts, long long *lines,
+                                float *floors) {
+    if (!v || !sids || !counts || n == 0 || n > v->max_streams || (!out && per_cap)) return ATKDSP_E_ARG;
+    if (gap_bins < 2) gap_bins = 2;
+    if (min_run < 2) min_run = 2;
+    const size_t kcap = per_cap < QSET_CAP ? per_cap : QSET_CAP;
+    const float **ptr_h = (const float **)v->qs_h;
+    atkgpu_chan *rec_h = (atkgpu_chan *)(ptr_h + v->max_streams);
+    unsigned long long *found_h = (unsigned long long *)(rec_h + (size_t)v->max_streams * QSET_CAP);
+    float *floor_h = (float *)(found_h + v->max_streams);
+    char *dbase = (char *)v->qs_d, *hbase = (char *)v->qs_h;
+#define QDEV(p) ((void *)(dbase + ((char *)(p) - hbase)))
+    unsigned slot[4096];
+    if (n > 4096) return ATKDSP_E_ARG;
+    pthread_mutex_lock(&v->qmu);
+    pthread_mutex_lock(&v->mu);
+    unsigned nl = 0;
+    size_t cols = 0;
+    int rc = ATKDSP_OK;
+    for (unsigned i = 0; i < n && !rc; ++i) {
+        const int sid = sids[i];
+        if (sid < 0 || (unsigned)sid >= v->max_streams || !v->st[sid].used || !v->st[sid].dN) { rc = ATKDSP_E_ARG; break; }
+        const sstream *s = &v->st[sid];
+        if (cols && s->dcols != cols) { rc = ATKDSP_E_ARG; break; }   /* one line length per query */
+        cols = s->dcols;
+        const long long L = s->lines_done - 1;
+        if (lines) lines[i] = L;
+        counts[i] = 0;
+        if (L < 0) continue;
+        ptr_h[nl] = s->dring + (size_t)(L % s->dslots) * s->nel * cols;
+        slot[nl++] = i;
+    }
+    const int failed = v->failed;
+    pthread_mutex_unlock(&v->mu);
+    if (!rc && failed) rc = ATKDSP_E_FFT;
+    if (!rc && nl) {
+        if (atkgpu_detect_line_set(v->qstream, (const float *const *)QDEV(ptr_h), cols, nl, threshold_db, gap_bins,
+                                   min_run, (atkgpu_chan *)QDEV(rec_h), QSET_CAP, (unsigned long long *)QDEV(found_h),
+                                   (float *)QDEV(floor_h)) ||
+            atkgpu_sync(v->qstream))
+            rc = ATKDSP_E_FFT;
+    }
+    for (unsigned j = 0; !rc && j < nl; ++j) {
+        const unsigned i = slot[j];
+        const unsigned long long found = found_h[j];
+        const size_t kept = found < kcap ? (size_t)found : kcap;
+        atkdsp_channel *o = out + (size_t)i * per_cap;
+        memcpy(o, rec_h + (size_t)j * QSET_CAP, kept * sizeof(atkgpu_chan));
+        if (found > kcap) {
+            counts[i] = ATKDSP_E_CAP;
+        } else {
+            atk_sort_channels(o, kept);
+            counts[i] = (ptrdiff_t)kept;
+        }
+        if (floors) floors[i] = floor_h[j];
+    }
+#undef QDEV
+    pthread_mutex_unlock(&v->qmu);
+    return rc;
 }
 
 int atkgpu_server_drain(atkgpu_server *v) {

@@ -39,7 +39,10 @@
  * line as it completes (each stream's lines in turn; the floor is the median
  * of the smoothed line, found on the device; threshold 6 dB, gap 4, min run
  * 2), as a signal-finding consumer of the lines would; the JSON adds the
- * lines it covered, the channels found and the call latency.
+ * lines it covered, the channels found and the call latency. With --detect 2
+ * the consumer instead asks every 50 ms (one line period) for the newest line
+ * of every stream at once (atkgpu_server_detect_latest: one launch), and the
+ * latency is that of the whole query.
  *
  * Before the clock starts every stream gets one chunk of CHUNK_MAX samples and
  * the server is drained and its statistics reset: the figures are the steady
@@ -49,7 +52,7 @@
  * usage: serve_load [--seconds S] [--streams N] [--speed X] [--budget-gib G]
  *                   [--max-batch B] [--seed K] [--mix c4|uniform] [--arrays A]
  *                   [--display 0|1] [--ring 0|1] [--consumers C] [--ring-blocks B]
- *                   [--detect 0|1]
+ *                   [--detect 0|1|2]
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -320,6 +323,37 @@ static void *line_detector(void *arg) {
     return NULL;
 }
 
+/* --detect 2: the newest line of every display stream, every 50 ms, one query */
+static void *line_detector_latest(void *arg) {
+    (void)arg;
+    int *sids = (int *)malloc((size_t)g_ns * sizeof(int));
+    ptrdiff_t *counts = (ptrdiff_t *)malloc((size_t)g_ns * sizeof(ptrdiff_t));
+    long long *lines = (long long *)malloc((size_t)g_ns * sizeof(long long));
+    atkdsp_channel *ch = (atkdsp_channel *)malloc((size_t)g_ns * 64 * sizeof(atkdsp_channel));
+    int n = 0;
+    for (int i = 0; i < g_ns; ++i)
+        if (g_st[i].nel == 1) sids[n++] = g_st[i].sid;
+    while (!g_producers_done && n) {
+        const double t0 = now_s();
+        if (atkgpu_server_detect_latest(g_srv, sids, (unsigned)n, 6.0f, 4, 2, ch, 64, counts, lines, NULL) == 0) {
+            const double t1 = now_s();
+            if (g_det_n < g_det_cap) g_det_lat[g_det_n++] = (t1 - t0) * 1e3;
+            for (int i = 0; i < n; ++i)
+                if (lines[i] >= 0) {
+                    g_det_lines++;
+                    g_det_chans += counts[i] < 0 ? 64 : counts[i];
+                }
+        }
+        struct timespec ts = {0, 50000000L};
+        nanosleep(&ts, NULL);
+    }
+    free(sids);
+    free(counts);
+    free(lines);
+    free(ch);
+    return NULL;
+}
+
 int main(int argc, char **argv) {
     double seconds = 5.0, speed = 1.0, budget_gib = 32.0;
     int streams = 128, max_batch = 256;
@@ -434,7 +468,7 @@ int main(int argc, char **argv) {
     for (long t = 0; t < NTHREADS; ++t) pthread_create(&th[t], NULL, submitter, (void *)t);
     if (g_ring)
         for (long t = 0; t < g_consumers; ++t) pthread_create(&rc[t], NULL, ring_consumer, (void *)t);
-    if (g_det_lat) pthread_create(&dth, NULL, line_detector, NULL);
+    if (g_det_lat) pthread_create(&dth, NULL, g_detect == 2 ? line_detector_latest : line_detector, NULL);
     for (int t = 0; t < NTHREADS; ++t) pthread_join(th[t], NULL);
     g_producers_done = 1;
     if (g_ring)
@@ -474,7 +508,7 @@ int main(int argc, char **argv) {
            st.mean_ms, st.hbm_used / 1073741824.0, st.hbm_budget / 1073741824.0, display, lines, g_ring, g_lost,
            g_picked ? 1e3 * g_pick_sum / (double)g_picked : 0.0, 1e3 * g_pick_max,
            g_picked ? 1e6 * g_call_sum / (double)g_picked : 0.0, 1e3 * g_call_max, g_ring ? g_consumers : 0, g_ring ? g_ring_blocks : 0, 1e3 * g_pass_max, 1e3 * g_late_max,
-           g_det_lat != NULL, g_det_lines, g_det_missed, g_det_chans, dp50, dp99, dmax, cpus);
+           g_det_lat ? g_detect : 0, g_det_lines, g_det_missed, g_det_chans, dp50, dp99, dmax, cpus);
     atkgpu_server_destroy(g_srv);
     for (int s = 0; s < admitted; ++s) {
         free(g_st[s].src);

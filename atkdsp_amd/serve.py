@@ -63,6 +63,9 @@ def _bind(lib) -> None:
         "atkgpu_server_read_line": (C.c_int, [vp, C.c_int, C.c_longlong, P(C.c_float), sz]),
         "atkgpu_server_detect": (C.c_ssize_t, [vp, C.c_int, C.c_longlong, C.c_uint, C.c_float, C.c_float,
                                                C.c_int, C.c_int, P(_ChannelC), sz, P(C.c_float)]),
+        "atkgpu_server_detect_latest": (C.c_int, [vp, P(C.c_int), C.c_uint, C.c_float, C.c_int, C.c_int,
+                                                  P(_ChannelC), sz, P(C.c_ssize_t), P(C.c_longlong),
+                                                  P(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -196,6 +199,30 @@ class Server:
                                                       float(threshold_db), int(gap_bins), int(min_run), buf,
                                                       int(max_channels), C.byref(fl))), "server_detect")
         return _channels(buf, n), float(fl.value)
+
+    def detect_latest(self, sids, threshold_db: float = 6.0, gap_bins: int = 4, min_run: int = 2,
+                      max_channels: int = 64):
+        """detect over the newest completed line of every listed stream in one
+        launch: per stream {line, channels (strongest first), floor, capped}
+        (capped: more than max_channels found; channels are then the first
+        ones in bin order); line -1 = no line yet."""
+        n = len(sids)
+        ids = (C.c_int * n)(*[int(x) for x in sids])
+        buf = (_ChannelC * (n * max(int(max_channels), 1)))()
+        cnt = (C.c_ssize_t * n)()
+        lin = (C.c_longlong * n)()
+        fl = (C.c_float * n)()
+        _check(int(self._lib.atkgpu_server_detect_latest(self._h, ids, n, float(threshold_db), int(gap_bins),
+                                                         int(min_run), buf, int(max_channels), cnt, lin, fl)),
+               "server_detect_latest")
+        out = []
+        for i in range(n):
+            capped = cnt[i] == -3
+            k = int(max_channels) if capped else _check(int(cnt[i]), "server_detect_latest")
+            base = i * int(max_channels)
+            out.append({"line": int(lin[i]), "channels": _channels(buf[base:base + k], k),
+                        "floor": float(fl[i]), "capped": capped})
+        return out
     def drain(self) -> None:
         _check(self._lib.atkgpu_server_drain(self._h), "server_drain")
 

@@ -179,6 +179,15 @@ ATKGPU_API int       atkgpu_server_read_line(atkgpu_server *v, int sid, long lon
 ATKGPU_API ptrdiff_t atkgpu_server_detect(atkgpu_server *v, int sid, long long line, unsigned element,
                                           float floor_db, float threshold_db, int gap_bins, int min_run,
                                           atkdsp_channel *out, size_t out_cap, float *floor_used);
+/* the same over the newest completed line (element 0) of each of n streams
+ * (one display line length among them), in one launch, the floor of each
+ * the median of its smoothed line: for stream i, counts[i] channels at
+ * out + i*per_cap, strongest first (ATKDSP_E_CAP past min(per_cap, 256),
+ * the first ones then in bin order), lines[i] the line used (-1: none yet,
+ * count 0) and floors[i] its floor; lines and floors may be NULL */
+ATKGPU_API int       atkgpu_server_detect_latest(atkgpu_server *v, const int *sids, unsigned n, float threshold_db,
+                                                 int gap_bins, int min_run, atkdsp_channel *out, size_t per_cap,
+                                                 ptrdiff_t *counts, long long *lines, float *floors);
 
 /* ---- the filterbank in matrix-core form: S streams of one shape ----------------
  * The polyphase fold as a matrix product on the MFMA units, from raw 8-bit

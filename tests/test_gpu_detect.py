@@ -143,6 +143,21 @@ def test_server_detect_on_the_device_lines(gpu):
                 _same(chans2, O.detect_channels(line, -90.0, 3.0, 4, 2), f"floor -90, stream {s} line {L}")
                 checked += 1
     assert checked >= 30
+    # the newest lines in one launch equal the one-line queries (a query holds
+    # one line length: the 1024- and 512-column streams go separately)
+    with pytest.raises(gpu.AtkDspError):
+        srv.detect_latest([sid, aid])
+    for group in ([sid, sid], [aid]):
+        res = srv.detect_latest(group, threshold_db=6.0, gap_bins=4, min_run=2, max_channels=256)
+        for s_, r in zip(group, res):
+            one, fl1 = srv.detect(s_, r["line"], element=0, threshold_db=6.0, gap_bins=4, min_run=2)
+            assert r["line"] == srv.lines(s_) - 1 and r["floor"] == fl1 and not r["capped"]
+            assert [c.as_tuple() for c in r["channels"]] == [c.as_tuple() for c in one]
+    one, _ = srv.detect(sid, srv.lines(sid) - 1, threshold_db=6.0, gap_bins=4, min_run=2)
+    assert len(one) >= 2
+    capped = srv.detect_latest([sid], threshold_db=6.0, gap_bins=4, min_run=2, max_channels=1)[0]
+    assert capped["capped"] and len(capped["channels"]) == 1
+    assert capped["channels"][0].start_bin == min(c.start_bin for c in one)    # the first in bin order
     with pytest.raises(gpu.AtkDspError):
         srv.detect(sid, srv.lines(sid) + 100)                # not completed by the chunks submitted
     with pytest.raises(gpu.AtkDspError):
