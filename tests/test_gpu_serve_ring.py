@@ -17,14 +17,17 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 FS = 40e6
-CI16, CF32 = 2, 4
+CU8, CI8, CI16, CI16Q11, CF32 = 0, 1, 2, 3, 4
+NAMES = {CU8: "cu8", CI8: "ci8", CI16: "ci16", CI16Q11: "ci16q11", CF32: "cf32"}
 
 
 def _block(seq, fmt=CI16, nel=1):
     """the bytes of block seq: a deterministic function of seq, 1..8192 samples per element"""
     rng = np.random.default_rng(1000 + seq)
     n = int(rng.integers(1, 8193))
-    if fmt == CI16:
+    if fmt in (CU8, CI8):
+        return rng.integers(0, 256, size=2 * n * nel, dtype=np.uint8)
+    if fmt in (CI16, CI16Q11):
         return rng.integers(-2048, 2048, size=2 * n * nel, dtype=np.int16).view(np.uint8)
     return rng.standard_normal(2 * n * nel).astype(np.float32).view(np.uint8)
 
@@ -44,25 +47,26 @@ def _direct(blocks, nel=1, fmt="ci16"):
     return [srv.read(sid, t) for t in ts]
 
 
-def test_ring_blocks_serve_like_direct_submission(gpu):
-    for fmt, code, nel in (("ci16", CI16, 1), ("cf32", CF32, 3)):
-        blocks = [_block(s, code, nel) for s in range(12)]
-        ring = gpu.Ring(8192 * 8 * nel, 16, 2)
-        cons = ring.open()
-        srv, sid = _server(nel, fmt)
-        ts = []
-        for s, b in enumerate(blocks):
-            ring.write(b, first_sample=10_000 * s, format=code)
-            t, meta, lost = srv.submit_ring(sid, ring, cons)
-            assert t == s and lost == 0 and meta["seq"] == s and meta["first_sample"] == 10_000 * s
-            ts.append(t)
-        assert srv.submit_ring(sid, ring, cons) == (None, None, 0)
-        srv.drain()
-        ref = _direct(blocks, nel, fmt)
-        for t, y in zip(ts, ref):
-            assert np.array_equal(srv.read(sid, t), y), (fmt, t)
-        st = ring.stats()
-        assert st["read"][cons] == 12 and st["lost"][cons] == 0
+@pytest.mark.parametrize("code,nel", [(CU8, 1), (CI8, 1), (CI16, 1), (CI16Q11, 1), (CF32, 1), (CF32, 3), (CI16, 2)])
+def test_ring_blocks_serve_like_direct_submission(gpu, code, nel):
+    fmt = NAMES[code]
+    blocks = [_block(s, code, nel) for s in range(12)]
+    ring = gpu.Ring(8192 * 8 * nel, 16, 2)
+    cons = ring.open()
+    srv, sid = _server(nel, fmt)
+    ts = []
+    for s, b in enumerate(blocks):
+        ring.write(b, first_sample=10_000 * s, format=code)
+        t, meta, lost = srv.submit_ring(sid, ring, cons)
+        assert t == s and lost == 0 and meta["seq"] == s and meta["first_sample"] == 10_000 * s
+        ts.append(t)
+    assert srv.submit_ring(sid, ring, cons) == (None, None, 0)
+    srv.drain()
+    ref = _direct(blocks, nel, fmt)
+    for t, y in zip(ts, ref):
+        assert np.array_equal(srv.read(sid, t), y), (fmt, t)
+    st = ring.stats()
+    assert st["read"][cons] == 12 and st["lost"][cons] == 0
 
 
 def test_lapped_consumer_reports_lost_blocks_and_skips_them(gpu):
