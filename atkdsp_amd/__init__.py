@@ -767,6 +767,8 @@ class Ring:
         self._lib = load()
         if mem is not None:
             self._buf = np.frombuffer(mem, np.uint8)
+            if not self._buf.flags.writeable:        # consumers and the producer write into the ring
+                raise AtkDspError("ring_attach: the ring's memory must be writable")
             self._r = self._lib.atkdsp_ring_attach(self._buf.ctypes.data, self._buf.size)
             if not self._r:
                 raise AtkDspError("ring_attach: not a ring of this version")

@@ -127,6 +127,8 @@ def test_attach_refuses_what_is_not_a_ring():
         A.Ring(mem=ring.memory[:256])                        # shorter than the ring
     with pytest.raises(A.AtkDspError):
         A.Ring(0, 4, 2)
+    with pytest.raises(A.AtkDspError):                         # read-only memory
+        A.Ring(mem=ring.memory.tobytes())
     assert A.ring_bytes(128, 1, 2) == 0 and A.ring_bytes(128, 4, 17) == 0
 
 
