@@ -61,6 +61,16 @@ class Channel:
                 f"peak {self.peak_db:.1f} dB, snr {self.snr_db:.1f} dB)")
 
 
+class _CfarConfig(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("n_ref", C.c_int32), ("n_guard", C.c_int32), ("k_frames", C.c_int32),
+                ("pfa", C.c_double), ("looks", C.c_double), ("os_k", C.c_int32), ("open_m", C.c_int32),
+                ("open_n", C.c_int32), ("close_k", C.c_int32), ("merge_gap", C.c_int32), ("min_bins", C.c_int32),
+                ("max_tracks", C.c_int32), ("reserved", C.c_int32), ("rho2", C.c_double * 4)]
+
+
+CFAR_MODES = {"ca": 0, "os": 1, "go": 2, "so": 3}
+
+
 class _RingMeta(C.Structure):
     _fields_ = [("seq", C.c_uint64), ("first_sample", C.c_uint64), ("t_utc_ns", C.c_int64),
                 ("fc_hz", C.c_double), ("fs_hz", C.c_double), ("gain_db", C.c_float), ("format", C.c_int32),
@@ -118,6 +128,10 @@ def _bind(lib) -> None:
         "atkdsp_decimate_max": (None, [fp, sz, fp, sz]),
         "atkdsp_stitch_max": (None, [fp, sz, d, d, d, d, d, d, fp, sz]),
         "atkdsp_median": (C.c_float, [fp, sz, fp]),
+        "atkdsp_cfar_defaults": (None, [C.POINTER(_CfarConfig)]),
+        "atkdsp_cfar_alpha": (C.c_int, [C.c_int, C.c_int, d, d, C.c_int, C.POINTER(d), C.POINTER(d), C.POINTER(d)]),
+        "atkdsp_cfar_n_eff": (d, [C.c_int, C.POINTER(d)]),
+        "atkdsp_cfar_window_rho2": (C.c_int, [fp, sz, C.POINTER(d)]),
         "atkdsp_detect_channels": (C.c_ssize_t, [fp, sz, C.c_float, C.c_float, C.c_int, C.c_int, fp,
                                                  C.POINTER(_ChannelC), sz]),
         "atkdsp_pfb_create": (vp, [d, u, u, u, d, u]),
@@ -470,6 +484,47 @@ def detect_channels(line, floor_db: float, threshold_db: float, gap_bins: int, m
                                                    int(gap_bins), int(min_run), _fp(scratch), buf,
                                                    int(max_channels))), "detect_channels")
     return _channels(buf, got)
+
+
+def _rho2_arg(rho2):
+    if rho2 is None:
+        return None
+    vals = [float(v) for v in rho2]
+    if len(vals) != 4:
+        raise AtkDspError("rho2 needs four correlations (bins 1..4 apart)")
+    return (C.c_double * 4)(*vals)
+
+
+def cfar_defaults() -> dict:
+    """The default CFAR configuration as a dict."""
+    c = _CfarConfig()
+    load().atkdsp_cfar_defaults(C.byref(c))
+    d = {k: getattr(c, k) for k, _ in _CfarConfig._fields_ if k not in ("reserved", "rho2")}
+    d["rho2"] = tuple(c.rho2)
+    return d
+
+
+def cfar_alpha(mode, n_ref: int, pfa: float, looks: float = 1.0, os_k: int = 0, rho2=None) -> tuple:
+    """(alpha, E[T]): the CFAR threshold factor for a false-alarm rate pfa per
+    cell, with the window's bin correlation rho2 (four values, or None)."""
+    mode = CFAR_MODES[mode] if isinstance(mode, str) else int(mode)
+    a, et = C.c_double(), C.c_double()
+    _check(load().atkdsp_cfar_alpha(mode, int(n_ref), float(pfa), float(looks), int(os_k), _rho2_arg(rho2),
+                                    C.byref(a), C.byref(et)), "cfar_alpha")
+    return a.value, et.value
+
+
+def cfar_n_eff(n_ref: int, rho2=None) -> float:
+    """Equivalent independent cells among 2 n_ref window-correlated ones."""
+    return float(load().atkdsp_cfar_n_eff(int(n_ref), _rho2_arg(rho2)))
+
+
+def cfar_window_rho2(window) -> tuple:
+    """The power correlation of periodogram bins 1..4 apart through window."""
+    w = _f32(window, "window")
+    out = (C.c_double * 4)()
+    _check(load().atkdsp_cfar_window_rho2(_fp(w), w.size, out), "cfar_window_rho2")
+    return tuple(float(v) for v in out)
 
 
 def stitch_max(seg, seg_lo_hz: float, seg_hz_per_bin: float, keep_lo_hz: float,
@@ -875,6 +930,7 @@ __all__ = [
     "set_threads", "get_threads", "FMT", "DET", "WIN", "bytes_per_sample",
     "unpack", "unpack_dc", "Fft", "window", "power_db", "spectrum_reduce", "spectrum_stats",
     "window_stats", "db_to_pixels", "decimate_max", "stitch_max", "median", "detect_channels", "Channel",
+    "cfar_defaults", "cfar_alpha", "cfar_n_eff", "cfar_window_rho2", "CFAR_MODES",
     "Pfb", "pfb_cov_accumulate", "pfb_cov_reset", "pfb_channel_power",
     "design_lowpass", "ArbResampler", "Osx", "Ring", "ring_bytes",
 ]

@@ -59,6 +59,7 @@ IMPLEMENTED = {
     "atkdsp_ring_open", "atkdsp_ring_close", "atkdsp_ring_read", "atkdsp_ring_peek", "atkdsp_ring_release",
     "atkdsp_ring_copy_range", "atkdsp_ring_stats", "atkdsp_ring_describe",
     "atkdsp_median", "atkdsp_detect_channels",
+    "atkdsp_cfar_defaults", "atkdsp_cfar_alpha", "atkdsp_cfar_n_eff", "atkdsp_cfar_window_rho2",
 }
 
 _STUB_BODY = {

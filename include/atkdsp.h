@@ -18,7 +18,7 @@
  *   - NaN means "unmeasured" (no fast-math anywhere).
  *
  * The library also exports every other name of the reference ABI (334 in
- * all; 73 are implemented here) as a documented stub: *_create returns NULL, signed counts return
+ * all; 77 are implemented here) as a documented stub: *_create returns NULL, signed counts return
  * ATKDSP_E_ARG, unsigned counts 0, floating results NaN, void functions do
  * nothing. Those are outside the wideband front end.
  */
@@ -137,6 +137,41 @@ ATKDSP_API ptrdiff_t atkdsp_detect_channels(const float *line, size_t n, float f
                                             float threshold_db, int gap_bins, int min_run,
                                             float *smooth_scratch,
                                             atkdsp_channel *out, size_t out_cap);
+
+/* ---- CFAR threshold design: reference include/atkdsp.h:1733-1860,
+ * src/cfar.c:23-268. The defaults, the equivalent independent cells of a
+ * window-correlated periodogram, that correlation for a window, and the
+ * threshold factor alpha for a false-alarm rate (host code, design time; the
+ * solver is this library's own and agrees with the reference's to its
+ * stated accuracy). The frame test and the burst tracker (atkdsp_cfar_create
+ * and the rest) are not built here. ---- */
+#define ATKDSP_CFAR_CA 0
+#define ATKDSP_CFAR_OS 1
+#define ATKDSP_CFAR_GO 2
+#define ATKDSP_CFAR_SO 3
+#define ATKDSP_CFAR_RHO_MAX 4
+typedef struct {
+    int32_t mode;        /* ATKDSP_CFAR_CA / OS / GO / SO */
+    int32_t n_ref;       /* reference cells each side, 1..256 */
+    int32_t n_guard;     /* guard cells each side, 0..256 */
+    int32_t k_frames;    /* frames averaged before testing, 1..64 */
+    double  pfa;         /* per cell per frame, 1e-15..0.5 */
+    double  looks;       /* periodograms averaged into one frame, 0.25..1e4 */
+    int32_t os_k;        /* OS rank 1..2 n_ref; 0 = ceil(3N/4) */
+    int32_t open_m;
+    int32_t open_n;
+    int32_t close_k;
+    int32_t merge_gap;
+    int32_t min_bins;
+    int32_t max_tracks;
+    int32_t reserved;
+    double  rho2[4];     /* power correlation of bins 1..4 apart, each in [0, 1) */
+} atkdsp_cfar_config;    /* 96 bytes, as the reference header compiles (its comment says 104) */
+ATKDSP_API void   atkdsp_cfar_defaults(atkdsp_cfar_config *cfg);
+ATKDSP_API int    atkdsp_cfar_alpha(int mode, int n_ref, double pfa, double looks, int os_k, const double *rho2,
+                                    double *alpha, double *et);
+ATKDSP_API double atkdsp_cfar_n_eff(int n_ref, const double *rho2);
+ATKDSP_API int    atkdsp_cfar_window_rho2(const float *window, size_t n, double *rho2);
 
 /* ---- NCO: reference include/atkdsp.h:309-319, src/nco.c:27-98.
  * out[n] = in[n] * exp(j*(phase + n*step)); the phase advances by n*step,

@@ -2,7 +2,7 @@
 # Build the reference atkdsp C library from its own sources into oracle/_ref/,
 # for use as a checker only (golden vectors, the CPU baseline timing).
 #
-# Compiles the eleven reference files the wideband path needs, in place under
+# Compiles the thirteen reference files the path and its next rows need, in place under
 # /root/reference, with the flags of the reference's build.sh (C99, -O3, AVX2,
 # FMA, OpenMP, never fast-math). Nothing is copied into this repository; the
 # output (oracle/_ref/atkdsp_ref.so) is git-ignored. Exits 0 without building
@@ -20,7 +20,7 @@ cd "$REF"
 gcc -std=c99 -O3 -fno-math-errno -mavx2 -mfma -fopenmp -fPIC -shared \
     -fvisibility=hidden -DATKDSP_BUILD=1 -Iinclude -Ivendor/pocketfft \
     src/atkdsp.c src/convert.c src/channelizer.c src/fft.c src/display.c \
-    src/detect.c src/design.c src/resample.c src/nco.c src/ring.c vendor/pocketfft/pocketfft.c \
+    src/detect.c src/design.c src/resample.c src/nco.c src/ring.c src/cfar.c src/stat.c vendor/pocketfft/pocketfft.c \
     -o "$OUT/atkdsp_ref.so.tmp" -lm -Wl,--no-undefined 2>"$OUT/build.log"
 mv "$OUT/atkdsp_ref.so.tmp" "$OUT/atkdsp_ref.so"
 echo "build_ref: $OUT/atkdsp_ref.so" >&2

@@ -76,6 +76,10 @@ class Ref:
             "atkdsp_osx_out_max": (sz, [vp, C.c_int, sz]),
             "atkdsp_osx_process_n": (C.c_ssize_t, [vp, P(cf), u, sz, P(cf), P(sz), sz]),
             "atkdsp_median": (C.c_float, [fp, sz, fp]),
+            "atkdsp_cfar_defaults": (None, [vp]),
+            "atkdsp_cfar_alpha": (C.c_int, [C.c_int, C.c_int, d, d, C.c_int, vp, P(d), P(d)]),
+            "atkdsp_cfar_n_eff": (d, [C.c_int, vp]),
+            "atkdsp_cfar_window_rho2": (C.c_int, [fp, sz, P(d)]),
             "atkdsp_detect_channels": (C.c_ssize_t, [fp, sz, C.c_float, C.c_float, C.c_int, C.c_int, fp,
                                                      P(ChanC), sz]),
         }
@@ -210,6 +214,29 @@ class Ref:
         kept = cap if n == -3 else max(n, 0)                 # E_CAP: the first cap found, bin order
         recs = [(c.start_bin, c.end_bin, c.centroid_bin, c.peak_db, c.snr_db) for c in buf[:kept]]
         return recs, int(n)
+
+    def cfar_alpha(self, mode, n_ref, pfa, looks=1.0, os_k=0, rho2=None):
+        """(return code, alpha, E[T])"""
+        a, et = C.c_double(), C.c_double()
+        r = (C.c_double * 4)(*rho2) if rho2 is not None else None
+        rc = self.L.atkdsp_cfar_alpha(int(mode), int(n_ref), float(pfa), float(looks), int(os_k),
+                                      C.cast(r, C.c_void_p) if r is not None else None, C.byref(a), C.byref(et))
+        return int(rc), a.value, et.value
+
+    def cfar_n_eff(self, n_ref, rho2=None):
+        r = (C.c_double * 4)(*rho2) if rho2 is not None else None
+        return float(self.L.atkdsp_cfar_n_eff(int(n_ref), C.cast(r, C.c_void_p) if r is not None else None))
+
+    def cfar_window_rho2(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        out = (C.c_double * 4)()
+        rc = self.L.atkdsp_cfar_window_rho2(self._f(w), w.size, out)
+        return int(rc), tuple(out)
+
+    def cfar_defaults_bytes(self):
+        buf = (C.c_uint8 * 96)()
+        self.L.atkdsp_cfar_defaults(C.cast(buf, C.c_void_p))
+        return bytes(buf)
 
     def pfb(self, fs, nchan, os_, K, atten, max_nel=1):
         return RefPfb(self, fs, nchan, os_, K, atten, max_nel)

@@ -329,6 +329,32 @@ def detect_golden():
          m_y=np.array([r.median(x) for x in med_x], np.float32))
 
 
+CFAR_CASES = [(mode, n, pfa, looks, k, rho)
+              for mode in (0, 1, 2, 3) for n in (1, 2, 8, 16, 64) for pfa in (0.5, 1e-3, 1e-6, 1e-12, 1e-15)
+              for looks in (0.25, 1.0, 2.5, 10.0) for k in ((0, 1) if mode == 1 else (0,))
+              for rho in (0, 1)]
+
+
+def cfar_golden():
+    """The reference's CFAR threshold design: alpha and E[T] over modes, cell
+    counts, rates, looks, OS ranks and window correlation (Hann's, or none);
+    n_eff; window correlations; the default configuration's bytes."""
+    r = Ref()
+    hann = (4.0 / 9.0, 1.0 / 36.0, 0.0, 0.0)
+    rc, al, et = [], [], []
+    for mode, n, pfa, looks, k, rho in CFAR_CASES:
+        c, a, e = r.cfar_alpha(mode, n, pfa, looks, k, hann if rho else None)
+        rc.append(c)
+        al.append(a)
+        et.append(e)
+    rho2 = {f"rho2_{kind}_{n}": np.array(r.cfar_window_rho2(oracle.window(kind, n))[1])
+            for kind in (0, 1, 2, 3) for n in (1000, 4096)}
+    save("cfar", c_args=np.array([c[:5] for c in CFAR_CASES], np.float64), c_rho=np.array([c[5] for c in CFAR_CASES]),
+         c_rc=np.array(rc), c_alpha=np.array(al), c_et=np.array(et),
+         n_eff=np.array([[r.cfar_n_eff(n, hann), r.cfar_n_eff(n, None)] for n in range(1, 40)]),
+         defaults=np.frombuffer(r.cfar_defaults_bytes(), np.uint8), **rho2)
+
+
 if __name__ == "__main__":
     import sys
     if sys.argv[1:] == ["nco"]:
@@ -337,8 +363,11 @@ if __name__ == "__main__":
         osx_golden()
     elif sys.argv[1:] == ["detect"]:
         detect_golden()
+    elif sys.argv[1:] == ["cfar"]:
+        cfar_golden()
     else:
         main()
         nco_golden()
         osx_golden()
         detect_golden()
+        cfar_golden()
