@@ -69,6 +69,18 @@ class _CfarConfig(C.Structure):
 
 
 CFAR_MODES = {"ca": 0, "os": 1, "go": 2, "so": 3}
+CFAR_EVENT = {1: "open", 2: "update", 3: "close"}
+# the 64-byte event record (atkdsp_cfar_event) as a numpy record type
+CFAR_EVENT_DTYPE = np.dtype([("t0", "<f8"), ("t1", "<f8"), ("fc_bin", "<f8"), ("type", "<i4"), ("id", "<u4"),
+                             ("flags", "<u4"), ("lo_bin", "<i4"), ("hi_bin", "<i4"), ("n_hits", "<i4"),
+                             ("level_db", "<f4"), ("noise_db", "<f4"), ("snr_db", "<f4"),
+                             ("band_snr_db", "<f4")])
+assert CFAR_EVENT_DTYPE.itemsize == 64
+
+
+class _CfarCounters(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("frames", "warmup", "cells_tested", "cell_hits", "untestable",
+                                           "opened", "closed", "updates", "dropped", "resets", "tracks")]
 
 
 class _RingMeta(C.Structure):
@@ -90,7 +102,7 @@ class _Nco(C.Structure):
 
 
 _ERRORS = {-1: "bad argument", -2: "out of memory", -3: "output buffer too small",
-           -4: "FFT backend or device failure"}
+           -4: "FFT backend or device failure", -7: "nothing to return yet"}
 
 _lib = None
 _lib_path: Path | None = None
@@ -132,6 +144,17 @@ def _bind(lib) -> None:
         "atkdsp_cfar_alpha": (C.c_int, [C.c_int, C.c_int, d, d, C.c_int, C.POINTER(d), C.POINTER(d), C.POINTER(d)]),
         "atkdsp_cfar_n_eff": (d, [C.c_int, C.POINTER(d)]),
         "atkdsp_cfar_window_rho2": (C.c_int, [fp, sz, C.POINTER(d)]),
+        "atkdsp_cfar_create": (vp, [sz, C.POINTER(_CfarConfig)]),
+        "atkdsp_cfar_destroy": (None, [vp]),
+        "atkdsp_cfar_reset": (None, [vp]),
+        "atkdsp_cfar_set_mask": (C.c_int, [vp, vp]),
+        "atkdsp_cfar_events_max": (sz, [vp, sz]),
+        "atkdsp_cfar_process": (C.c_ssize_t, [vp, fp, sz, d, d, vp, sz]),
+        "atkdsp_cfar_flush": (C.c_ssize_t, [vp, vp, sz]),
+        "atkdsp_cfar_last": (C.c_int, [vp, fp, fp, vp]),
+        "atkdsp_cfar_info": (C.c_int, [vp, C.POINTER(d), C.POINTER(d), C.POINTER(d), C.POINTER(_CfarConfig)]),
+        "atkdsp_cfar_describe": (C.c_int, [vp, C.c_char_p, sz]),
+        "atkdsp_cfar_stats": (C.c_int, [vp, C.POINTER(_CfarCounters)]),
         "atkdsp_detect_channels": (C.c_ssize_t, [fp, sz, C.c_float, C.c_float, C.c_int, C.c_int, fp,
                                                  C.POINTER(_ChannelC), sz]),
         "atkdsp_pfb_create": (vp, [d, u, u, u, d, u]),
@@ -525,6 +548,89 @@ def cfar_window_rho2(window) -> tuple:
     out = (C.c_double * 4)()
     _check(load().atkdsp_cfar_window_rho2(_fp(w), w.size, out), "cfar_window_rho2")
     return tuple(float(v) for v in out)
+
+
+class Cfar:
+    """CFAR detection over spectrum frames and the burst tracker (atkdsp_cfar,
+    reference include/atkdsp.h:1733-1925, src/cfar.c:270-819). Frames of
+    nbins dB values are tested on the GPU; events come back as records of
+    CFAR_EVENT_DTYPE. Keyword arguments override the default configuration
+    (mode as "ca" / "os" / "go" / "so" or 0..3, rho2 as four values)."""
+
+    def __init__(self, nbins, **cfg):
+        self._lib = load()
+        c = _CfarConfig()
+        self._lib.atkdsp_cfar_defaults(C.byref(c))
+        for k, v in cfg.items():
+            if k == "mode":
+                v = CFAR_MODES[v] if isinstance(v, str) else int(v)
+            if k == "rho2":
+                c.rho2[:] = [float(x) for x in v]
+            elif k in dict(_CfarConfig._fields_) and k != "reserved":
+                setattr(c, k, v)
+            else:
+                raise AtkDspError(f"cfar: unknown setting {k}")
+        self.nbins = int(nbins)
+        self._p = self._lib.atkdsp_cfar_create(self.nbins, C.byref(c))
+        if not self._p:
+            raise AtkDspError("cfar_create: a setting is out of range (or there is no GPU)")
+
+    def __del__(self):
+        p, self._p = getattr(self, "_p", None), None
+        if p:
+            self._lib.atkdsp_cfar_destroy(p)
+
+    def reset(self):
+        self._lib.atkdsp_cfar_reset(self._p)
+
+    def set_mask(self, mask=None):
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        if m is not None and m.size != self.nbins:
+            raise AtkDspError("cfar mask needs nbins values")
+        _check(self._lib.atkdsp_cfar_set_mask(self._p, None if m is None else m.ctypes.data), "cfar_set_mask")
+
+    def events_max(self, nframes):
+        return int(self._lib.atkdsp_cfar_events_max(self._p, int(nframes)))
+
+    def process(self, frames_db, t0=0.0, dt=1.0) -> np.ndarray:
+        f = np.ascontiguousarray(frames_db, dtype=np.float32).reshape(-1)
+        if f.size % self.nbins:
+            raise AtkDspError("frames_db must hold whole frames of nbins values")
+        nf = f.size // self.nbins
+        ev = np.zeros(max(self.events_max(nf), 1), CFAR_EVENT_DTYPE)
+        n = _check(int(self._lib.atkdsp_cfar_process(self._p, _fp(f), nf, float(t0), float(dt), ev.ctypes.data,
+                                                     ev.size if nf else 0)), "cfar_process")
+        return ev[:n].copy()
+
+    def flush(self) -> np.ndarray:
+        ev = np.zeros(65536, CFAR_EVENT_DTYPE)
+        n = _check(int(self._lib.atkdsp_cfar_flush(self._p, ev.ctypes.data, ev.size)), "cfar_flush")
+        return ev[:n].copy()
+
+    def last(self):
+        """(average dB, threshold dB, hits) of the last tested frame"""
+        a = np.empty(self.nbins, np.float32)
+        t = np.empty(self.nbins, np.float32)
+        h = np.empty(self.nbins, np.uint8)
+        _check(self._lib.atkdsp_cfar_last(self._p, _fp(a), _fp(t), h.ctypes.data), "cfar_last")
+        return a, t, h
+
+    def info(self) -> dict:
+        a, aca, et, c = C.c_double(), C.c_double(), C.c_double(), _CfarConfig()
+        _check(self._lib.atkdsp_cfar_info(self._p, C.byref(a), C.byref(aca), C.byref(et), C.byref(c)), "cfar_info")
+        d = {k: getattr(c, k) for k, _ in _CfarConfig._fields_ if k not in ("reserved", "rho2")}
+        d.update(rho2=tuple(c.rho2), alpha=a.value, alpha_ca=aca.value, et=et.value)
+        return d
+
+    def describe(self) -> str:
+        buf = C.create_string_buffer(512)
+        _check(self._lib.atkdsp_cfar_describe(self._p, buf, 512), "cfar_describe")
+        return buf.value.decode()
+
+    def stats(self) -> dict:
+        st = _CfarCounters()
+        _check(self._lib.atkdsp_cfar_stats(self._p, C.byref(st)), "cfar_stats")
+        return {k: int(getattr(st, k)) for k, _ in _CfarCounters._fields_}
 
 
 def stitch_max(seg, seg_lo_hz: float, seg_hz_per_bin: float, keep_lo_hz: float,
@@ -930,7 +1036,8 @@ __all__ = [
     "set_threads", "get_threads", "FMT", "DET", "WIN", "bytes_per_sample",
     "unpack", "unpack_dc", "Fft", "window", "power_db", "spectrum_reduce", "spectrum_stats",
     "window_stats", "db_to_pixels", "decimate_max", "stitch_max", "median", "detect_channels", "Channel",
-    "cfar_defaults", "cfar_alpha", "cfar_n_eff", "cfar_window_rho2", "CFAR_MODES",
+    "cfar_defaults", "cfar_alpha", "cfar_n_eff", "cfar_window_rho2", "CFAR_MODES", "Cfar", "CFAR_EVENT",
+    "CFAR_EVENT_DTYPE",
     "Pfb", "pfb_cov_accumulate", "pfb_cov_reset", "pfb_channel_power",
     "design_lowpass", "ArbResampler", "Osx", "Ring", "ring_bytes",
 ]

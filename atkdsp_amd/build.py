@@ -60,6 +60,9 @@ IMPLEMENTED = {
     "atkdsp_ring_copy_range", "atkdsp_ring_stats", "atkdsp_ring_describe",
     "atkdsp_median", "atkdsp_detect_channels",
     "atkdsp_cfar_defaults", "atkdsp_cfar_alpha", "atkdsp_cfar_n_eff", "atkdsp_cfar_window_rho2",
+    "atkdsp_cfar_create", "atkdsp_cfar_destroy", "atkdsp_cfar_reset", "atkdsp_cfar_set_mask",
+    "atkdsp_cfar_events_max", "atkdsp_cfar_process", "atkdsp_cfar_flush", "atkdsp_cfar_last",
+    "atkdsp_cfar_info", "atkdsp_cfar_describe", "atkdsp_cfar_stats",
 }
 
 _STUB_BODY = {

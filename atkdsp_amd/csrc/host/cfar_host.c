@@ -127,6 +127,8 @@ static void lgam_pq(double a, double x, double *lp, double *lq) {
     *lp = log1m_exp(*lq);
 }
 
+void atk_cfar_lgam_pq(double a, double x, double *lp, double *lq) { lgam_pq(a, x, lp, lq); }
+
 /* the continued fraction of the incomplete beta (modified Lentz) */
 static double beta_cf(double a, double b, double x) {
     const double tiny = 1e-300;

@@ -41,4 +41,7 @@ atk_pool *atk_pool_get(void);
 /* channel records strongest first, with the reference's comparator and qsort */
 void atk_sort_channels(atkdsp_channel *c, size_t n);
 
+/* ln P(a, x) and ln Q(a, x), the regularized incomplete gamma (cfar_host.c) */
+void atk_cfar_lgam_pq(double a, double x, double *lp, double *lq);
+
 #endif

@@ -18,7 +18,7 @@
  *   - NaN means "unmeasured" (no fast-math anywhere).
  *
  * The library also exports every other name of the reference ABI (334 in
- * all; 77 are implemented here) as a documented stub: *_create returns NULL, signed counts return
+ * all; 88 are implemented here) as a documented stub: *_create returns NULL, signed counts return
  * ATKDSP_E_ARG, unsigned counts 0, floating results NaN, void functions do
  * nothing. Those are outside the wideband front end.
  */
@@ -44,6 +44,7 @@ extern "C" {
 #define ATKDSP_E_CAP   -3
 #define ATKDSP_E_FFT   -4   /* also: a HIP launch or copy failed       */
 #define ATKDSP_E_OVERRUN -6 /* a ring reader was lapped: the data is gone */
+#define ATKDSP_E_EMPTY -7   /* nothing to return yet */
 
 /* complex float32, numpy complex64 layout: reference include/atkdsp.h:212 */
 typedef struct { float re, im; } atkdsp_cf32;
@@ -143,8 +144,7 @@ ATKDSP_API ptrdiff_t atkdsp_detect_channels(const float *line, size_t n, float f
  * window-correlated periodogram, that correlation for a window, and the
  * threshold factor alpha for a false-alarm rate (host code, design time; the
  * solver is this library's own and agrees with the reference's to its
- * stated accuracy). The frame test and the burst tracker (atkdsp_cfar_create
- * and the rest) are not built here. ---- */
+ * stated accuracy). ---- */
 #define ATKDSP_CFAR_CA 0
 #define ATKDSP_CFAR_OS 1
 #define ATKDSP_CFAR_GO 2
@@ -172,6 +172,57 @@ ATKDSP_API int    atkdsp_cfar_alpha(int mode, int n_ref, double pfa, double look
                                     double *alpha, double *et);
 ATKDSP_API double atkdsp_cfar_n_eff(int n_ref, const double *rho2);
 ATKDSP_API int    atkdsp_cfar_window_rho2(const float *window, size_t n, double *rho2);
+
+/* ---- CFAR detection over spectrum frames and the burst tracker: reference
+ * include/atkdsp.h:1733-1925, src/cfar.c:270-819. Frames of nbins dB values
+ * go to the GPU, which forms power, the k_frames average and the CA / OS /
+ * GO / SO test of every bin against its reference cells (FP64); the tracker
+ * that turns hit bins into OPEN / UPDATE / CLOSE events runs on the host,
+ * frame by frame. ---- */
+#define ATKDSP_CFAR_OPEN   1
+#define ATKDSP_CFAR_UPDATE 2
+#define ATKDSP_CFAR_CLOSE  3
+#define ATKDSP_CFAR_T0_COARSE 1u   /* flags */
+#define ATKDSP_CFAR_T1_COARSE 2u
+#define ATKDSP_CFAR_EDGE      4u   /* a bin of the span borrowed or lacked reference cells */
+#define ATKDSP_CFAR_MASK_ADJ  8u   /* a masked bin within merge_gap + 1 of the span */
+#define ATKDSP_CFAR_FLUSHED  16u
+#define ATKDSP_CFAR_EDGE_PFA 1e-3  /* the single-frame band test behind t0 / t1 */
+typedef struct {
+    double   t0, t1;     /* seconds; t1 NaN before CLOSE */
+    double   fc_bin;     /* excess-weighted centroid */
+    int32_t  type;
+    uint32_t id;         /* from 1, per handle */
+    uint32_t flags;
+    int32_t  lo_bin, hi_bin;
+    int32_t  n_hits;     /* frames with a hit */
+    float    level_db, noise_db, snr_db, band_snr_db;
+} atkdsp_cfar_event;     /* 64 bytes */
+typedef struct {
+    uint64_t frames, warmup;
+    uint64_t cells_tested, cell_hits, untestable;
+    uint64_t opened, closed, updates;
+    uint64_t dropped;    /* blobs refused at max_tracks */
+    uint64_t resets;
+    uint64_t tracks;     /* pending + open now */
+} atkdsp_cfar_counters;  /* 88 bytes */
+typedef struct atkdsp_cfar atkdsp_cfar;
+/* NULL on an invalid configuration or without a GPU */
+ATKDSP_API atkdsp_cfar *atkdsp_cfar_create(size_t nbins, const atkdsp_cfar_config *cfg);
+ATKDSP_API void         atkdsp_cfar_destroy(atkdsp_cfar *c);
+ATKDSP_API void         atkdsp_cfar_reset(atkdsp_cfar *c);         /* frames and tracks forgotten; mask kept */
+ATKDSP_API int          atkdsp_cfar_set_mask(atkdsp_cfar *c, const uint8_t *mask);   /* NULL clears */
+ATKDSP_API size_t       atkdsp_cfar_events_max(const atkdsp_cfar *c, size_t nframes);
+/* E_CAP, before any frame is taken, when cap < events_max(nframes) */
+ATKDSP_API ptrdiff_t    atkdsp_cfar_process(atkdsp_cfar *c, const float *frames_db, size_t nframes,
+                                            double frame_t0, double frame_dt, atkdsp_cfar_event *ev, size_t cap);
+ATKDSP_API ptrdiff_t    atkdsp_cfar_flush(atkdsp_cfar *c, atkdsp_cfar_event *ev, size_t cap);
+/* the last tested frame's average and threshold (dB) and hits; E_EMPTY before one */
+ATKDSP_API int          atkdsp_cfar_last(const atkdsp_cfar *c, float *avg_db, float *thr_db, uint8_t *hits);
+ATKDSP_API int          atkdsp_cfar_info(const atkdsp_cfar *c, double *alpha, double *alpha_ca, double *et,
+                                         atkdsp_cfar_config *cfg);
+ATKDSP_API int          atkdsp_cfar_describe(const atkdsp_cfar *c, char *buf, size_t cap);
+ATKDSP_API int          atkdsp_cfar_stats(const atkdsp_cfar *c, atkdsp_cfar_counters *st);
 
 /* ---- NCO: reference include/atkdsp.h:309-319, src/nco.c:27-98.
  * out[n] = in[n] * exp(j*(phase + n*step)); the phase advances by n*step,

@@ -12,7 +12,10 @@ Each function restates the reference C (file:line given) in numpy:
     reference's own tests measure against (tests/test_channelizer.py:397-398);
   - the spectrum line restates the reference's mixed arithmetic: window and
     FFT in FP64, power and detector in float32 (src/fft.c:91-108, :177-191),
-    stats planes in FP64 (src/fft.c:317-331).
+    stats planes in FP64 (src/fft.c:317-331);
+  - the CFAR frame test in FP64 and its burst tracker (src/cfar.c:336-365,
+    :523-726), with alpha, CA's alpha and E[T] given (the threshold design is
+    pinned on its own against closed forms and the reference's values).
 
 How each function is pinned (tests/test_oracle.py): against golden vectors
 written by the compiled reference (tests/golden/, made by
@@ -732,6 +735,333 @@ class Osx:
     def process(self, x):
         outs, cnts = self.process_n([x], 1)
         return [o[0] if o is not None else None for o in outs], cnts
+
+
+# ---- CFAR detection and the burst tracker: src/cfar.c:270-819 -------------------
+CFAR_EVENT = np.dtype([("t0", "<f8"), ("t1", "<f8"), ("fc_bin", "<f8"), ("type", "<i4"), ("id", "<u4"),
+                       ("flags", "<u4"), ("lo_bin", "<i4"), ("hi_bin", "<i4"), ("n_hits", "<i4"),
+                       ("level_db", "<f4"), ("noise_db", "<f4"), ("snr_db", "<f4"), ("band_snr_db", "<f4")])
+CFAR_DEFAULTS = dict(mode=0, n_ref=16, n_guard=4, k_frames=1, pfa=1e-6, looks=1.0, os_k=0, open_m=2, open_n=3,
+                     close_k=3, merge_gap=1, min_bins=1, max_tracks=256, rho2=(0.0, 0.0, 0.0, 0.0))
+
+
+def cfar_tables(nbins: int, n_ref: int, n_guard: int, mask):
+    """Each bin's reference cells (src/cfar.c:336-365): up to n_ref unmasked
+    bins beyond the guard on the left (nearest first), then on the right, then
+    a short side's shortfall taken beyond the other side's cells (left's
+    first). Returns (cells (nbins x 2 n_ref), ok, sided)."""
+    N = 2 * n_ref
+    m = np.zeros(nbins, bool) if mask is None else np.asarray(mask) != 0
+    cells = np.zeros((nbins, N), np.int64)
+    ok = np.zeros(nbins, bool)
+    sided = np.zeros(nbins, bool)
+    free = np.flatnonzero(~m)
+    for i in range(nbins):
+        if m[i]:
+            continue
+        lo_side = free[free < i - n_guard][::-1]          # nearest first
+        hi_side = free[free > i + n_guard]
+        left, right = lo_side[:n_ref], hi_side[:n_ref]
+        more = list(hi_side[n_ref:][:n_ref - len(left)]) + list(lo_side[n_ref:][:n_ref - len(right)])
+        if len(left) + len(right) + len(more) < N:
+            continue
+        cells[i] = np.concatenate([left, right, np.array(more, np.int64)])
+        ok[i] = True
+        sided[i] = not more
+    return cells, ok, sided
+
+
+def cfar_test(X, cells, ok, sided, mode, n_ref, os_k, alpha, alpha_ca, et):
+    """The test of one averaged frame X (src/cfar.c:547-590): the statistic
+    T over each bin's cells (CA the mean, summed in table order; GO / SO the
+    larger / smaller side mean, CA with CA's alpha and unit scale where a side
+    borrowed; OS the os_k-th smallest, NaN if any cell is NaN); thr = a T,
+    noise = T / E[T], hit X > thr. Returns (thr, noise, hits)."""
+    N = 2 * n_ref
+    V = X[cells]                                          # nbins x N
+    a = np.full(X.size, alpha)
+    scale = np.full(X.size, et)
+    if mode == 1:
+        nan = np.isnan(V).any(axis=1)
+        Z = np.partition(np.where(np.isnan(V), 0.0, V), os_k - 1, axis=1)[:, os_k - 1]
+        Z[nan] = np.nan
+    else:
+        tot = np.zeros(X.size)
+        for q in range(N):
+            tot = tot + V[:, q]
+        Z = tot / N
+        if mode in (2, 3):
+            L = np.zeros(X.size)
+            R = np.zeros(X.size)
+            for q in range(n_ref):
+                L = L + V[:, q]
+                R = R + V[:, n_ref + q]
+            L, R = L / n_ref, R / n_ref
+            side = np.maximum(L, R) if mode == 2 else np.minimum(L, R)
+            Z = np.where(sided, side, Z)
+            a = np.where(sided, alpha, alpha_ca)
+            scale = np.where(sided, et, 1.0)
+    Z = np.where(ok, Z, np.nan)
+    thr = a * Z
+    with np.errstate(invalid="ignore"):
+        hits = ok & (X > thr)
+    return thr, Z / scale, hits
+
+
+def _gamma_beta(nb: int, looks: float, rho2) -> float:
+    """beta with P(Gamma(s) / s > beta) = 1e-3, s the equivalent independent
+    looks over nb contiguous bins (src/cfar.c:253-268), by scipy's inverse."""
+    from scipy.special import gammainccinv
+    pairs = sum((nb - m) * rho2[m - 1] for m in range(1, 5) if m < nb)
+    s = nb / (1.0 + 2.0 * pairs / nb) * looks
+    return float(gammainccinv(s, 1e-3) / s)
+
+
+class Cfar:
+    """The reference's CFAR handle on the CPU (src/cfar.c:270-819). alpha,
+    alpha_ca and et are given (the reference's values for the
+    configuration)."""
+
+    def __init__(self, nbins, cfg, alpha, alpha_ca, et):
+        c = dict(CFAR_DEFAULTS)
+        c.update(cfg)
+        self.c = c
+        self.nb = nbins
+        self.N = 2 * c["n_ref"]
+        if c["mode"] == 1 and c["os_k"] == 0:
+            c["os_k"] = (3 * self.N + 3) // 4
+        self.alpha, self.alpha_ca, self.et = alpha, alpha_ca, et
+        self.mask = np.zeros(nbins, bool)
+        self.cells, self.ok, self.sided = cfar_tables(nbins, c["n_ref"], c["n_guard"], None)
+        self.k = c["k_frames"]
+        self.hist = np.zeros((self.k, nbins))
+        self.ht = np.zeros(self.k)
+        self.seen = self.frame = 0
+        self.tracks = []
+        self.next_id = 1
+        self.beta = {}
+        self.st = dict(frames=0, warmup=0, cells_tested=0, cell_hits=0, untestable=0, opened=0, closed=0, updates=0,
+                       dropped=0, resets=0, tracks=0)
+        self.last = None
+
+    def set_mask(self, mask):
+        self.mask = np.zeros(self.nb, bool) if mask is None else np.asarray(mask) != 0
+        self.cells, self.ok, self.sided = cfar_tables(self.nb, self.c["n_ref"], self.c["n_guard"], self.mask)
+
+    def reset(self):
+        self.tracks = []
+        self.seen = self.frame = 0
+        self.hist[:] = 0.0
+        self.last = None
+        self.st["resets"] += 1
+
+    def _band(self, raw, noise, lo, hi):
+        sel = np.flatnonzero(self.ok[lo:hi + 1]) + lo
+        if not sel.size:
+            return False
+        sr = sz = 0.0
+        for b in sel:
+            sr += raw[b]
+            sz += noise[b]
+        if sel.size not in self.beta:
+            self.beta[sel.size] = _gamma_beta(sel.size, self.c["looks"], self.c["rho2"])
+        return sr > self.beta[sel.size] * sz
+
+    def _flags(self, t):
+        f = t["flags"]
+        sp = slice(t["ulo"], t["uhi"] + 1)
+        if (~self.sided[sp]).any() or (~self.ok[sp]).any():
+            f |= 4
+        m = self.c["merge_gap"] + 1
+        if self.mask[max(t["ulo"] - m, 0):min(t["uhi"] + m + 1, self.nb)].any():
+            f |= 8
+        return f
+
+    def _event(self, t, typ, t1, extra=0):
+        e = np.zeros(1, CFAR_EVENT)
+        e["type"], e["id"], e["flags"] = typ, t["id"], self._flags(t) | extra
+        e["lo_bin"], e["hi_bin"], e["n_hits"] = t["ulo"], t["uhi"], t["n_hits"]
+        e["t0"], e["t1"] = t["t0"], t1
+        with np.errstate(invalid="ignore"):
+            e["fc_bin"] = t["sbe"] / t["se"] if t["se"] > 0 else 0.5 * (t["ulo"] + t["uhi"])
+        lv = np.float32(10.0 * math.log10(t["px"])) if t["px"] > 0 else np.float32(-np.inf)
+        nz = np.float32(10.0 * math.log10(t["pn"])) if t["pn"] > 0 else np.float32(-np.inf)
+        e["level_db"], e["noise_db"] = lv, nz
+        e["snr_db"] = lv - nz if t["pn"] > 0 else np.inf
+        e["band_snr_db"] = np.float32(10.0 * math.log10(t["band"])) if t["band"] > 0 else -np.inf
+        return e
+
+    def _close(self, t, extra=0):
+        k = self.k
+        if k == 1:
+            t1 = t["t_hit_end"]
+        elif t["last_raw"] >= t["last_hit"] - (k - 1):
+            t1 = t["t_raw_end"]
+        else:
+            t1 = t["t_hit_end"] - 0.5 * (k - 1) * t["dt_hit"]
+            t["flags"] |= 2
+        self.st["closed"] += 1
+        return self._event(t, 3, t1, extra)
+
+    def _frame(self, row, tf, dt):
+        c, nb, k = self.c, self.nb, self.k
+        self.st["frames"] += 1
+        F = self.frame
+        slot = F % k
+        v = np.maximum(row.astype(np.float64), -300.0)
+        v = np.where(np.isnan(row), np.nan, v)
+        self.hist[slot] = np.exp(v * 0.23025850929940458)
+        self.ht[slot] = tf
+        self.seen += 1
+        self.frame += 1
+        if self.seen < k:
+            self.st["warmup"] += 1
+            return []
+        raw = self.hist[slot]
+        tot = np.zeros(nb)
+        for r in range(k):
+            tot = tot + self.hist[(F + 1 + r) % k]
+        X = tot / k
+        thr, noise, hits = cfar_test(X, self.cells, self.ok, self.sided, c["mode"], c["n_ref"], c["os_k"],
+                                     self.alpha, self.alpha_ca, self.et)
+        self.st["cells_tested"] += int(self.ok.sum())
+        self.st["cell_hits"] += int(hits.sum())
+        self.st["untestable"] += int((~self.ok & ~self.mask).sum())
+        self.last = (X, thr, hits)
+        # blobs: hit bins no more than merge_gap non-hit bins apart, min_bins wide
+        g = c["merge_gap"]
+        idx = np.flatnonzero(hits)
+        blobs = []
+        if idx.size:
+            cuts = np.flatnonzero(np.diff(idx) - 1 > g) + 1
+            for grp in np.split(idx, cuts):
+                lo, hi = int(grp[0]), int(grp[-1])
+                if hi - lo + 1 < c["min_bins"]:
+                    continue
+                b = dict(lo=lo, hi=hi, sbe=0.0, se=0.0, sx=0.0, sn=0.0, track=-1)
+                pk = -1
+                for q in grp:
+                    e = max(X[q] - noise[q], 0.0) if not np.isnan(X[q] - noise[q]) else X[q] - noise[q]
+                    b["sbe"] += q * e
+                    b["se"] += e
+                    b["sx"] += X[q]
+                    b["sn"] += noise[q]
+                    if pk < 0 or X[q] > X[pk]:
+                        pk = q
+                b["px"], b["pn"] = X[pk], noise[pk]
+                blobs.append(b)
+        # association to the existing track overlapping most (ties: the older)
+        existing = len(self.tracks)
+        for t in self.tracks:
+            t["has_obs"] = False
+        m = g + 1
+        for b in blobs:
+            best, bov = -1, 0
+            for ti in range(existing):
+                t = self.tracks[ti]
+                if b["lo"] <= t["cur_hi"] + m and b["hi"] >= t["cur_lo"] - m:
+                    ov = min(b["hi"], t["cur_hi"]) - max(b["lo"], t["cur_lo"]) + 1
+                    if best < 0 or ov > bov:
+                        best, bov = ti, ov
+            b["track"] = best
+        for b in blobs:
+            if b["track"] >= 0:
+                continue
+            if len(self.tracks) >= c["max_tracks"]:
+                self.st["dropped"] += 1
+                continue
+            self.tracks.append(dict(id=self.next_id, state=0, hist=0, misses=0, is_new=True, n_hits=0,
+                                    cur_lo=b["lo"], cur_hi=b["hi"], ulo=b["lo"], uhi=b["hi"], sbe=0.0, se=0.0,
+                                    px=-1.0, pn=0.0, band=0.0, flags=0, t0=0.0, last_hit=-1, last_raw=-1,
+                                    t_hit_end=0.0, dt_hit=dt, t_raw_end=0.0, has_obs=False))
+            self.next_id = (self.next_id + 1) & 0xFFFFFFFF or 1
+            b["track"] = len(self.tracks) - 1
+        for b in blobs:
+            if b["track"] < 0:
+                continue
+            t = self.tracks[b["track"]]
+            if not t["has_obs"]:
+                t.update(has_obs=True, olo=b["lo"], ohi=b["hi"], fsx=0.0, fsn=0.0)
+            else:
+                t["olo"], t["ohi"] = min(t["olo"], b["lo"]), max(t["ohi"], b["hi"])
+            t["sbe"] += b["sbe"]
+            t["se"] += b["se"]
+            t["fsx"] += b["sx"]
+            t["fsn"] += b["sn"]
+            if b["px"] > t["px"]:
+                t["px"], t["pn"] = b["px"], b["pn"]
+        # each track in id order: window, state, event
+        wmask = (1 << c["open_n"]) - 1
+        events, keep = [], []
+        for t in self.tracks:
+            grew = False
+            if t["has_obs"]:
+                lo, hi = t["olo"], t["ohi"]
+                t["hist"] = ((t["hist"] << 1) | 1) & wmask
+                t["misses"] = 0
+                t["n_hits"] += 1
+                t["cur_lo"], t["cur_hi"] = lo, hi
+                if not t["is_new"]:
+                    grew = lo < t["ulo"] or hi > t["uhi"]
+                t["ulo"], t["uhi"] = min(t["ulo"], lo), max(t["uhi"], hi)
+                if t["fsn"] > 0 and t["fsx"] / t["fsn"] > t["band"]:
+                    t["band"] = t["fsx"] / t["fsn"]
+                t["last_hit"] = F
+                t["t_hit_end"] = tf + dt
+                t["dt_hit"] = dt
+                if t["is_new"]:
+                    t["is_new"] = False
+                    if k == 1:
+                        t["t0"] = tf
+                    else:
+                        for r in range(k):
+                            s = (F + 1 + r) % k
+                            if self._band(self.hist[s], noise, lo, hi):
+                                t["t0"] = self.ht[s]
+                                break
+                        else:
+                            t["t0"] = tf - 0.5 * (k - 1) * dt
+                            t["flags"] |= 1
+            else:
+                t["hist"] = (t["hist"] << 1) & wmask
+                t["misses"] += 1
+            t["has_obs"] = False
+            if k > 1 and self._band(raw, noise, t["cur_lo"], t["cur_hi"]):
+                t["last_raw"] = F
+                t["t_raw_end"] = tf + dt
+            if t["state"] == 0:
+                if bin(t["hist"]).count("1") >= c["open_m"]:
+                    t["state"] = 1
+                    self.st["opened"] += 1
+                    events.append(self._event(t, 1, np.nan))
+                elif t["hist"] == 0:
+                    continue
+            elif t["misses"] >= c["close_k"]:
+                events.append(self._close(t))
+                continue
+            elif grew:
+                self.st["updates"] += 1
+                events.append(self._event(t, 2, np.nan))
+            keep.append(t)
+        self.tracks = keep
+        return events
+
+    def process(self, frames, t0, dt):
+        frames = np.asarray(frames, np.float32).reshape(-1, self.nb)
+        ev = []
+        for j in range(frames.shape[0]):
+            ev += self._frame(frames[j], t0 + j * dt, dt)
+        return np.concatenate(ev) if ev else np.zeros(0, CFAR_EVENT)
+
+    def flush(self):
+        ev = [self._close(t, 16) for t in self.tracks if t["state"] == 1]
+        self.tracks = []
+        return np.concatenate(ev) if ev else np.zeros(0, CFAR_EVENT)
+
+    def stats(self):
+        st = dict(self.st)
+        st["tracks"] = len(self.tracks)
+        return st
 
 
 def rel_err(a, b) -> float:

@@ -27,6 +27,18 @@ class NcoState(C.Structure):
     _fields_ = [("phase", C.c_double), ("step", C.c_double)]
 
 
+class CfarCfg(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("n_ref", C.c_int32), ("n_guard", C.c_int32), ("k_frames", C.c_int32),
+                ("pfa", C.c_double), ("looks", C.c_double), ("os_k", C.c_int32), ("open_m", C.c_int32),
+                ("open_n", C.c_int32), ("close_k", C.c_int32), ("merge_gap", C.c_int32), ("min_bins", C.c_int32),
+                ("max_tracks", C.c_int32), ("reserved", C.c_int32), ("rho2", C.c_double * 4)]
+
+
+CFAR_EVENT = np.dtype([("t0", "<f8"), ("t1", "<f8"), ("fc_bin", "<f8"), ("type", "<i4"), ("id", "<u4"),
+                       ("flags", "<u4"), ("lo_bin", "<i4"), ("hi_bin", "<i4"), ("n_hits", "<i4"),
+                       ("level_db", "<f4"), ("noise_db", "<f4"), ("snr_db", "<f4"), ("band_snr_db", "<f4")])
+
+
 def available() -> bool:
     return REF_SO.exists()
 
@@ -80,6 +92,17 @@ class Ref:
             "atkdsp_cfar_alpha": (C.c_int, [C.c_int, C.c_int, d, d, C.c_int, vp, P(d), P(d)]),
             "atkdsp_cfar_n_eff": (d, [C.c_int, vp]),
             "atkdsp_cfar_window_rho2": (C.c_int, [fp, sz, P(d)]),
+            "atkdsp_cfar_create": (vp, [sz, P(CfarCfg)]),
+            "atkdsp_cfar_destroy": (None, [vp]),
+            "atkdsp_cfar_reset": (None, [vp]),
+            "atkdsp_cfar_set_mask": (C.c_int, [vp, vp]),
+            "atkdsp_cfar_events_max": (sz, [vp, sz]),
+            "atkdsp_cfar_process": (C.c_ssize_t, [vp, fp, sz, d, d, vp, sz]),
+            "atkdsp_cfar_flush": (C.c_ssize_t, [vp, vp, sz]),
+            "atkdsp_cfar_last": (C.c_int, [vp, fp, fp, vp]),
+            "atkdsp_cfar_info": (C.c_int, [vp, P(d), P(d), P(d), P(CfarCfg)]),
+            "atkdsp_cfar_describe": (C.c_int, [vp, C.c_char_p, sz]),
+            "atkdsp_cfar_stats": (C.c_int, [vp, vp]),
             "atkdsp_detect_channels": (C.c_ssize_t, [fp, sz, C.c_float, C.c_float, C.c_int, C.c_int, fp,
                                                      P(ChanC), sz]),
         }
@@ -237,6 +260,55 @@ class Ref:
         buf = (C.c_uint8 * 96)()
         self.L.atkdsp_cfar_defaults(C.cast(buf, C.c_void_p))
         return bytes(buf)
+
+    def cfar_run(self, nbins, cfg: dict, ops):
+        """Runs a CFAR handle through ops: ("process", frames (nf x nbins
+        float32), t0, dt), ("mask", bytes or None), ("reset",), ("flush",).
+        Returns (events per op, the 11 counters, last() as (avg, thr, hits) or
+        None, (alpha, alpha_ca, et), describe text); None if create fails."""
+        c = CfarCfg()
+        self.L.atkdsp_cfar_defaults(C.cast(C.pointer(c), C.c_void_p))
+        for k, v in cfg.items():
+            if k == "rho2":
+                c.rho2[:] = list(v)
+            else:
+                setattr(c, k, v)
+        h = self.L.atkdsp_cfar_create(nbins, C.byref(c))
+        if not h:
+            return None
+        try:
+            out = []
+            for op in ops:
+                if op[0] == "process":
+                    fr = np.ascontiguousarray(op[1], np.float32)
+                    nf = fr.size // nbins
+                    ev = np.zeros(max(self.L.atkdsp_cfar_events_max(h, nf), 1), CFAR_EVENT)
+                    n = self.L.atkdsp_cfar_process(h, self._f(fr), nf, op[2], op[3], ev.ctypes.data, ev.size)
+                    assert n >= 0, n
+                    out.append(ev[:n].copy())
+                elif op[0] == "flush":
+                    ev = np.zeros(65536, CFAR_EVENT)
+                    n = self.L.atkdsp_cfar_flush(h, ev.ctypes.data, ev.size)
+                    assert n >= 0, n
+                    out.append(ev[:n].copy())
+                elif op[0] == "mask":
+                    m = None if op[1] is None else np.ascontiguousarray(op[1], np.uint8)
+                    assert self.L.atkdsp_cfar_set_mask(h, None if m is None else m.ctypes.data) == 0
+                    out.append(np.zeros(0, CFAR_EVENT))
+                elif op[0] == "reset":
+                    self.L.atkdsp_cfar_reset(h)
+                    out.append(np.zeros(0, CFAR_EVENT))
+            st = np.zeros(11, np.uint64)
+            assert self.L.atkdsp_cfar_stats(h, st.ctypes.data) == 0
+            a, t, hh = np.empty(nbins, np.float32), np.empty(nbins, np.float32), np.empty(nbins, np.uint8)
+            last = (a, t, hh) if self.L.atkdsp_cfar_last(h, self._f(a), self._f(t), hh.ctypes.data) == 0 else None
+            al, aca, et = C.c_double(), C.c_double(), C.c_double()
+            self.L.atkdsp_cfar_info(h, C.byref(al), C.byref(aca), C.byref(et), None)
+            buf = C.create_string_buffer(512)
+            self.L.atkdsp_cfar_describe(h, buf, 512)
+            return out, st, last, (al.value, aca.value, et.value), buf.value.decode()
+        finally:
+            self.L.atkdsp_cfar_destroy(h)
 
     def pfb(self, fs, nchan, os_, K, atten, max_nel=1):
         return RefPfb(self, fs, nchan, os_, K, atten, max_nel)

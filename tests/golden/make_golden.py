@@ -355,6 +355,104 @@ def cfar_golden():
          defaults=np.frombuffer(r.cfar_defaults_bytes(), np.uint8), **rho2)
 
 
+HANN = [4.0 / 9.0, 1.0 / 36.0, 0.0, 0.0]
+
+
+def cfar_scene(seed, nbins, nframes, looks, bursts, floor_db=-100.0, ripple_db=6.0):
+    """Spectrum frames in dB: gamma(looks) noise power of unit mean over a
+    slowly rippling floor, plus bursts (f0, f1, b0, b1, snr_db) of the same
+    statistics; stored as float16 so the fixture stays small (the frames are
+    defined as the float16 values)."""
+    rng = np.random.default_rng(seed)
+    p = rng.gamma(looks, 1.0 / looks, (nframes, nbins))
+    for f0, f1, b0, b1, snr in bursts:
+        f1 = min(f1, nframes)
+        if f0 >= f1:
+            continue
+        p[f0:f1, b0:b1] += 10.0 ** (snr / 10.0) * rng.gamma(looks, 1.0 / looks, (f1 - f0, b1 - b0))
+    floor = floor_db + ripple_db * np.sin(np.arange(nbins) * (2.0 * np.pi / nbins) * 3.0)
+    return (10.0 * np.log10(p) + floor).astype(np.float16)
+
+
+def cfar_scenarios():
+    """(name, nbins, config, frames, ops, masks): ops index rows of frames"""
+    out = []
+    B = [(5, 30, 200, 206, 15.0), (10, 12, 300, 301, 18.0), (20, 28, 500, 504, 14.0), (28, 40, 500, 511, 14.0),
+         (8, 50, 2, 7, 16.0), (15, 45, 700, 704, 13.0), (15, 45, 706, 709, 13.0), (40, 60, 900, 930, 12.0),
+         (0, 60, 1015, 1024, 20.0)]
+    fr = cfar_scene(101, 1024, 60, 1.0, B)
+    out.append(("ca_default", 1024, {}, fr,
+                [("process", 0, 1), ("process", 1, 7), ("process", 8, 20), ("process", 28, 32), ("flush",)], {}))
+    fr = cfar_scene(102, 1024, 60, 1.0, B)
+    out.append(("os_k4", 1024, dict(mode=1, k_frames=4, merge_gap=2, min_bins=2, open_m=3, open_n=4, close_k=4), fr,
+                [("process", 0, 3), ("process", 3, 50), ("process", 53, 7), ("flush",)], {}))
+    fr = cfar_scene(103, 1024, 70, 1.0, B + [(10, 50, 420, 425, 15.0), (10, 50, 395, 399, 15.0)])
+    m1 = np.zeros(1024, np.uint8)
+    m1[400:420] = 1
+    m1[0:10] = 1
+    m2 = np.zeros(1024, np.uint8)
+    m2[690:702] = 1
+    out.append(("go_mask", 1024, dict(mode=2, k_frames=2), fr,
+                [("mask", "m1"), ("process", 0, 30), ("mask", "m2"), ("process", 30, 30), ("mask", None),
+                 ("process", 60, 10), ("flush",)], {"m1": m1, "m2": m2}))
+    fr = cfar_scene(104, 512, 48, 1.0, [(4, 40, 100, 110, 12.0), (20, 30, 300, 302, 15.0)])
+    out.append(("so_busy", 512, dict(mode=3, pfa=1e-2, max_tracks=8, open_m=1, open_n=1, close_k=1), fr,
+                [("process", i, 16) for i in range(0, 48, 16)] + [("flush",)], {}))
+    fr = cfar_scene(105, 2048, 50, 4.0, [(5, 40, 1000, 1012, 8.0), (12, 20, 1500, 1501, 10.0),
+                                         (30, 48, 60, 70, 9.0)])
+    out.append(("ca_hann_looks", 2048, dict(rho2=HANN, looks=4.0, k_frames=3, open_m=3, open_n=5, close_k=5), fr,
+                [("process", 0, 25), ("reset",), ("process", 25, 25), ("flush",)], {}))
+    fr = cfar_scene(106, 1024, 40, 1.0, B).astype(np.float32)
+    fr[5:9, 150:160] = np.nan
+    fr[12:14, 205:207] = np.nan
+    fr[20:22, 600:640] = -np.inf
+    fr[25, 800:802] = 250.0
+    fr[30:33, 350] = np.inf
+    fr = fr.astype(np.float16)
+    out.append(("ca_nan_inf", 1024, dict(k_frames=2), fr, [("process", 0, 40), ("flush",)], {}))
+    fr = cfar_scene(107, 64, 80, 1.0, [(10, 60, 30, 32, 28.0), (20, 70, 0, 2, 28.0), (5, 15, 50, 51, 30.0)])
+    m3 = np.ones(64, np.uint8)
+    m3[[0, 1, 33]] = 0
+    out.append(("os_small", 64, dict(mode=1, n_ref=2, n_guard=2, os_k=3, pfa=1e-3, merge_gap=0), fr,
+                [("process", 0, 40), ("mask", "m3"), ("process", 40, 5), ("mask", None), ("process", 45, 35),
+                 ("flush",)], {"m3": m3}))
+    return out
+
+
+def cfar_events_golden():
+    """The reference's CFAR detector and tracker over the scenarios above:
+    every event of every call, the counters, the last frame's average /
+    threshold / hits, alpha / alpha_ca / E[T] and the description."""
+    import json
+    r = Ref()
+    arrs, meta = {}, []
+    for name, nbins, cfg, fr, ops, masks in cfar_scenarios():
+        f32 = fr.astype(np.float32)
+        calls = []
+        for op in ops:
+            if op[0] == "process":
+                calls.append(("process", f32[op[1]:op[1] + op[2]], op[1] * 0.01, 0.01))
+            elif op[0] == "mask":
+                calls.append(("mask", None if op[1] is None else masks[op[1]]))
+            else:
+                calls.append(op)
+        res = r.cfar_run(nbins, cfg, calls)
+        assert res is not None, name
+        evs, st, last, info, desc = res
+        arrs[f"{name}_frames"] = fr
+        for k, v in masks.items():
+            arrs[f"{name}_{k}"] = v
+        arrs[f"{name}_events"] = np.concatenate(evs)
+        arrs[f"{name}_evop"] = np.concatenate([np.full(len(e), i, np.int32) for i, e in enumerate(evs)])
+        arrs[f"{name}_stats"] = st
+        arrs[f"{name}_last"] = np.stack([last[0], last[1], last[2].astype(np.float32)])
+        arrs[f"{name}_info"] = np.array(info)
+        meta.append(dict(name=name, nbins=nbins, cfg=cfg, ops=ops, describe=desc))
+        print(name, {k: int(v) for k, v in zip(("frames", "warmup", "tested", "hits", "untestable", "opened",
+                                                   "closed", "updates", "dropped", "resets", "tracks"), st)})
+    save("cfar_events", meta=np.array(json.dumps(meta)), **arrs)
+
+
 if __name__ == "__main__":
     import sys
     if sys.argv[1:] == ["nco"]:
@@ -365,9 +463,12 @@ if __name__ == "__main__":
         detect_golden()
     elif sys.argv[1:] == ["cfar"]:
         cfar_golden()
+    elif sys.argv[1:] == ["cfar_events"]:
+        cfar_events_golden()
     else:
         main()
         nco_golden()
         osx_golden()
         detect_golden()
         cfar_golden()
+        cfar_events_golden()
