@@ -597,13 +597,13 @@ class Cfar:
         if f.size % self.nbins:
             raise AtkDspError("frames_db must hold whole frames of nbins values")
         nf = f.size // self.nbins
-        ev = np.zeros(max(self.events_max(nf), 1), CFAR_EVENT_DTYPE)
+        ev = np.empty(max(self.events_max(nf), 1), CFAR_EVENT_DTYPE)
         n = _check(int(self._lib.atkdsp_cfar_process(self._p, _fp(f), nf, float(t0), float(dt), ev.ctypes.data,
                                                      ev.size if nf else 0)), "cfar_process")
         return ev[:n].copy()
 
     def flush(self) -> np.ndarray:
-        ev = np.zeros(65536, CFAR_EVENT_DTYPE)
+        ev = np.empty(65536, CFAR_EVENT_DTYPE)
         n = _check(int(self._lib.atkdsp_cfar_flush(self._p, ev.ctypes.data, ev.size)), "cfar_flush")
         return ev[:n].copy()
 

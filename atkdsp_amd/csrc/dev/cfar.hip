@@ -2,7 +2,7 @@
 // on gfx950: for each frame of a call and each bin, in FP64 as the reference,
 //   raw    power from dB, 10^(v/10), values below -300 dB read as -300;
 //   X      the mean of the raw power of the last k_frames frames, summed
-//          oldest to newest;
+//          oldest to newest (with one frame, X is the raw row itself);
 //   T      the reference statistic over the bin's N reference cells, taken
 //          from a table (left cells nearest first, right nearest first, then
 //          any borrowed from the far side): CA their mean in table order; GO /
@@ -10,8 +10,11 @@
 //          borrowed falls back to CA with CA's alpha); OS the os_k-th
 //          smallest; a NaN cell makes T NaN;
 //   thr    alpha T, noise T / E[T], hit X > thr.
-// One thread per (frame, bin); a frame's raw rows sit after the k-1 rows of
-// the frames before it, so a call needs no history kernel.
+// One thread per (frame, bin). A frame's raw rows sit after the k-1 rows of
+// the frames before it, so a call needs no history kernel. OS ranks its
+// cells by counting (each cell's rank = the cells below it, ties by
+// position); with N <= 64 the cells sit in LDS, 64 per thread, else in
+// per-lane scratch.
 #include <hip/hip_runtime.h>
 #include "atkgpu_dev.h"
 
@@ -20,6 +23,8 @@
 namespace {
 
 constexpr double LN10_10 = 0.23025850929940458;   // ln(10) / 10
+constexpr int TB = 128;                            // threads (bins) per block
+constexpr int OS_LDS_N = 64;
 
 __global__ void k_cfar_raw(const float *__restrict__ db, size_t n, double *__restrict__ raw) {
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -29,24 +34,29 @@ __global__ void k_cfar_raw(const float *__restrict__ db, size_t n, double *__res
     raw[i] = exp(v * LN10_10);
 }
 
-__global__ void k_cfar_test(const double *__restrict__ raw, int nbins, int nframes, int k, int mode, int N,
-                            int os_k, double alpha, double alpha_ca, double et, const int *__restrict__ cells,
-                            const unsigned char *__restrict__ ok, const unsigned char *__restrict__ sided,
-                            double *__restrict__ X, double *__restrict__ thr, double *__restrict__ noise,
-                            unsigned char *__restrict__ hits) {
+// X of frame j = rows j .. j+k-1 of raw, oldest first
+__global__ void k_cfar_avg(const double *__restrict__ raw, int nbins, int nframes, int k, double *__restrict__ X) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (i >= nbins || j >= nframes) return;
-    // X of every bin of frame j (rows j .. j+k-1 of raw, oldest first)
-    const double *r0 = raw + (size_t)j * nbins;
-    auto avg = [&](int b) {
-        double s = 0.0;
-        for (int r = 0; r < k; ++r) s += r0[(size_t)r * nbins + b];
-        return s / k;
-    };
-    const double x = avg(i);
+    const double *r0 = raw + (size_t)j * nbins + i;
+    double s = 0.0;
+    for (int r = 0; r < k; ++r) s += r0[(size_t)r * nbins];
+    X[(size_t)j * nbins + i] = s / k;
+}
+
+// OSF: 0 CA / GO / SO only, 1 OS with its cells in LDS, 2 OS in scratch
+template <int OSF>
+__global__ __launch_bounds__(TB) void k_cfar_test(const double *__restrict__ Xall, int nbins, int nframes, int mode,
+                                                  int N, int os_k, double alpha, double alpha_ca, double et,
+                                                  const int *__restrict__ cells, const unsigned char *__restrict__ ok,
+                                                  const unsigned char *__restrict__ sided, double *__restrict__ thr,
+                                                  double *__restrict__ noise, unsigned char *__restrict__ hits) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= nbins || j >= nframes) return;
+    const double *X = Xall + (size_t)j * nbins;
     const size_t o = (size_t)j * nbins + i;
-    X[o] = x;
     if (!ok[i]) {
         thr[o] = __builtin_nan("");
         noise[o] = __builtin_nan("");
@@ -55,42 +65,53 @@ __global__ void k_cfar_test(const double *__restrict__ raw, int nbins, int nfram
     }
     const int *c = cells + (size_t)i * N;
     double z, a = alpha, scale = et;
-    if (mode == 1) {
-        // the os_k-th smallest: count, for each cell, the cells below it (ties
-        // broken by position); a NaN cell makes T NaN
-        double v[512];
+    if constexpr (OSF != 0) {
+        double *v;
+        double vs_scratch[OSF == 2 ? 512 : 1];
+        if constexpr (OSF == 1) {
+            __shared__ double vs[OS_LDS_N * TB];
+            v = vs + threadIdx.x;                      // cell q at v[q * TB]
+        } else {
+            v = vs_scratch;
+        }
+        constexpr int STRIDE = OSF == 1 ? TB : 1;
         bool nan = false;
         for (int q = 0; q < N; ++q) {
-            v[q] = avg(c[q]);
-            nan = nan || (v[q] != v[q]);
+            const double x = X[c[q]];
+            v[q * STRIDE] = x;
+            nan = nan || (x != x);
         }
         z = __builtin_nan("");
         if (!nan) {
             for (int q = 0; q < N; ++q) {
+                const double vq = v[q * STRIDE];
                 int below = 0;
-                for (int p = 0; p < N; ++p) below += (v[p] < v[q]) || (v[p] == v[q] && p < q);
-                if (below == os_k - 1) { z = v[q]; break; }
+                for (int p = 0; p < N; ++p) {
+                    const double vp = v[p * STRIDE];
+                    below += (vp < vq) || (vp == vq && p < q);
+                }
+                if (below == os_k - 1) { z = vq; break; }
             }
         }
     } else if ((mode == 2 || mode == 3) && sided[i]) {
         const int n = N / 2;
         double L = 0.0, R = 0.0;
-        for (int q = 0; q < n; ++q) L += avg(c[q]);
-        for (int q = n; q < N; ++q) R += avg(c[q]);
+        for (int q = 0; q < n; ++q) L += X[c[q]];
+        for (int q = n; q < N; ++q) R += X[c[q]];
         L /= n;
         R /= n;
         if (L != L || R != R) z = __builtin_nan("");
         else z = mode == 2 ? (L > R ? L : R) : (L < R ? L : R);
     } else {
         double s = 0.0;
-        for (int q = 0; q < N; ++q) s += avg(c[q]);
+        for (int q = 0; q < N; ++q) s += X[c[q]];
         z = s / N;
         if (mode == 2 || mode == 3) { a = alpha_ca; scale = 1.0; }
     }
     const double t = a * z;
     thr[o] = t;
     noise[o] = z / scale;
-    hits[o] = x > t ? 1 : 0;
+    hits[o] = X[i] > t ? 1 : 0;
 }
 
 }  // namespace
@@ -105,9 +126,17 @@ extern "C" int atkgpu_cfar_test(void *s, const double *raw, int nbins, int nfram
                                 double alpha, double alpha_ca, double et, const int *cells, const unsigned char *ok,
                                 const unsigned char *sided, double *X, double *thr, double *noise,
                                 unsigned char *hits) {
-    if (!raw || nbins < 1 || nframes < 1 || k < 1 || N < 2 || N > 512 || !cells || !ok || !sided) return -1;
-    hipLaunchKernelGGL(k_cfar_test, dim3((unsigned)((nbins + 127) / 128), (unsigned)nframes), dim3(128), 0,
-                       (hipStream_t)s, raw, nbins, nframes, k, mode, N, os_k, alpha, alpha_ca, et, cells, ok, sided,
-                       X, thr, noise, hits);
+    if (!raw || nbins < 1 || nframes < 1 || nframes > 65535 || k < 1 || N < 2 || N > 512 || (N & 1) || !cells ||
+        !ok || !sided || !thr || !noise || !hits || (k > 1 && !X) || (mode == 1 && (os_k < 1 || os_k > N)))
+        return -1;
+    const dim3 grid((unsigned)((nbins + TB - 1) / TB), (unsigned)nframes);
+    const double *Xin = raw;
+    if (k > 1) {
+        hipLaunchKernelGGL(k_cfar_avg, grid, dim3(TB), 0, (hipStream_t)s, raw, nbins, nframes, k, X);
+        Xin = X;
+    }
+    auto kern = mode != 1 ? k_cfar_test<0> : N <= OS_LDS_N ? k_cfar_test<1> : k_cfar_test<2>;
+    hipLaunchKernelGGL(kern, grid, dim3(TB), 0, (hipStream_t)s, Xin, nbins, nframes, mode, N, os_k, alpha, alpha_ca,
+                       et, cells, ok, sided, thr, noise, hits);
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }

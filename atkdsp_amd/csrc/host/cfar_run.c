@@ -72,9 +72,12 @@ struct atkdsp_cfar {
     float *d_db;
     double *d_raw, *d_hist, *d_X, *d_thr, *d_noise;
     size_t slab;
-    /* host copies of a slab's results */
+    /* host copies of a slab's results (pinned): the average, noise and hits
+     * of every frame, the power of every frame when k_frames > 1, the
+     * threshold of the slab's last frame */
     double *X, *thr, *noise, *raw;
     uint8_t *hits;
+    size_t n_ok, n_untestable;   /* bins tested / unmasked but lacking cells, per frame */
     /* the last k frames' power and times, ring by frame number */
     double *ring, *ring_t, *ring_dt;
     int64_t seen, frame;
@@ -143,6 +146,11 @@ static void build_tables(atkdsp_cfar *c) {
         c->ok[i] = 1;
         c->sided[i] = nbw == 0;
     }
+    c->n_ok = c->n_untestable = 0;
+    for (long i = 0; i < nb; ++i) {
+        c->n_ok += c->ok[i];
+        c->n_untestable += !c->ok[i] && !c->mask[i];
+    }
 }
 
 static int upload_tables(atkdsp_cfar *c) {
@@ -159,7 +167,7 @@ void atkdsp_cfar_destroy(atkdsp_cfar *c) {
     atkgpu_dfree(c->d_thr); atkgpu_dfree(c->d_noise);
     if (c->stream) atkgpu_stream_destroy(c->stream);
     free(c->mask); free(c->ok); free(c->sided); free(c->cells);
-    free(c->X); free(c->thr); free(c->noise); free(c->raw); free(c->hits);
+    atkgpu_hfree(c->X); atkgpu_hfree(c->thr); atkgpu_hfree(c->noise); atkgpu_hfree(c->raw); atkgpu_hfree(c->hits);
     free(c->ring); free(c->ring_t); free(c->ring_dt);
     free(c->last_X); free(c->last_thr); free(c->last_hits);
     free(c->beta); free(c->tracks); free(c->blobs);
@@ -200,11 +208,11 @@ atkdsp_cfar *atkdsp_cfar_create(size_t nbins, const atkdsp_cfar_config *cfg) {
     c->ok = (uint8_t *)calloc(nb, 1);
     c->sided = (uint8_t *)calloc(nb, 1);
     c->cells = (int32_t *)calloc(nb * (size_t)c->N, sizeof(int32_t));
-    c->X = (double *)malloc(slab * nb * sizeof(double));
-    c->thr = (double *)malloc(slab * nb * sizeof(double));
-    c->noise = (double *)malloc(slab * nb * sizeof(double));
-    c->raw = (double *)malloc(slab * nb * sizeof(double));
-    c->hits = (uint8_t *)malloc(slab * nb);
+    c->X = (double *)atkgpu_hmalloc(slab * nb * sizeof(double));
+    c->thr = (double *)atkgpu_hmalloc(nb * sizeof(double));
+    c->noise = (double *)atkgpu_hmalloc(slab * nb * sizeof(double));
+    c->raw = k > 1 ? (double *)atkgpu_hmalloc(slab * nb * sizeof(double)) : NULL;
+    c->hits = (uint8_t *)atkgpu_hmalloc(slab * nb + 8);
     c->ring = (double *)calloc(k * nb, sizeof(double));
     c->ring_t = (double *)calloc(k, sizeof(double));
     c->ring_dt = (double *)calloc(k, sizeof(double));
@@ -214,7 +222,7 @@ atkdsp_cfar *atkdsp_cfar_create(size_t nbins, const atkdsp_cfar_config *cfg) {
     c->beta = (double *)malloc((nb + 1) * sizeof(double));
     c->tracks = (cfar_track *)calloc((size_t)c->cfg.max_tracks, sizeof(cfar_track));
     c->blobs = (cfar_blob *)calloc(nb, sizeof(cfar_blob));
-    int ok = c->mask && c->ok && c->sided && c->cells && c->X && c->thr && c->noise && c->raw && c->hits &&
+    int ok = c->mask && c->ok && c->sided && c->cells && c->X && c->thr && c->noise && (c->raw || k == 1) && c->hits &&
              c->ring && c->ring_t && c->ring_dt && c->last_X && c->last_thr && c->last_hits && c->beta &&
              c->tracks && c->blobs && !atkgpu_stream_create(&c->stream);
     if (ok) {
@@ -225,10 +233,10 @@ atkdsp_cfar *atkdsp_cfar_create(size_t nbins, const atkdsp_cfar_config *cfg) {
         c->d_db = (float *)atkgpu_dmalloc(slab * nb * sizeof(float));
         c->d_raw = (double *)atkgpu_dmalloc((k - 1 + slab) * nb * sizeof(double));
         c->d_hist = (double *)atkgpu_dmalloc((k > 1 ? k - 1 : 1) * nb * sizeof(double));
-        c->d_X = (double *)atkgpu_dmalloc(slab * nb * sizeof(double));
+        c->d_X = k > 1 ? (double *)atkgpu_dmalloc(slab * nb * sizeof(double)) : NULL;
         c->d_thr = (double *)atkgpu_dmalloc(slab * nb * sizeof(double));
         c->d_noise = (double *)atkgpu_dmalloc(slab * nb * sizeof(double));
-        ok = c->d_cells && c->d_ok && c->d_sided && c->d_hits && c->d_db && c->d_raw && c->d_hist && c->d_X &&
+        ok = c->d_cells && c->d_ok && c->d_sided && c->d_hits && c->d_db && c->d_raw && c->d_hist && (c->d_X || k == 1) &&
              c->d_thr && c->d_noise;
     }
     if (ok && k > 1) ok = !atkgpu_fill_u32(c->stream, c->d_hist, 0, (k - 1) * nb * 2) && !atkgpu_sync(c->stream);
@@ -340,8 +348,9 @@ static void emit_close(atkdsp_cfar *c, cfar_track *t, uint32_t extra, atkdsp_cfa
     emit(c, t, ATKDSP_CFAR_CLOSE, t1, extra, e);
 }
 
-/* one tested frame: X, thr, noise, hits and pw (this frame's power) are the
- * device's results; returns the events written */
+/* one tested frame: X, noise, hits and pw (this frame's power; NULL with one
+ * frame averaged, where no band test runs) are the device's results; returns
+ * the events written */
 static size_t track_frame(atkdsp_cfar *c, const double *X, const double *noise, const uint8_t *hits,
                           const double *pw, double tf, double dt, atkdsp_cfar_event *ev) {
     const size_t nb = c->nbins;
@@ -351,7 +360,16 @@ static size_t track_frame(atkdsp_cfar *c, const double *X, const double *noise, 
     const long gap = c->cfg.merge_gap;
     size_t nblob = 0;
     for (long i = 0; i < (long)nb;) {
-        if (!hits[i]) { ++i; continue; }
+        if (!hits[i]) {
+            ++i;
+            while (i + 8 <= (long)nb) {                         /* whole zero words at a time */
+                uint64_t w;
+                memcpy(&w, hits + i, 8);
+                if (w) break;
+                i += 8;
+            }
+            continue;
+        }
         long hi = i, prev = i;
         for (long j = i + 1; j < (long)nb && j - prev - 1 <= gap; ++j)
             if (hits[j]) { hi = j; prev = j; }
@@ -506,7 +524,9 @@ ptrdiff_t atkdsp_cfar_process(atkdsp_cfar *c, const float *frames_db, size_t nfr
     size_t nev = 0;
     for (size_t f0 = 0; f0 < nframes; f0 += c->slab) {
         const size_t nf = nframes - f0 < c->slab ? nframes - f0 : c->slab;
-        /* the raw rows: k-1 frames of history, then this slab */
+        /* the raw rows: k-1 frames of history, then this slab; with one frame
+         * averaged, the raw rows are the averages */
+        const double *dX = k > 1 ? c->d_X : c->d_raw;
         if (atkgpu_h2d(c->stream, c->d_db, frames_db + f0 * nb, nf * nb * sizeof(float)) ||
             (k > 1 && atkgpu_d2d(c->stream, c->d_raw, c->d_hist, (k - 1) * nb * sizeof(double))) ||
             atkgpu_cfar_raw(c->stream, c->d_db, nf * nb, c->d_raw + (k - 1) * nb) ||
@@ -514,18 +534,18 @@ ptrdiff_t atkdsp_cfar_process(atkdsp_cfar *c, const float *frames_db, size_t nfr
                              c->alpha_ca, c->et, c->d_cells, c->d_ok, c->d_sided, c->d_X, c->d_thr, c->d_noise,
                              c->d_hits) ||
             (k > 1 && atkgpu_d2d(c->stream, c->d_hist, c->d_raw + nf * nb, (k - 1) * nb * sizeof(double))) ||
-            atkgpu_d2h(c->stream, c->X, c->d_X, nf * nb * sizeof(double)) ||
-            atkgpu_d2h(c->stream, c->thr, c->d_thr, nf * nb * sizeof(double)) ||
+            atkgpu_d2h(c->stream, c->X, dX, nf * nb * sizeof(double)) ||
+            atkgpu_d2h(c->stream, c->thr, c->d_thr + (nf - 1) * nb, nb * sizeof(double)) ||
             atkgpu_d2h(c->stream, c->noise, c->d_noise, nf * nb * sizeof(double)) ||
             atkgpu_d2h(c->stream, c->hits, c->d_hits, nf * nb) ||
-            atkgpu_d2h(c->stream, c->raw, c->d_raw + (k - 1) * nb, nf * nb * sizeof(double)) ||
+            (k > 1 && atkgpu_d2h(c->stream, c->raw, c->d_raw + (k - 1) * nb, nf * nb * sizeof(double))) ||
             atkgpu_sync(c->stream))
             return ATKDSP_E_FFT;
         for (size_t j = 0; j < nf; ++j) {
             const double tf = frame_t0 + (double)(f0 + j) * frame_dt;
             const size_t slot = (size_t)(c->frame % (int64_t)k);
-            const double *pw = c->raw + j * nb;
-            memcpy(c->ring + slot * nb, pw, nb * sizeof(double));
+            const double *pw = k > 1 ? c->raw + j * nb : NULL;
+            if (k > 1) memcpy(c->ring + slot * nb, pw, nb * sizeof(double));
             c->ring_t[slot] = tf;
             c->ring_dt[slot] = frame_dt;
             c->st.frames++;
@@ -534,19 +554,23 @@ ptrdiff_t atkdsp_cfar_process(atkdsp_cfar *c, const float *frames_db, size_t nfr
             if (c->seen < (int64_t)k) { c->st.warmup++; continue; }
             const double *X = c->X + j * nb, *noise = c->noise + j * nb;
             const uint8_t *hits = c->hits + j * nb;
-            uint64_t tested = 0, nhit = 0, untest = 0;
-            for (size_t i = 0; i < nb; ++i) {
-                if (!c->ok[i]) { if (!c->mask[i]) ++untest; continue; }
-                ++tested;
-                nhit += hits[i];
+            uint64_t nhit = 0;
+            size_t i = 0;
+            for (; i + 8 <= nb; i += 8) {                      /* hits are 0 / 1 bytes */
+                uint64_t w;
+                memcpy(&w, hits + i, 8);
+                nhit += (uint64_t)__builtin_popcountll(w);
             }
-            c->st.cells_tested += tested;
+            for (; i < nb; ++i) nhit += hits[i];
+            c->st.cells_tested += c->n_ok;
             c->st.cell_hits += nhit;
-            c->st.untestable += untest;
-            memcpy(c->last_X, X, nb * sizeof(double));
-            memcpy(c->last_thr, c->thr + j * nb, nb * sizeof(double));
-            memcpy(c->last_hits, hits, nb);
-            c->has_last = 1;
+            c->st.untestable += c->n_untestable;
+            if (j == nf - 1) {                                 /* the call's last tested frame so far */
+                memcpy(c->last_X, X, nb * sizeof(double));
+                memcpy(c->last_thr, c->thr, nb * sizeof(double));
+                memcpy(c->last_hits, hits, nb);
+                c->has_last = 1;
+            }
             nev += track_frame(c, X, noise, hits, pw, tf, frame_dt, ev + nev);
         }
     }

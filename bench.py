@@ -176,6 +176,44 @@ def osx_extract(n: int = 2_000_000, nchan: int = 32, calls: int = 10, cpu_second
     return res
 
 
+def cfar_frames(nbins: int = 4096, nframes: int = 512, calls: int = 5):
+    """CFAR detection and burst tracking through the drop-in ABI: frames of
+    4096 dB bins (host buffers, as the reference's callers pass them) in calls
+    of 512 frames, for CA (one frame), OS (four-frame average) and GO (two),
+    with the default 16 + 16 reference cells and 4 guard cells. Beside each,
+    the reference C on the same frames (one host thread: its detector is
+    serial)."""
+    import atkdsp_amd as A
+    rng = np.random.default_rng(1)
+    p = rng.exponential(1.0, (nframes, nbins))
+    p[:, 1000:1008] += 40.0
+    p[100:300, 2000:2003] += 30.0
+    fr = (10.0 * np.log10(p) - 90.0).astype(np.float32)
+    out = {"workload": f"{nbins}-bin dB frames in calls of {nframes}, 16+16 reference cells, 4 guard, "
+                       "Pfa 1e-6, open 2 of 3, close after 3; host buffers in, events out"}
+    try:
+        from oracle.ref import Ref, available
+        ref = Ref() if available() else None
+    except Exception:                               # noqa: BLE001
+        ref = None
+    for name, cfg in (("ca_k1", dict(mode=0)), ("os_k4", dict(mode=1, k_frames=4)), ("go_k2", dict(mode=2, k_frames=2))):
+        c = A.Cfar(nbins, **cfg)
+        c.process(fr[:64], 0.0, 0.01)                # warm-up
+        t0 = time.perf_counter()
+        nev = 0
+        for r in range(calls):
+            nev += len(c.process(fr, float(r), 0.01))
+        us = (time.perf_counter() - t0) / (calls * nframes) * 1e6
+        line = {"us_per_frame": round(us, 2), "frames_per_s": round(1e6 / us), "events_per_call": nev // calls}
+        if ref is not None:
+            t0 = time.perf_counter()
+            ref.cfar_run(nbins, cfg, [("process", fr, 0.0, 0.01)])
+            rus = (time.perf_counter() - t0) / nframes * 1e6
+            line["cpu_reference"] = {"us_per_frame": round(rus, 2), "threads": 1}
+        out[name] = line
+    return out
+
+
 def nco_phase_run(seconds: float = 10.0, fs: float = 40e6, p: int = 1234, q: int = 10007, chunk: int = 65536):
     """10 s of 40 MSPS through the device-resident NCO in 65,536-sample runs,
     f = fs * p / q: the FP64 phase after every run against the exact phase
@@ -302,6 +340,8 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="0 skips the serving runs")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the precision-form comparison and the NCO phase run")
+    ap.add_argument("--cfar-only", action="store_true",
+                    help="only time the CFAR detector and tracker (and the reference C beside it) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB -> 1 GiB copy kernel 5 times (the reference point for "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -322,6 +362,9 @@ def main() -> int:
         return 2
     import atkdsp_amd
     from atkdsp_amd.chain import Chain
+    if args.cfar_only:
+        print(json.dumps({"cfar": cfar_frames()}))
+        return 0
 
     S = args.streams
     print(f"[bench] generating {S} streams of {N_BLOCK} samples", file=sys.stderr, flush=True)
@@ -372,6 +415,7 @@ def main() -> int:
         extras["nco_phase_run"] = nco_phase_run()
         extras["coherent_array"] = coherent_array()
         extras["osx_extract"] = osx_extract()
+        extras["cfar"] = cfar_frames()
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, streams[0])
     uni = serve.get("uniform", {})
     out = {

@@ -282,7 +282,7 @@ class Ref:
                 if op[0] == "process":
                     fr = np.ascontiguousarray(op[1], np.float32)
                     nf = fr.size // nbins
-                    ev = np.zeros(max(self.L.atkdsp_cfar_events_max(h, nf), 1), CFAR_EVENT)
+                    ev = np.empty(max(self.L.atkdsp_cfar_events_max(h, nf), 1), CFAR_EVENT)
                     n = self.L.atkdsp_cfar_process(h, self._f(fr), nf, op[2], op[3], ev.ctypes.data, ev.size)
                     assert n >= 0, n
                     out.append(ev[:n].copy())
