@@ -516,65 +516,95 @@ static size_t track_frame(atkdsp_cfar *c, const double *X, const double *noise, 
     return nev;
 }
 
+/* nf frames already in d_db, the first of them frame f0 of the call: the
+ * device test, the results back, the tracker; returns the events written or
+ * ATKDSP_E_FFT */
+static ptrdiff_t run_slab(atkdsp_cfar *c, size_t nf, size_t f0, double frame_t0, double frame_dt,
+                          atkdsp_cfar_event *ev) {
+    const size_t nb = c->nbins, k = (size_t)c->cfg.k_frames;
+    /* the raw rows: k-1 frames of history, then this slab; with one frame
+     * averaged, the raw rows are the averages */
+    const double *dX = k > 1 ? c->d_X : c->d_raw;
+    if ((k > 1 && atkgpu_d2d(c->stream, c->d_raw, c->d_hist, (k - 1) * nb * sizeof(double))) ||
+        atkgpu_cfar_raw(c->stream, c->d_db, nf * nb, c->d_raw + (k - 1) * nb) ||
+        atkgpu_cfar_test(c->stream, c->d_raw, (int)nb, (int)nf, (int)k, c->cfg.mode, c->N, c->cfg.os_k, c->alpha,
+                         c->alpha_ca, c->et, c->d_cells, c->d_ok, c->d_sided, c->d_X, c->d_thr, c->d_noise,
+                         c->d_hits) ||
+        (k > 1 && atkgpu_d2d(c->stream, c->d_hist, c->d_raw + nf * nb, (k - 1) * nb * sizeof(double))) ||
+        atkgpu_d2h(c->stream, c->X, dX, nf * nb * sizeof(double)) ||
+        atkgpu_d2h(c->stream, c->thr, c->d_thr + (nf - 1) * nb, nb * sizeof(double)) ||
+        atkgpu_d2h(c->stream, c->noise, c->d_noise, nf * nb * sizeof(double)) ||
+        atkgpu_d2h(c->stream, c->hits, c->d_hits, nf * nb) ||
+        (k > 1 && atkgpu_d2h(c->stream, c->raw, c->d_raw + (k - 1) * nb, nf * nb * sizeof(double))) ||
+        atkgpu_sync(c->stream))
+        return ATKDSP_E_FFT;
+    size_t nev = 0;
+    for (size_t j = 0; j < nf; ++j) {
+        const double tf = frame_t0 + (double)(f0 + j) * frame_dt;
+        const size_t slot = (size_t)(c->frame % (int64_t)k);
+        const double *pw = k > 1 ? c->raw + j * nb : NULL;
+        if (k > 1) memcpy(c->ring + slot * nb, pw, nb * sizeof(double));
+        c->ring_t[slot] = tf;
+        c->ring_dt[slot] = frame_dt;
+        c->st.frames++;
+        c->seen++;
+        c->frame++;
+        if (c->seen < (int64_t)k) { c->st.warmup++; continue; }
+        const double *X = c->X + j * nb, *noise = c->noise + j * nb;
+        const uint8_t *hits = c->hits + j * nb;
+        uint64_t nhit = 0;
+        size_t i = 0;
+        for (; i + 8 <= nb; i += 8) {                          /* hits are 0 / 1 bytes */
+            uint64_t w;
+            memcpy(&w, hits + i, 8);
+            nhit += (uint64_t)__builtin_popcountll(w);
+        }
+        for (; i < nb; ++i) nhit += hits[i];
+        c->st.cells_tested += c->n_ok;
+        c->st.cell_hits += nhit;
+        c->st.untestable += c->n_untestable;
+        if (j == nf - 1) {                                     /* the call's last tested frame so far */
+            memcpy(c->last_X, X, nb * sizeof(double));
+            memcpy(c->last_thr, c->thr, nb * sizeof(double));
+            memcpy(c->last_hits, hits, nb);
+            c->has_last = 1;
+        }
+        nev += track_frame(c, X, noise, hits, pw, tf, frame_dt, ev + nev);
+    }
+    return (ptrdiff_t)nev;
+}
+
 ptrdiff_t atkdsp_cfar_process(atkdsp_cfar *c, const float *frames_db, size_t nframes, double frame_t0,
                               double frame_dt, atkdsp_cfar_event *ev, size_t cap) {
     if (!c || (!frames_db && nframes) || (!ev && nframes)) return ATKDSP_E_ARG;
     if (cap < atkdsp_cfar_events_max(c, nframes)) return ATKDSP_E_CAP;
-    const size_t nb = c->nbins, k = (size_t)c->cfg.k_frames;
+    const size_t nb = c->nbins;
     size_t nev = 0;
     for (size_t f0 = 0; f0 < nframes; f0 += c->slab) {
         const size_t nf = nframes - f0 < c->slab ? nframes - f0 : c->slab;
-        /* the raw rows: k-1 frames of history, then this slab; with one frame
-         * averaged, the raw rows are the averages */
-        const double *dX = k > 1 ? c->d_X : c->d_raw;
-        if (atkgpu_h2d(c->stream, c->d_db, frames_db + f0 * nb, nf * nb * sizeof(float)) ||
-            (k > 1 && atkgpu_d2d(c->stream, c->d_raw, c->d_hist, (k - 1) * nb * sizeof(double))) ||
-            atkgpu_cfar_raw(c->stream, c->d_db, nf * nb, c->d_raw + (k - 1) * nb) ||
-            atkgpu_cfar_test(c->stream, c->d_raw, (int)nb, (int)nf, (int)k, c->cfg.mode, c->N, c->cfg.os_k, c->alpha,
-                             c->alpha_ca, c->et, c->d_cells, c->d_ok, c->d_sided, c->d_X, c->d_thr, c->d_noise,
-                             c->d_hits) ||
-            (k > 1 && atkgpu_d2d(c->stream, c->d_hist, c->d_raw + nf * nb, (k - 1) * nb * sizeof(double))) ||
-            atkgpu_d2h(c->stream, c->X, dX, nf * nb * sizeof(double)) ||
-            atkgpu_d2h(c->stream, c->thr, c->d_thr + (nf - 1) * nb, nb * sizeof(double)) ||
-            atkgpu_d2h(c->stream, c->noise, c->d_noise, nf * nb * sizeof(double)) ||
-            atkgpu_d2h(c->stream, c->hits, c->d_hits, nf * nb) ||
-            (k > 1 && atkgpu_d2h(c->stream, c->raw, c->d_raw + (k - 1) * nb, nf * nb * sizeof(double))) ||
-            atkgpu_sync(c->stream))
-            return ATKDSP_E_FFT;
-        for (size_t j = 0; j < nf; ++j) {
-            const double tf = frame_t0 + (double)(f0 + j) * frame_dt;
-            const size_t slot = (size_t)(c->frame % (int64_t)k);
-            const double *pw = k > 1 ? c->raw + j * nb : NULL;
-            if (k > 1) memcpy(c->ring + slot * nb, pw, nb * sizeof(double));
-            c->ring_t[slot] = tf;
-            c->ring_dt[slot] = frame_dt;
-            c->st.frames++;
-            c->seen++;
-            c->frame++;
-            if (c->seen < (int64_t)k) { c->st.warmup++; continue; }
-            const double *X = c->X + j * nb, *noise = c->noise + j * nb;
-            const uint8_t *hits = c->hits + j * nb;
-            uint64_t nhit = 0;
-            size_t i = 0;
-            for (; i + 8 <= nb; i += 8) {                      /* hits are 0 / 1 bytes */
-                uint64_t w;
-                memcpy(&w, hits + i, 8);
-                nhit += (uint64_t)__builtin_popcountll(w);
-            }
-            for (; i < nb; ++i) nhit += hits[i];
-            c->st.cells_tested += c->n_ok;
-            c->st.cell_hits += nhit;
-            c->st.untestable += c->n_untestable;
-            if (j == nf - 1) {                                 /* the call's last tested frame so far */
-                memcpy(c->last_X, X, nb * sizeof(double));
-                memcpy(c->last_thr, c->thr, nb * sizeof(double));
-                memcpy(c->last_hits, hits, nb);
-                c->has_last = 1;
-            }
-            nev += track_frame(c, X, noise, hits, pw, tf, frame_dt, ev + nev);
-        }
+        if (atkgpu_h2d(c->stream, c->d_db, frames_db + f0 * nb, nf * nb * sizeof(float))) return ATKDSP_E_FFT;
+        const ptrdiff_t got = run_slab(c, nf, f0, frame_t0, frame_dt, ev + nev);
+        if (got < 0) return got;
+        nev += (size_t)got;
     }
     return (ptrdiff_t)nev;
+}
+
+size_t atk_cfar_nbins(const atkdsp_cfar *c) { return c->nbins; }
+size_t atk_cfar_slab(const atkdsp_cfar *c) { return c->slab; }
+
+int atk_cfar_stage_rows(atkdsp_cfar *c, const float *const *rows, size_t n) {
+    if (!c || !rows || n > c->slab) return ATKDSP_E_ARG;
+    for (size_t j = 0; j < n; ++j)
+        if (atkgpu_d2d(c->stream, c->d_db + j * c->nbins, rows[j], c->nbins * sizeof(float))) return ATKDSP_E_FFT;
+    return atkgpu_sync(c->stream) ? ATKDSP_E_FFT : ATKDSP_OK;
+}
+
+ptrdiff_t atk_cfar_process_staged(atkdsp_cfar *c, size_t n, double frame_t0, double frame_dt,
+                                  atkdsp_cfar_event *ev, size_t cap) {
+    if (!c || n > c->slab || (!ev && n)) return ATKDSP_E_ARG;
+    if (cap < atkdsp_cfar_events_max(c, n)) return ATKDSP_E_CAP;
+    return n ? run_slab(c, n, 0, frame_t0, frame_dt, ev) : 0;
 }
 
 ptrdiff_t atkdsp_cfar_flush(atkdsp_cfar *c, atkdsp_cfar_event *ev, size_t cap) {

@@ -41,6 +41,16 @@ atk_pool *atk_pool_get(void);
 /* channel records strongest first, with the reference's comparator and qsort */
 void atk_sort_channels(atkdsp_channel *c, size_t n);
 
+/* CFAR over frames that live on the device (cfar_run.c): at most atk_cfar_slab
+ * frames, each a row of atk_cfar_nbins dB values, copied into the handle's
+ * own staging by atk_cfar_stage_rows, then tested and tracked by
+ * atk_cfar_process_staged as atkdsp_cfar_process would */
+size_t    atk_cfar_nbins(const atkdsp_cfar *c);
+size_t    atk_cfar_slab(const atkdsp_cfar *c);
+int       atk_cfar_stage_rows(atkdsp_cfar *c, const float *const *rows, size_t n);
+ptrdiff_t atk_cfar_process_staged(atkdsp_cfar *c, size_t n, double frame_t0, double frame_dt,
+                                  atkdsp_cfar_event *ev, size_t cap);
+
 /* ln P(a, x) and ln Q(a, x), the regularized incomplete gamma (cfar_host.c) */
 void atk_cfar_lgam_pq(double a, double x, double *lp, double *lq);
 

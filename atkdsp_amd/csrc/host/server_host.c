@@ -1103,6 +1103,44 @@ ptrdiff_t atkgpu_server_detect(atkgpu_server *v, int sid, long long line, unsign
     return (ptrdiff_t)kept;
 }
 
+/* CFAR over served lines where they live: lines [first, first + n) of one
+ * element, each a frame of dcols dB values, go through the handle's device
+ * test and tracker as atkdsp_cfar_process would take them from the host.
+ * The lines are copied into the handle's staging on its own stream; if the
+ * ring overwrote the oldest of them meanwhile, ATKDSP_E_OVERRUN and the
+ * handle is unchanged */
+ptrdiff_t atkgpu_server_cfar(atkgpu_server *v, int sid, unsigned element, atkdsp_cfar *c, long long first, size_t n,
+                             double frame_t0, double frame_dt, atkdsp_cfar_event *ev, size_t cap) {
+    if (!v || !c || sid < 0 || (unsigned)sid >= v->max_streams || first < 0 || (!ev && n)) return ATKDSP_E_ARG;
+    if (n == 0) return 0;
+    if (n > atk_cfar_slab(c) || n > 256) return ATKDSP_E_ARG;
+    if (cap < atkdsp_cfar_events_max(c, n)) return ATKDSP_E_CAP;
+    const float *rows[256];
+    pthread_mutex_lock(&v->mu);
+    sstream *s = &v->st[sid];
+    const long long last = first + (long long)n - 1;
+    if (!s->used || !s->dN || element >= s->nel || s->dcols != atk_cfar_nbins(c) ||
+        last >= disp_frames_at(s->g_sub, s->dN, s->dhop) / s->dF) {
+        pthread_mutex_unlock(&v->mu);
+        return ATKDSP_E_ARG;
+    }
+    while (s->lines_done <= last && !v->failed) pthread_cond_wait(&v->cv_done, &v->mu);
+    const int failed = v->failed;
+    const int gone = first + (long long)s->dslots < s->lines_disp || (long long)n > (long long)s->dslots;
+    const size_t len = (size_t)s->nel * s->dcols;
+    for (size_t i = 0; i < n; ++i)
+        rows[i] = s->dring + (size_t)((first + (long long)i) % s->dslots) * len + (size_t)element * s->dcols;
+    pthread_mutex_unlock(&v->mu);
+    if (failed) return ATKDSP_E_FFT;
+    if (gone) return ATKDSP_E_ARG;
+    if (atk_cfar_stage_rows(c, rows, n)) return ATKDSP_E_FFT;
+    pthread_mutex_lock(&v->mu);
+    const int overrun = first + (long long)s->dslots < s->lines_disp;
+    pthread_mutex_unlock(&v->mu);
+    if (overrun) return ATKDSP_E_OVERRUN;
+    return atk_cfar_process_staged(c, n, frame_t0, frame_dt, ev, cap);
+}
+
 /* the newest completed line (element 0) of each listed stream, all in one
  * launch: per stream i, out[i*per_cap ...] strongest first and counts[i]
  * (ATKDSP_E_CAP past min(per_cap, QSET_CAP) channels, the first ones then in

@@ -17,7 +17,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import AtkDspError, _ChannelC, _Cf32, _RingMeta, _channels, _check, _fmt, _meta_dict, load
+from . import CFAR_EVENT_DTYPE, AtkDspError, _ChannelC, _Cf32, _RingMeta, _channels, _check, _fmt, _meta_dict, load
 
 RING_EMPTY = -64               # atkgpu.h ATKGPU_RING_EMPTY: the ring consumer has nothing new
 
@@ -66,6 +66,8 @@ def _bind(lib) -> None:
         "atkgpu_server_detect_latest": (C.c_int, [vp, P(C.c_int), C.c_uint, C.c_float, C.c_int, C.c_int,
                                                   P(_ChannelC), sz, P(C.c_ssize_t), P(C.c_longlong),
                                                   P(C.c_float)]),
+        "atkgpu_server_cfar": (C.c_ssize_t, [vp, C.c_int, C.c_uint, vp, C.c_longlong, sz, C.c_double, C.c_double,
+                                             vp, sz]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -199,6 +201,16 @@ class Server:
                                                       float(threshold_db), int(gap_bins), int(min_run), buf,
                                                       int(max_channels), C.byref(fl))), "server_detect")
         return _channels(buf, n), float(fl.value)
+
+    def cfar(self, sid: int, cfar, first: int, n: int, t0: float = 0.0, dt: float = 1.0, element: int = 0):
+        """CFAR (an atkdsp_amd.Cfar over the stream's display columns) over
+        served lines [first, first + n) where they live on the device: the
+        events, as Cfar.process would give for the same lines."""
+        ev = np.empty(max(cfar.events_max(int(n)), 1), CFAR_EVENT_DTYPE)
+        got = _check(int(self._lib.atkgpu_server_cfar(self._h, int(sid), int(element), cfar._p, int(first), int(n),
+                                                      float(t0), float(dt), ev.ctypes.data, ev.size)),
+                     "server_cfar")
+        return ev[:got].copy()
 
     def detect_latest(self, sids, threshold_db: float = 6.0, gap_bins: int = 4, min_run: int = 2,
                       max_channels: int = 64):

@@ -179,6 +179,18 @@ ATKGPU_API int       atkgpu_server_read_line(atkgpu_server *v, int sid, long lon
 ATKGPU_API ptrdiff_t atkgpu_server_detect(atkgpu_server *v, int sid, long long line, unsigned element,
                                           float floor_db, float threshold_db, int gap_bins, int min_run,
                                           atkdsp_channel *out, size_t out_cap, float *floor_used);
+/* CFAR (atkdsp_cfar, include/atkdsp.h) over served lines where they live in
+ * HBM: lines [first, first + n) of one element are the handle's next n frames
+ * (frame times frame_t0 + i frame_dt), tested on the device and tracked as
+ * atkdsp_cfar_process would take them from host memory; events as there.
+ * The handle's nbins must equal the stream's display columns; n at most 256
+ * and at most the lines the ring keeps. Waits for the lines to complete.
+ * ATKDSP_E_ARG when a line is not completed by the chunks submitted or has
+ * left the ring; ATKDSP_E_OVERRUN (handle unchanged) when the ring overwrote
+ * one while it was being copied. Calls on one handle must not overlap. */
+ATKGPU_API ptrdiff_t atkgpu_server_cfar(atkgpu_server *v, int sid, unsigned element, atkdsp_cfar *c,
+                                        long long first, size_t n, double frame_t0, double frame_dt,
+                                        atkdsp_cfar_event *ev, size_t cap);
 /* the same over the newest completed line (element 0) of each of n streams
  * (one display line length among them), in one launch, the floor of each
  * the median of its smoothed line: for stream i, counts[i] channels at

@@ -146,3 +146,74 @@ def test_os_ranks_in_lds_and_in_scratch(gpu, n_ref):
     same_cfar_events(got, exp, f"OS n_ref {n_ref}")
     assert c.stats() == o.stats()
     assert np.array_equal(c.last()[2], o.last[2].astype(np.uint8))
+
+
+def test_server_cfar_on_the_device_lines(gpu):
+    """CFAR over served display lines where they live (atkgpu_server_cfar):
+    the same events and counters as the drop-in handle given the same lines
+    read back to the host, over calls of 1 to 8 lines; a 2-element array
+    stream by element; refusals for lines not yet submitted, lines that have
+    left the ring, a column count the handle was not made for, an element the
+    stream lacks, and more lines than the ring keeps."""
+    from atkdsp_amd import synth
+    from atkdsp_amd.serve import Server
+    fs = 40e6
+    srv = Server(hbm_budget=2 << 30, max_streams=4, max_batch=16, staging_bytes=64 << 20)
+    disp = dict(nfft=4096, hop=2048, window="hann", detector="max", frames=9, cols=1024, slots=64)
+    sid = srv.add_stream(fmt="ci16", fs=fs, nchan=1024, oversample=2, taps_per_poly=16, chunk_max=65536,
+                         out_slots=2, display=disp)
+    small = srv.add_stream(fmt="ci16", fs=fs, nchan=1024, oversample=2, taps_per_poly=16, chunk_max=65536,
+                           out_slots=2, display=dict(disp, slots=4))
+    aid = srv.add_stream(fmt="cf32", fs=fs, nchan=256, oversample=2, taps_per_poly=12, chunk_max=32768,
+                         out_slots=2, nel=2, display=dict(disp, cols=512))
+    n = 600_000
+    z = synth.wideband_iq(21, n, fs, ntones=8, nbursts=0)
+    k = np.arange(n)
+    for f, a, b in ((5.3e6, 90_000, 300_000), (-11.7e6, 200_000, 520_000), (1.1e6, 400_000, 430_000)):
+        z[a:b] += 0.05 * np.exp(2j * np.pi * f / fs * k[a:b])
+    raw = synth.quantize(z, O.CI16)
+    for a in range(0, n, 50_000):
+        srv.submit(sid, raw[a * 4:(a + 50_000) * 4])
+        srv.submit(small, raw[a * 4:(a + 50_000) * 4])
+    arr = np.concatenate([z, z * np.exp(0.4j)]).astype(np.complex64)
+    for a in range(0, n, 25_000):
+        srv.submit(aid, np.concatenate([arr[a:a + 25_000], arr[n + a:n + a + 25_000]]))
+    srv.drain()
+    L = srv.lines(sid)
+    assert L >= 25 and srv.lines(small) == L
+    cfg = dict(mode="ca", n_ref=8, n_guard=2, k_frames=2, pfa=1e-4, open_m=2, open_n=3, close_k=2)
+    host = A.Cfar(1024, **cfg)
+    frames = np.stack([srv.read_line(sid, i) for i in range(L)])
+    dt = 0.0625                        # exact: calls' start times add up exactly
+    exp = np.concatenate([host.process(frames, 0.0, dt), host.flush()])
+    assert set(exp["type"]) == {1, 3} or set(exp["type"]) == {1, 2, 3}
+    dev = A.Cfar(1024, **cfg)
+    got, i, step = [], 0, 1
+    while i < L:
+        m = min(step, L - i)
+        got.append(srv.cfar(sid, dev, i, m, i * dt, dt))
+        i += m
+        step = step % 8 + 1
+    got = np.concatenate(got + [dev.flush()])
+    assert got.tobytes() == exp.tobytes()
+    assert dev.stats() == host.stats()
+    # the array stream, element by element
+    for e in range(2):
+        h = A.Cfar(512, **cfg)
+        fr = np.stack([srv.read_line(aid, i)[e] for i in range(srv.lines(aid))])
+        want = h.process(fr, 0.0, dt)
+        d = A.Cfar(512, **cfg)
+        assert srv.cfar(aid, d, 0, fr.shape[0], 0.0, dt, element=e).tobytes() == want.tobytes()
+    with pytest.raises(A.AtkDspError):
+        srv.cfar(sid, A.Cfar(1024, **cfg), L + 50, 1)      # not completed by the chunks submitted
+    with pytest.raises(A.AtkDspError):
+        srv.cfar(small, A.Cfar(1024, **cfg), 0, 1)         # left the 4-line ring
+    ok = srv.cfar(small, A.Cfar(1024, **cfg), L - 4, 4)    # the 4 lines it keeps
+    assert ok.dtype == A.CFAR_EVENT_DTYPE
+    with pytest.raises(A.AtkDspError):
+        srv.cfar(small, A.Cfar(1024, **cfg), L - 5, 5)     # more lines than the ring keeps
+    with pytest.raises(A.AtkDspError):
+        srv.cfar(sid, A.Cfar(512, **cfg), 0, 1)            # columns differ from the handle's bins
+    with pytest.raises(A.AtkDspError):
+        srv.cfar(aid, A.Cfar(512, **cfg), 0, 1, element=2)  # no such element
+    srv.close()
