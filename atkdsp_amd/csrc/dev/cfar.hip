@@ -11,10 +11,11 @@
 //          smallest; a NaN cell makes T NaN;
 //   thr    alpha T, noise T / E[T], hit X > thr.
 // One thread per (frame, bin). A frame's raw rows sit after the k-1 rows of
-// the frames before it, so a call needs no history kernel. OS ranks its
-// cells by counting (each cell's rank = the cells below it, ties by
-// position); with N <= 64 the cells sit in LDS, 64 per thread, else in
-// per-lane scratch.
+// the frames before it, so a call needs no history kernel. OS with up to 64
+// cells sorts them in registers (a bitonic network over the next power of two,
+// padded with +inf, which cannot change the k-th smallest of the N); above
+// 64 it ranks them by counting in per-lane scratch (each cell's rank = the
+// cells below it, ties by position). Both give the os_k-th smallest value.
 #include <hip/hip_runtime.h>
 #include "atkgpu_dev.h"
 
@@ -24,7 +25,7 @@ namespace {
 
 constexpr double LN10_10 = 0.23025850929940458;   // ln(10) / 10
 constexpr int TB = 128;                            // threads (bins) per block
-constexpr int OS_LDS_N = 64;
+constexpr int OS_REG_N = 64;
 
 __global__ void k_cfar_raw(const float *__restrict__ db, size_t n, double *__restrict__ raw) {
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -45,7 +46,42 @@ __global__ void k_cfar_avg(const double *__restrict__ raw, int nbins, int nframe
     X[(size_t)j * nbins + i] = s / k;
 }
 
-// OSF: 0 CA / GO / SO only, 1 OS with its cells in LDS, 2 OS in scratch
+// the os_k-th smallest (1-based) of N <= P values held in registers: a
+// bitonic sort of the P registers, every index a compile-time constant
+template <int P>
+__device__ __forceinline__ double os_select_reg(const double *__restrict__ X, const int *__restrict__ c, int N,
+                                                int os_k) {
+    double v[P];
+    bool nan = false;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        v[q] = q < N ? X[c[q < N ? q : 0]] : __builtin_inf();
+        nan = nan || (v[q] != v[q]);
+    }
+    if (nan) return __builtin_nan("");
+#pragma unroll
+    for (int k = 2; k <= P; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const double a = v[i], b = v[l];
+                    const double lo = a < b ? a : b, hi = a < b ? b : a;
+                    const bool up = (i & k) == 0;
+                    v[i] = up ? lo : hi;
+                    v[l] = up ? hi : lo;
+                }
+            }
+    double z = v[0];
+#pragma unroll
+    for (int q = 1; q < P; ++q) z = q == os_k - 1 ? v[q] : z;
+    return z;
+}
+
+// OSF: 0 CA / GO / SO only; 4..64 OS with its cells sorted in OSF registers;
+// 2 OS ranked in scratch
 template <int OSF>
 __global__ __launch_bounds__(TB) void k_cfar_test(const double *__restrict__ Xall, int nbins, int nframes, int mode,
                                                   int N, int os_k, double alpha, double alpha_ca, double et,
@@ -65,29 +101,23 @@ __global__ __launch_bounds__(TB) void k_cfar_test(const double *__restrict__ Xal
     }
     const int *c = cells + (size_t)i * N;
     double z, a = alpha, scale = et;
-    if constexpr (OSF != 0) {
-        double *v;
-        double vs_scratch[OSF == 2 ? 512 : 1];
-        if constexpr (OSF == 1) {
-            __shared__ double vs[OS_LDS_N * TB];
-            v = vs + threadIdx.x;                      // cell q at v[q * TB]
-        } else {
-            v = vs_scratch;
-        }
-        constexpr int STRIDE = OSF == 1 ? TB : 1;
+    if constexpr (OSF >= 4) {
+        z = os_select_reg<OSF>(X, c, N, os_k);
+    } else if constexpr (OSF == 2) {
+        double v[512];
         bool nan = false;
         for (int q = 0; q < N; ++q) {
             const double x = X[c[q]];
-            v[q * STRIDE] = x;
+            v[q] = x;
             nan = nan || (x != x);
         }
         z = __builtin_nan("");
         if (!nan) {
             for (int q = 0; q < N; ++q) {
-                const double vq = v[q * STRIDE];
+                const double vq = v[q];
                 int below = 0;
                 for (int p = 0; p < N; ++p) {
-                    const double vp = v[p * STRIDE];
+                    const double vp = v[p];
                     below += (vp < vq) || (vp == vq && p < q);
                 }
                 if (below == os_k - 1) { z = vq; break; }
@@ -135,7 +165,13 @@ extern "C" int atkgpu_cfar_test(void *s, const double *raw, int nbins, int nfram
         hipLaunchKernelGGL(k_cfar_avg, grid, dim3(TB), 0, (hipStream_t)s, raw, nbins, nframes, k, X);
         Xin = X;
     }
-    auto kern = mode != 1 ? k_cfar_test<0> : N <= OS_LDS_N ? k_cfar_test<1> : k_cfar_test<2>;
+    auto kern = mode != 1 ? k_cfar_test<0>
+                : N <= 4  ? k_cfar_test<4>
+                : N <= 8  ? k_cfar_test<8>
+                : N <= 16 ? k_cfar_test<16>
+                : N <= 32 ? k_cfar_test<32>
+                : N <= OS_REG_N ? k_cfar_test<64>
+                : k_cfar_test<2>;
     hipLaunchKernelGGL(kern, grid, dim3(TB), 0, (hipStream_t)s, Xin, nbins, nframes, mode, N, os_k, alpha, alpha_ca,
                        et, cells, ok, sided, thr, noise, hits);
     return hipGetLastError() == hipSuccess ? 0 : -4;

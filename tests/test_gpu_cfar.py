@@ -126,17 +126,18 @@ def test_refusals(gpu):
     assert not h.any() and np.allclose(a, -100.0, atol=1e-4)
 
 
-@pytest.mark.parametrize("n_ref", [32, 33, 64])
-def test_os_ranks_in_lds_and_in_scratch(gpu, n_ref):
-    """OS with 64 cells ranks them in LDS; 66 and 128 cells in per-lane
-    scratch: both against the oracle over 1536 bins x 40 frames."""
+@pytest.mark.parametrize("n_ref", [3, 5, 32, 33, 64])
+def test_os_in_registers_and_in_scratch(gpu, n_ref):
+    """OS sorts up to 64 cells in registers (6 and 10 cells padded to 8 and
+    16, 64 exactly) and ranks 66 and 128 cells in per-lane scratch: each
+    against the oracle over 1536 bins x 40 frames."""
     rng = np.random.default_rng(n_ref)
     p = rng.exponential(1.0, (40, 1536))
     p[5:30, 700:706] += 60.0
     p[10:40, 100:102] += 80.0
     p[12:14, 1500:1503] += 200.0
     fr = (10.0 * np.log10(p) - 80.0).astype(np.float32)
-    cfg = dict(mode=1, n_ref=n_ref, n_guard=2, k_frames=2, os_k=n_ref + 3)
+    cfg = dict(mode=1, n_ref=n_ref, n_guard=2, k_frames=2, os_k=min(n_ref + 3, 2 * n_ref))
     c = A.Cfar(1536, **cfg)
     got = np.concatenate([c.process(fr[:17], 0.0, 0.25), c.process(fr[17:], 17 * 0.25, 0.25), c.flush()])
     info = c.info()
