@@ -416,6 +416,16 @@ def cfar_scenarios():
     out.append(("os_small", 64, dict(mode=1, n_ref=2, n_guard=2, os_k=3, pfa=1e-3, merge_gap=0), fr,
                 [("process", 0, 40), ("mask", "m3"), ("process", 40, 5), ("mask", None), ("process", 45, 35),
                  ("flush",)], {"m3": m3}))
+    # narrow 5 dB bursts under an 8-frame average: many are found only on the
+    # average, so t0 / t1 fall back to their coarse estimates
+    rng = np.random.default_rng(5)
+    B = []
+    for i in range(30):
+        f0, d = int(rng.integers(0, 60)), int(rng.integers(15, 40))
+        B.append((f0, f0 + d, 40 + i * 32, 41 + i * 32, 5.0))
+    fr = cfar_scene(108, 1024, 100, 1.0, B)
+    out.append(("ca_weak_k8", 1024, dict(k_frames=8, pfa=1e-4, close_k=3), fr,
+                [("process", 0, 37), ("process", 37, 63), ("flush",)], {}))
     return out
 
 

@@ -2,12 +2,13 @@
 include/atkdsp.h:1733-1925, src/cfar.c:270-819) against the reference build's
 events (tests/golden/cfar_events.npz, made by tests/golden/make_golden.py).
 
-Seven scenarios: CA with the defaults; OS with four-frame averaging, a merge
+Eight scenarios: CA with the defaults; OS with four-frame averaging, a merge
 gap and a minimum width; GO with masks changed mid-stream; SO at a high
 false-alarm rate with few track slots (blobs dropped); CA over four-look frames
 with the Hann window's bin correlation, three-frame averaging and a reset; CA
 with NaN, -inf, +inf and very large dB values; OS on 64 bins with a mask that
-leaves bins untestable. Each replays the same calls (frames split over calls
+leaves bins untestable; narrow 5 dB bursts under an 8-frame average, found
+only on the average, so that t0 and t1 take their coarse estimates. Each replays the same calls (frames split over calls
 of 1 to 50 frames) and compares every event: type, id, flags, span and hit
 count exactly; t0 / t1 exactly (both are frame times); the centroid within
 1e-9 and the dB figures within 2e-6 relative (the GPU forms power with its

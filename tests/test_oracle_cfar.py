@@ -1,6 +1,6 @@
 """The CFAR oracle (oracle/oracle.py: cfar_tables, cfar_test, Cfar; reference
 src/cfar.c:270-819) pinned against the reference build's events
-(tests/golden/cfar_events.npz): the same seven scenarios as the GPU parity
+(tests/golden/cfar_events.npz): the same eight scenarios as the GPU parity
 test, replayed on the CPU with the reference's alpha, CA alpha and E[T];
 every event, the counters and the last frame's hits. The reference-cell
 table is also checked against its definition on small masked cases, by
@@ -58,3 +58,56 @@ def test_reference_cells_by_their_definition():
                 assert not ok[i]
                 continue
             assert ok[i] and list(cells[i]) == want and sided[i] == (len(left) == len(right) == n_ref)
+
+
+def _replay_all(mod):
+    """Every scenario through a (possibly altered) oracle module; True when
+    all of them match the reference's events, counters and hits."""
+    try:
+        for m in META:
+            name = m["name"]
+            frames = G[f"{name}_frames"].astype(np.float32)
+            c = mod.Cfar(m["nbins"], m["cfg"], *G[f"{name}_info"])
+            ev, evop = G[f"{name}_events"], G[f"{name}_evop"]
+            for i, op in enumerate(m["ops"]):
+                if op[0] == "process":
+                    got = c.process(frames[op[1]:op[1] + op[2]], op[1] * 0.01, 0.01)
+                elif op[0] == "mask":
+                    c.set_mask(None if op[1] is None else G[f"{name}_{op[1]}"])
+                    continue
+                elif op[0] == "reset":
+                    c.reset()
+                    continue
+                else:
+                    got = c.flush()
+                same_cfar_events(got, ev[evop == i], name)
+            assert list(c.stats().values()) == [int(v) for v in G[f"{name}_stats"]]
+    except AssertionError:
+        return False
+    return True
+
+
+@pytest.mark.parametrize("old,new", [
+    ("if best < 0 or ov > bov:", "if best < 0 or ov >= bov:"),                      # ties to the newer track
+    ("lo_side = free[free < i - n_guard][::-1]", "lo_side = free[free < i - n_guard]"),        # far cells first
+    ('if bin(t["hist"]).count("1") >= c["open_m"]:', 'if bin(t["hist"]).count("1") > c["open_m"]:'),
+    ('t1 = t["t_hit_end"] - 0.5 * (k - 1) * t["dt_hit"]', 't1 = t["t_hit_end"] - 0.5 * k * t["dt_hit"]'),
+    ("side = np.maximum(L, R) if mode == 2 else np.minimum(L, R)",
+     "side = np.minimum(L, R) if mode == 2 else np.maximum(L, R)"),
+    ("cuts = np.flatnonzero(np.diff(idx) - 1 > g) + 1", "cuts = np.flatnonzero(np.diff(idx) - 1 >= g) + 1"),
+    ("for r in range(k):\n                            s = (F + 1 + r) % k", "for r in range(k):\n                            s = (F + k - r) % k"),
+])
+def test_planted_mistakes_fail_the_fixtures(old, new):
+    """Plausible mistakes planted in a copy of the oracle on the CPU (ties to
+    the newer track, the left cells taken farthest first, opening one hit
+    late, the coarse t1 off by half a frame, GO and SO swapped, a merge gap
+    one bin short, t0 searched newest first) each fail the reference's
+    fixtures; the unaltered copy passes. (The order of borrowed cells only
+    moves CA's sum by rounding; the brute-force table test pins it.)"""
+    import types
+    src = open(O.__file__).read()
+    assert src.count(old) == 1, old
+    for text, want in ((src, True), (src.replace(old, new), False)):
+        mod = types.ModuleType("oracle_copy")
+        exec(compile(text, O.__file__, "exec"), mod.__dict__)
+        assert _replay_all(mod) is want
