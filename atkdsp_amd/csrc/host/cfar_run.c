@@ -280,7 +280,7 @@ static double edge_beta(atkdsp_cfar *c, int nb) {
     double pairs = 0.0;
     for (int m = 1; m <= ATKDSP_CFAR_RHO_MAX && m < nb; ++m) pairs += (double)(nb - m) * c->cfg.rho2[m - 1];
     const double s = (double)nb / (1.0 + 2.0 * pairs / (double)nb) * c->cfg.looks;
-    const double target = log(1e-3);
+    const double target = log(ATKDSP_CFAR_EDGE_PFA);
     double a = log(1e-6), b = log(1e6);
     for (int i = 0; i < 100; ++i) {
         const double mid = 0.5 * (a + b);
