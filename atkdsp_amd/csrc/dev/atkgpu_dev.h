@@ -290,11 +290,14 @@ const void *atkgpu_fftplan_twiddles(const atkgpu_fftplan *p);
 int atkgpu_cfar_raw(void *s, const float *db, size_t n, double *raw);
 /* the CFAR test of nframes frames of nbins bins; raw holds k-1+nframes rows,
  * the k-1 frames before the first; cells / ok / sided are the per-bin
- * reference-cell table (N per bin). Writes X, thr, noise (nframes x nbins) and
- * hits. mode 0 CA, 1 OS, 2 GO, 3 SO; N <= 512. */
+ * reference-cell table (N per bin), reg flags the bins whose table is the
+ * plain window at n_guard (left i-g-1-q, right i+g+1+q). Writes X (k > 1),
+ * thr, noise (nframes x nbins) and hits. mode 0 CA, 1 OS, 2 GO, 3 SO;
+ * N <= 512, n_guard + N/2 <= 512. */
 int atkgpu_cfar_test(void *s, const double *raw, int nbins, int nframes, int k, int mode, int N, int os_k,
-                     double alpha, double alpha_ca, double et, const int *cells, const unsigned char *ok,
-                     const unsigned char *sided, double *X, double *thr, double *noise, unsigned char *hits);
+                     int n_guard, double alpha, double alpha_ca, double et, const int *cells, const unsigned char *ok,
+                     const unsigned char *sided, const unsigned char *reg, double *X, double *thr, double *noise,
+                     unsigned char *hits);
 
 #ifdef __cplusplus
 }

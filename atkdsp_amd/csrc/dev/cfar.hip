@@ -11,7 +11,12 @@
 //          smallest; a NaN cell makes T NaN;
 //   thr    alpha T, noise T / E[T], hit X > thr.
 // One thread per (frame, bin). A frame's raw rows sit after the k-1 rows of
-// the frames before it, so a call needs no history kernel. OS with up to 64
+// the frames before it, so a call needs no history kernel. A block first
+// stages its 128 bins' averages and n_guard + n_ref on either side in LDS
+// (one coalesced read); a bin whose table is the plain window (flag `reg`:
+// left cells i-g-1, i-g-2, ..., right cells i+g+1, ..., none masked or
+// borrowed) takes its cells from there in table order, with no table reads;
+// other bins gather through the table. OS with up to 64
 // cells sorts them in registers (a bitonic network over the next power of two,
 // padded with +inf, which cannot change the k-th smallest of the N); above
 // 64 it ranks them by counting in per-lane scratch (each cell's rank = the
@@ -26,6 +31,7 @@ namespace {
 constexpr double LN10_10 = 0.23025850929940458;   // ln(10) / 10
 constexpr int TB = 128;                            // threads (bins) per block
 constexpr int OS_REG_N = 64;
+constexpr int TILE_R = 512;                        // n_guard + n_ref at most
 
 __global__ void k_cfar_raw(const float *__restrict__ db, size_t n, double *__restrict__ raw) {
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -48,14 +54,13 @@ __global__ void k_cfar_avg(const double *__restrict__ raw, int nbins, int nframe
 
 // the os_k-th smallest (1-based) of N <= P values held in registers: a
 // bitonic sort of the P registers, every index a compile-time constant
-template <int P>
-__device__ __forceinline__ double os_select_reg(const double *__restrict__ X, const int *__restrict__ c, int N,
-                                                int os_k) {
+template <int P, typename Cell>
+__device__ __forceinline__ double os_select_reg(Cell cell, int N, int os_k) {
     double v[P];
     bool nan = false;
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-        v[q] = q < N ? X[c[q < N ? q : 0]] : __builtin_inf();
+        v[q] = q < N ? cell(q < N ? q : 0) : __builtin_inf();
         nan = nan || (v[q] != v[q]);
     }
     if (nan) return __builtin_nan("");
@@ -80,34 +85,21 @@ __device__ __forceinline__ double os_select_reg(const double *__restrict__ X, co
     return z;
 }
 
-// OSF: 0 CA / GO / SO only; 4..64 OS with its cells sorted in OSF registers;
-// 2 OS ranked in scratch
-template <int OSF>
-__global__ __launch_bounds__(TB) void k_cfar_test(const double *__restrict__ Xall, int nbins, int nframes, int mode,
-                                                  int N, int os_k, double alpha, double alpha_ca, double et,
-                                                  const int *__restrict__ cells, const unsigned char *__restrict__ ok,
-                                                  const unsigned char *__restrict__ sided, double *__restrict__ thr,
-                                                  double *__restrict__ noise, unsigned char *__restrict__ hits) {
-    const int i = blockIdx.x * TB + threadIdx.x;
-    const int j = blockIdx.y;
-    if (i >= nbins || j >= nframes) return;
-    const double *X = Xall + (size_t)j * nbins;
-    const size_t o = (size_t)j * nbins + i;
-    if (!ok[i]) {
-        thr[o] = __builtin_nan("");
-        noise[o] = __builtin_nan("");
-        hits[o] = 0;
-        return;
-    }
-    const int *c = cells + (size_t)i * N;
-    double z, a = alpha, scale = et;
+// the statistic T of one bin over its N cells (cell(q), q in table order) and
+// the factor and scale it takes: CA the mean; GO / SO the larger / smaller side
+// mean, CA with CA's alpha and unit scale where a side borrowed; OS the os_k-th
+// smallest (OSF >= 4: sorted in OSF registers; 2: ranked in scratch)
+template <int OSF, typename Cell>
+__device__ __forceinline__ void cfar_stat(Cell cell, int mode, int N, int os_k, bool sided, double alpha_ca,
+                                          double &z, double &a, double &scale) {
+    const int nr = N / 2;
     if constexpr (OSF >= 4) {
-        z = os_select_reg<OSF>(X, c, N, os_k);
+        z = os_select_reg<OSF>(cell, N, os_k);
     } else if constexpr (OSF == 2) {
         double v[512];
         bool nan = false;
         for (int q = 0; q < N; ++q) {
-            const double x = X[c[q]];
+            const double x = cell(q);
             v[q] = x;
             nan = nan || (x != x);
         }
@@ -123,25 +115,71 @@ __global__ __launch_bounds__(TB) void k_cfar_test(const double *__restrict__ Xal
                 if (below == os_k - 1) { z = vq; break; }
             }
         }
-    } else if ((mode == 2 || mode == 3) && sided[i]) {
-        const int n = N / 2;
+    } else if ((mode == 2 || mode == 3) && sided) {
         double L = 0.0, R = 0.0;
-        for (int q = 0; q < n; ++q) L += X[c[q]];
-        for (int q = n; q < N; ++q) R += X[c[q]];
-        L /= n;
-        R /= n;
+        for (int q = 0; q < nr; ++q) L += cell(q);
+        for (int q = nr; q < N; ++q) R += cell(q);
+        L /= nr;
+        R /= nr;
         if (L != L || R != R) z = __builtin_nan("");
         else z = mode == 2 ? (L > R ? L : R) : (L < R ? L : R);
     } else {
         double s = 0.0;
-        for (int q = 0; q < N; ++q) s += X[c[q]];
+        for (int q = 0; q < N; ++q) s += cell(q);
         z = s / N;
         if (mode == 2 || mode == 3) { a = alpha_ca; scale = 1.0; }
+    }
+}
+
+// OSF: 0 CA / GO / SO only; 4..64 OS with its cells sorted in OSF registers;
+// 2 OS ranked in scratch
+template <int OSF>
+__global__ __launch_bounds__(TB) void k_cfar_test(const double *__restrict__ Xall, int nbins, int nframes, int mode,
+                                                  int N, int os_k, int n_guard, double alpha, double alpha_ca,
+                                                  double et, const int *__restrict__ cells,
+                                                  const unsigned char *__restrict__ ok,
+                                                  const unsigned char *__restrict__ sided,
+                                                  const unsigned char *__restrict__ reg, double *__restrict__ thr,
+                                                  double *__restrict__ noise, unsigned char *__restrict__ hits) {
+    __shared__ double tile[TB + 2 * TILE_R];
+    const int i = blockIdx.x * TB + threadIdx.x;
+    const int j = blockIdx.y;
+    const int nr = N / 2, R = n_guard + nr;                 // R <= TILE_R (checked by the launcher)
+    const double *X = Xall + (size_t)j * nbins;
+    const int t0 = blockIdx.x * TB - R;                     // tile[t] = X[t0 + t]
+    for (int t = threadIdx.x; t < TB + 2 * R; t += TB) {
+        const int g = t0 + t;
+        tile[t] = (g >= 0 && g < nbins) ? X[g] : 0.0;
+    }
+    __syncthreads();
+    if (i >= nbins || j >= nframes) return;
+    const size_t o = (size_t)j * nbins + i;
+    if (!ok[i]) {
+        thr[o] = __builtin_nan("");
+        noise[o] = __builtin_nan("");
+        hits[o] = 0;
+        return;
+    }
+    const int *c = cells + (size_t)i * N;
+    const int li = i - t0;                                  // this bin in the tile
+    double z, a = alpha, scale = et;
+    if (reg[i]) {
+        // the plain window from the tile: cell q < nr at i-g-1-q, q >= nr at
+        // i+g+1+(q-nr). The empty asm statements keep this branch apart from
+        // the table branch, so that its reads stay LDS reads.
+        asm volatile("" ::: "memory");
+        cfar_stat<OSF>(
+            [&](int q) -> double { return tile[q < nr ? li - n_guard - 1 - q : li + n_guard + 1 + (q - nr)]; },
+            mode, N, os_k, sided[i] != 0, alpha_ca, z, a, scale);
+        asm volatile("" ::: "memory");
+    } else {
+        cfar_stat<OSF>([&](int q) -> double { return X[c[q]]; }, mode, N, os_k, sided[i] != 0, alpha_ca, z, a,
+                       scale);
     }
     const double t = a * z;
     thr[o] = t;
     noise[o] = z / scale;
-    hits[o] = X[i] > t ? 1 : 0;
+    hits[o] = tile[li] > t ? 1 : 0;
 }
 
 }  // namespace
@@ -153,11 +191,12 @@ extern "C" int atkgpu_cfar_raw(void *s, const float *db, size_t n, double *raw) 
 }
 
 extern "C" int atkgpu_cfar_test(void *s, const double *raw, int nbins, int nframes, int k, int mode, int N, int os_k,
-                                double alpha, double alpha_ca, double et, const int *cells, const unsigned char *ok,
-                                const unsigned char *sided, double *X, double *thr, double *noise,
-                                unsigned char *hits) {
+                                int n_guard, double alpha, double alpha_ca, double et, const int *cells,
+                                const unsigned char *ok, const unsigned char *sided, const unsigned char *reg,
+                                double *X, double *thr, double *noise, unsigned char *hits) {
     if (!raw || nbins < 1 || nframes < 1 || nframes > 65535 || k < 1 || N < 2 || N > 512 || (N & 1) || !cells ||
-        !ok || !sided || !thr || !noise || !hits || (k > 1 && !X) || (mode == 1 && (os_k < 1 || os_k > N)))
+        !ok || !sided || !reg || !thr || !noise || !hits || (k > 1 && !X) || (mode == 1 && (os_k < 1 || os_k > N)) ||
+        n_guard < 0 || n_guard + N / 2 > TILE_R)
         return -1;
     const dim3 grid((unsigned)((nbins + TB - 1) / TB), (unsigned)nframes);
     const double *Xin = raw;
@@ -172,7 +211,7 @@ extern "C" int atkgpu_cfar_test(void *s, const double *raw, int nbins, int nfram
                 : N <= 32 ? k_cfar_test<32>
                 : N <= OS_REG_N ? k_cfar_test<64>
                 : k_cfar_test<2>;
-    hipLaunchKernelGGL(kern, grid, dim3(TB), 0, (hipStream_t)s, Xin, nbins, nframes, mode, N, os_k, alpha, alpha_ca,
-                       et, cells, ok, sided, thr, noise, hits);
+    hipLaunchKernelGGL(kern, grid, dim3(TB), 0, (hipStream_t)s, Xin, nbins, nframes, mode, N, os_k, n_guard, alpha,
+                       alpha_ca, et, cells, ok, sided, reg, thr, noise, hits);
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }

@@ -63,12 +63,12 @@ struct atkdsp_cfar {
     size_t nbins;
     int N;
     double n_eff, K, alpha, alpha_ca, et;
-    uint8_t *mask, *ok, *sided;
+    uint8_t *mask, *ok, *sided, *reg;   /* reg: the table is the plain window */
     int32_t *cells;
     /* device */
     void *stream;
     int32_t *d_cells;
-    uint8_t *d_ok, *d_sided, *d_hits;
+    uint8_t *d_ok, *d_sided, *d_reg, *d_hits;
     float *d_db;
     double *d_raw, *d_hist, *d_X, *d_thr, *d_noise;
     size_t slab;
@@ -118,6 +118,7 @@ static void build_tables(atkdsp_cfar *c) {
     for (long i = 0; i < nb; ++i) {
         c->ok[i] = 0;
         c->sided[i] = 0;
+        c->reg[i] = 0;
         if (c->mask[i]) continue;
         int32_t *out = c->cells + (size_t)i * (size_t)c->N;
         int32_t left[256], right[256], borrowed[512];
@@ -145,6 +146,10 @@ static void build_tables(atkdsp_cfar *c) {
         for (long q = 0; q < nbw; ++q) out[o++] = borrowed[q];
         c->ok[i] = 1;
         c->sided[i] = nbw == 0;
+        int plain = nbw == 0;
+        for (long q = 0; q < nr && plain; ++q)
+            plain = out[q] == (int32_t)(i - ng - 1 - q) && out[nr + q] == (int32_t)(i + ng + 1 + q);
+        c->reg[i] = (uint8_t)plain;
     }
     c->n_ok = c->n_untestable = 0;
     for (long i = 0; i < nb; ++i) {
@@ -156,17 +161,19 @@ static void build_tables(atkdsp_cfar *c) {
 static int upload_tables(atkdsp_cfar *c) {
     return atkgpu_h2d(c->stream, c->d_cells, c->cells, c->nbins * (size_t)c->N * sizeof(int32_t)) ||
            atkgpu_h2d(c->stream, c->d_ok, c->ok, c->nbins) || atkgpu_h2d(c->stream, c->d_sided, c->sided, c->nbins) ||
+           atkgpu_h2d(c->stream, c->d_reg, c->reg, c->nbins) ||
            atkgpu_sync(c->stream);
 }
 
 void atkdsp_cfar_destroy(atkdsp_cfar *c) {
     if (!c) return;
     if (c->stream) atkgpu_sync(c->stream);
-    atkgpu_dfree(c->d_cells); atkgpu_dfree(c->d_ok); atkgpu_dfree(c->d_sided); atkgpu_dfree(c->d_hits);
+    atkgpu_dfree(c->d_cells); atkgpu_dfree(c->d_ok); atkgpu_dfree(c->d_sided); atkgpu_dfree(c->d_reg);
+    atkgpu_dfree(c->d_hits);
     atkgpu_dfree(c->d_db); atkgpu_dfree(c->d_raw); atkgpu_dfree(c->d_hist); atkgpu_dfree(c->d_X);
     atkgpu_dfree(c->d_thr); atkgpu_dfree(c->d_noise);
     if (c->stream) atkgpu_stream_destroy(c->stream);
-    free(c->mask); free(c->ok); free(c->sided); free(c->cells);
+    free(c->mask); free(c->ok); free(c->sided); free(c->reg); free(c->cells);
     atkgpu_hfree(c->X); atkgpu_hfree(c->thr); atkgpu_hfree(c->noise); atkgpu_hfree(c->raw); atkgpu_hfree(c->hits);
     free(c->ring); free(c->ring_t); free(c->ring_dt);
     free(c->last_X); free(c->last_thr); free(c->last_hits);
@@ -207,6 +214,7 @@ atkdsp_cfar *atkdsp_cfar_create(size_t nbins, const atkdsp_cfar_config *cfg) {
     c->mask = (uint8_t *)calloc(nb, 1);
     c->ok = (uint8_t *)calloc(nb, 1);
     c->sided = (uint8_t *)calloc(nb, 1);
+    c->reg = (uint8_t *)calloc(nb, 1);
     c->cells = (int32_t *)calloc(nb * (size_t)c->N, sizeof(int32_t));
     c->X = (double *)atkgpu_hmalloc(slab * nb * sizeof(double));
     c->thr = (double *)atkgpu_hmalloc(nb * sizeof(double));
@@ -222,13 +230,14 @@ atkdsp_cfar *atkdsp_cfar_create(size_t nbins, const atkdsp_cfar_config *cfg) {
     c->beta = (double *)malloc((nb + 1) * sizeof(double));
     c->tracks = (cfar_track *)calloc((size_t)c->cfg.max_tracks, sizeof(cfar_track));
     c->blobs = (cfar_blob *)calloc(nb, sizeof(cfar_blob));
-    int ok = c->mask && c->ok && c->sided && c->cells && c->X && c->thr && c->noise && (c->raw || k == 1) && c->hits &&
+    int ok = c->mask && c->ok && c->sided && c->reg && c->cells && c->X && c->thr && c->noise && (c->raw || k == 1) && c->hits &&
              c->ring && c->ring_t && c->ring_dt && c->last_X && c->last_thr && c->last_hits && c->beta &&
              c->tracks && c->blobs && !atkgpu_stream_create(&c->stream);
     if (ok) {
         c->d_cells = (int32_t *)atkgpu_dmalloc(nb * (size_t)c->N * sizeof(int32_t));
         c->d_ok = (uint8_t *)atkgpu_dmalloc(nb);
         c->d_sided = (uint8_t *)atkgpu_dmalloc(nb);
+        c->d_reg = (uint8_t *)atkgpu_dmalloc(nb);
         c->d_hits = (uint8_t *)atkgpu_dmalloc(slab * nb);
         c->d_db = (float *)atkgpu_dmalloc(slab * nb * sizeof(float));
         c->d_raw = (double *)atkgpu_dmalloc((k - 1 + slab) * nb * sizeof(double));
@@ -236,7 +245,7 @@ atkdsp_cfar *atkdsp_cfar_create(size_t nbins, const atkdsp_cfar_config *cfg) {
         c->d_X = k > 1 ? (double *)atkgpu_dmalloc(slab * nb * sizeof(double)) : NULL;
         c->d_thr = (double *)atkgpu_dmalloc(slab * nb * sizeof(double));
         c->d_noise = (double *)atkgpu_dmalloc(slab * nb * sizeof(double));
-        ok = c->d_cells && c->d_ok && c->d_sided && c->d_hits && c->d_db && c->d_raw && c->d_hist && (c->d_X || k == 1) &&
+        ok = c->d_cells && c->d_ok && c->d_sided && c->d_reg && c->d_hits && c->d_db && c->d_raw && c->d_hist && (c->d_X || k == 1) &&
              c->d_thr && c->d_noise;
     }
     if (ok && k > 1) ok = !atkgpu_fill_u32(c->stream, c->d_hist, 0, (k - 1) * nb * 2) && !atkgpu_sync(c->stream);
@@ -527,9 +536,9 @@ static ptrdiff_t run_slab(atkdsp_cfar *c, size_t nf, size_t f0, double frame_t0,
     const double *dX = k > 1 ? c->d_X : c->d_raw;
     if ((k > 1 && atkgpu_d2d(c->stream, c->d_raw, c->d_hist, (k - 1) * nb * sizeof(double))) ||
         atkgpu_cfar_raw(c->stream, c->d_db, nf * nb, c->d_raw + (k - 1) * nb) ||
-        atkgpu_cfar_test(c->stream, c->d_raw, (int)nb, (int)nf, (int)k, c->cfg.mode, c->N, c->cfg.os_k, c->alpha,
-                         c->alpha_ca, c->et, c->d_cells, c->d_ok, c->d_sided, c->d_X, c->d_thr, c->d_noise,
-                         c->d_hits) ||
+        atkgpu_cfar_test(c->stream, c->d_raw, (int)nb, (int)nf, (int)k, c->cfg.mode, c->N, c->cfg.os_k,
+                         c->cfg.n_guard, c->alpha, c->alpha_ca, c->et, c->d_cells, c->d_ok, c->d_sided, c->d_reg,
+                         c->d_X, c->d_thr, c->d_noise, c->d_hits) ||
         (k > 1 && atkgpu_d2d(c->stream, c->d_hist, c->d_raw + nf * nb, (k - 1) * nb * sizeof(double))) ||
         atkgpu_d2h(c->stream, c->X, dX, nf * nb * sizeof(double)) ||
         atkgpu_d2h(c->stream, c->thr, c->d_thr + (nf - 1) * nb, nb * sizeof(double)) ||
