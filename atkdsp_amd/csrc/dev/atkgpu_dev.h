@@ -294,6 +294,8 @@ int atkgpu_cfar_raw(void *s, const float *db, size_t n, double *raw);
  * plain window at n_guard (left i-g-1-q, right i+g+1+q). Writes X (k > 1),
  * thr, noise (nframes x nbins) and hits. mode 0 CA, 1 OS, 2 GO, 3 SO;
  * N <= 512, n_guard + N/2 <= 512. */
+/* out[2q], out[2q+1] = X[idx[q]], noise[idx[q]] for q < n (all device) */
+int atkgpu_cfar_gather(void *s, const double *X, const double *noise, const unsigned *idx, size_t n, double *out);
 int atkgpu_cfar_test(void *s, const double *raw, int nbins, int nframes, int k, int mode, int N, int os_k,
                      int n_guard, double alpha, double alpha_ca, double et, const int *cells, const unsigned char *ok,
                      const unsigned char *sided, const unsigned char *reg, double *X, double *thr, double *noise,

@@ -182,7 +182,27 @@ __global__ __launch_bounds__(TB) void k_cfar_test(const double *__restrict__ Xal
     hits[o] = tile[li] > t ? 1 : 0;
 }
 
+// out[2q], out[2q+1] = X[idx[q]], noise[idx[q]]: the values the tracker reads,
+// at the hit bins only (idx: frame * nbins + bin)
+__global__ void k_cfar_gather(const double *__restrict__ X, const double *__restrict__ noise,
+                              const unsigned *__restrict__ idx, size_t n, double *__restrict__ out) {
+    const size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const size_t i = idx[q];
+    out[2 * q] = X[i];
+    out[2 * q + 1] = noise[i];
+}
+
 }  // namespace
+
+extern "C" int atkgpu_cfar_gather(void *s, const double *X, const double *noise, const unsigned *idx, size_t n,
+                                  double *out) {
+    if (!X || !noise || !idx || !out) return -1;
+    if (!n) return 0;
+    hipLaunchKernelGGL(k_cfar_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)s, X, noise, idx,
+                       n, out);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
 
 extern "C" int atkgpu_cfar_raw(void *s, const float *db, size_t n, double *raw) {
     if (!db || !raw || !n) return -1;

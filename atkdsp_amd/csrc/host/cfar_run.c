@@ -77,6 +77,11 @@ struct atkdsp_cfar {
      * threshold of the slab's last frame */
     double *X, *thr, *noise, *raw;
     uint8_t *hits;
+    /* with one frame averaged the tracker reads X and noise at hit bins only:
+     * their indices go up and the values come back gathered, up to gcap */
+    unsigned *idx, *d_idx;
+    double *gath, *d_gath;
+    size_t gcap;
     size_t n_ok, n_untestable;   /* bins tested / unmasked but lacking cells, per frame */
     /* the last k frames' power and times, ring by frame number */
     double *ring, *ring_t, *ring_dt;
@@ -171,7 +176,8 @@ void atkdsp_cfar_destroy(atkdsp_cfar *c) {
     atkgpu_dfree(c->d_cells); atkgpu_dfree(c->d_ok); atkgpu_dfree(c->d_sided); atkgpu_dfree(c->d_reg);
     atkgpu_dfree(c->d_hits);
     atkgpu_dfree(c->d_db); atkgpu_dfree(c->d_raw); atkgpu_dfree(c->d_hist); atkgpu_dfree(c->d_X);
-    atkgpu_dfree(c->d_thr); atkgpu_dfree(c->d_noise);
+    atkgpu_dfree(c->d_thr); atkgpu_dfree(c->d_noise); atkgpu_dfree(c->d_idx); atkgpu_dfree(c->d_gath);
+    atkgpu_hfree(c->idx); atkgpu_hfree(c->gath);
     if (c->stream) atkgpu_stream_destroy(c->stream);
     free(c->mask); free(c->ok); free(c->sided); free(c->reg); free(c->cells);
     atkgpu_hfree(c->X); atkgpu_hfree(c->thr); atkgpu_hfree(c->noise); atkgpu_hfree(c->raw); atkgpu_hfree(c->hits);
@@ -221,6 +227,11 @@ atkdsp_cfar *atkdsp_cfar_create(size_t nbins, const atkdsp_cfar_config *cfg) {
     c->noise = (double *)atkgpu_hmalloc(slab * nb * sizeof(double));
     c->raw = k > 1 ? (double *)atkgpu_hmalloc(slab * nb * sizeof(double)) : NULL;
     c->hits = (uint8_t *)atkgpu_hmalloc(slab * nb + 8);
+    c->gcap = k == 1 ? (slab * nb + 3) / 4 : 0;
+    if (c->gcap) {
+        c->idx = (unsigned *)atkgpu_hmalloc(c->gcap * sizeof(unsigned));
+        c->gath = (double *)atkgpu_hmalloc(2 * c->gcap * sizeof(double));
+    }
     c->ring = (double *)calloc(k * nb, sizeof(double));
     c->ring_t = (double *)calloc(k, sizeof(double));
     c->ring_dt = (double *)calloc(k, sizeof(double));
@@ -245,8 +256,12 @@ atkdsp_cfar *atkdsp_cfar_create(size_t nbins, const atkdsp_cfar_config *cfg) {
         c->d_X = k > 1 ? (double *)atkgpu_dmalloc(slab * nb * sizeof(double)) : NULL;
         c->d_thr = (double *)atkgpu_dmalloc(slab * nb * sizeof(double));
         c->d_noise = (double *)atkgpu_dmalloc(slab * nb * sizeof(double));
+        if (c->gcap) {
+            c->d_idx = (unsigned *)atkgpu_dmalloc(c->gcap * sizeof(unsigned));
+            c->d_gath = (double *)atkgpu_dmalloc(2 * c->gcap * sizeof(double));
+        }
         ok = c->d_cells && c->d_ok && c->d_sided && c->d_reg && c->d_hits && c->d_db && c->d_raw && c->d_hist && (c->d_X || k == 1) &&
-             c->d_thr && c->d_noise;
+             c->d_thr && c->d_noise && (!c->gcap || (c->idx && c->gath && c->d_idx && c->d_gath));
     }
     if (ok && k > 1) ok = !atkgpu_fill_u32(c->stream, c->d_hist, 0, (k - 1) * nb * 2) && !atkgpu_sync(c->stream);
     if (!ok) { atkdsp_cfar_destroy(c); return NULL; }
@@ -358,8 +373,9 @@ static void emit_close(atkdsp_cfar *c, cfar_track *t, uint32_t extra, atkdsp_cfa
 }
 
 /* one tested frame: X, noise, hits and pw (this frame's power; NULL with one
- * frame averaged, where no band test runs) are the device's results; returns
- * the events written */
+ * frame averaged, where no band test runs) are the device's results, X and
+ * noise valid at least at the hit bins (the only ones read here); returns the
+ * events written */
 static size_t track_frame(atkdsp_cfar *c, const double *X, const double *noise, const uint8_t *hits,
                           const double *pw, double tf, double dt, atkdsp_cfar_event *ev) {
     const size_t nb = c->nbins;
@@ -540,13 +556,44 @@ static ptrdiff_t run_slab(atkdsp_cfar *c, size_t nf, size_t f0, double frame_t0,
                          c->cfg.n_guard, c->alpha, c->alpha_ca, c->et, c->d_cells, c->d_ok, c->d_sided, c->d_reg,
                          c->d_X, c->d_thr, c->d_noise, c->d_hits) ||
         (k > 1 && atkgpu_d2d(c->stream, c->d_hist, c->d_raw + nf * nb, (k - 1) * nb * sizeof(double))) ||
-        atkgpu_d2h(c->stream, c->X, dX, nf * nb * sizeof(double)) ||
         atkgpu_d2h(c->stream, c->thr, c->d_thr + (nf - 1) * nb, nb * sizeof(double)) ||
-        atkgpu_d2h(c->stream, c->noise, c->d_noise, nf * nb * sizeof(double)) ||
         atkgpu_d2h(c->stream, c->hits, c->d_hits, nf * nb) ||
-        (k > 1 && atkgpu_d2h(c->stream, c->raw, c->d_raw + (k - 1) * nb, nf * nb * sizeof(double))) ||
-        atkgpu_sync(c->stream))
+        (k > 1 && atkgpu_d2h(c->stream, c->raw, c->d_raw + (k - 1) * nb, nf * nb * sizeof(double))))
         return ATKDSP_E_FFT;
+    size_t nh = 0;
+    if (c->gcap) {
+        /* one frame averaged: the last frame's X whole (for atkdsp_cfar_last),
+         * then X and noise at the hit bins only, gathered on the device */
+        if (atkgpu_d2h(c->stream, c->X + (nf - 1) * nb, dX + (nf - 1) * nb, nb * sizeof(double)) ||
+            atkgpu_sync(c->stream))
+            return ATKDSP_E_FFT;
+        const size_t tot = nf * nb;
+        for (size_t i = 0; i < tot && nh <= c->gcap;) {
+            if (i + 8 <= tot) {                                /* whole zero words at a time */
+                uint64_t w;
+                memcpy(&w, c->hits + i, 8);
+                if (!w) { i += 8; continue; }
+            }
+            if (c->hits[i]) {
+                if (nh < c->gcap) c->idx[nh] = (unsigned)i;
+                ++nh;
+            }
+            ++i;
+        }
+    }
+    if (c->gcap && nh <= c->gcap) {
+        if (nh && (atkgpu_h2d(c->stream, c->d_idx, c->idx, nh * sizeof(unsigned)) ||
+                   atkgpu_cfar_gather(c->stream, dX, c->d_noise, c->d_idx, nh, c->d_gath) ||
+                   atkgpu_d2h(c->stream, c->gath, c->d_gath, 2 * nh * sizeof(double)) || atkgpu_sync(c->stream)))
+            return ATKDSP_E_FFT;
+        for (size_t q = 0; q < nh; ++q) {
+            c->X[c->idx[q]] = c->gath[2 * q];
+            c->noise[c->idx[q]] = c->gath[2 * q + 1];
+        }
+    } else if (atkgpu_d2h(c->stream, c->X, dX, nf * nb * sizeof(double)) ||
+               atkgpu_d2h(c->stream, c->noise, c->d_noise, nf * nb * sizeof(double)) || atkgpu_sync(c->stream)) {
+        return ATKDSP_E_FFT;
+    }
     size_t nev = 0;
     for (size_t j = 0; j < nf; ++j) {
         const double tf = frame_t0 + (double)(f0 + j) * frame_dt;
