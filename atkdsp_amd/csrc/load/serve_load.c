@@ -42,7 +42,11 @@
  * lines it covered, the channels found and the call latency. With --detect 2
  * the consumer instead asks every 50 ms (one line period) for the newest line
  * of every stream at once (atkgpu_server_detect_latest: one launch), and the
- * latency is that of the whole query.
+ * latency is that of the whole query. With --detect 3 the consumer runs CFAR
+ * (atkgpu_server_cfar: CA, 16 + 16 cells, 4 guard, Pfa 1e-6, two-line
+ * average, the burst tracker) over every completed line of every display
+ * stream, each stream through a handle of its own made before the clock
+ * starts; the JSON's detect_channels then counts the events.
  *
  * Before the clock starts every stream gets one chunk of CHUNK_MAX samples and
  * the server is drained and its statistics reset: the figures are the steady
@@ -52,7 +56,7 @@
  * usage: serve_load [--seconds S] [--streams N] [--speed X] [--budget-gib G]
  *                   [--max-batch B] [--seed K] [--mix c4|uniform] [--arrays A]
  *                   [--display 0|1] [--ring 0|1] [--consumers C] [--ring-blocks B]
- *                   [--detect 0|1|2]
+ *                   [--detect 0|1|2|3]
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -323,6 +327,53 @@ static void *line_detector(void *arg) {
     return NULL;
 }
 
+/* --detect 3: every completed line of every display stream through that
+ * stream's CFAR handle, on the device; a stream that fell behind the line ring
+ * skips to its newest line and its handle starts afresh */
+static atkdsp_cfar **g_cfar;
+static void *line_cfar(void *arg) {
+    (void)arg;
+    long long *next = (long long *)calloc((size_t)g_ns, sizeof(long long));
+    atkdsp_cfar_event *ev = (atkdsp_cfar_event *)malloc(4 * 256 * sizeof(atkdsp_cfar_event));
+    for (;;) {
+        const int done = g_producers_done;
+        int took = 0;
+        for (int i = 0; i < g_ns; ++i) {
+            if (!g_cfar[i]) continue;
+            const long long nl = atkgpu_server_lines(g_srv, g_st[i].sid);
+            if (nl <= next[i]) continue;
+            if (nl - next[i] > 3) {
+                g_det_missed += nl - 1 - next[i];
+                next[i] = nl - 1;
+                atkdsp_cfar_reset(g_cfar[i]);
+            }
+            const size_t n = (size_t)(nl - next[i]);
+            const double t0 = now_s();
+            const ptrdiff_t got = atkgpu_server_cfar(g_srv, g_st[i].sid, 0, g_cfar[i], next[i], n,
+                                                     0.05 * (double)next[i], 0.05, ev, 4 * 256);
+            const double t1 = now_s();
+            if (got < 0) {                            /* overwritten meanwhile: count them missed */
+                g_det_missed += (long long)n;
+                atkdsp_cfar_reset(g_cfar[i]);
+            } else {
+                if (g_det_n < g_det_cap) g_det_lat[g_det_n++] = (t1 - t0) * 1e3;
+                g_det_lines += (long long)n;
+                g_det_chans += got;
+            }
+            next[i] += (long long)n;
+            took = 1;
+        }
+        if (!took) {
+            if (done) break;
+            struct timespec ts = {0, 200000L};
+            nanosleep(&ts, NULL);
+        }
+    }
+    free(ev);
+    free(next);
+    return NULL;
+}
+
 /* --detect 2: the newest line of every display stream, every 50 ms, one query */
 static void *line_detector_latest(void *arg) {
     (void)arg;
@@ -461,6 +512,14 @@ int main(int argc, char **argv) {
         g_det_cap = 1u << 22;
         g_det_lat = (double *)malloc(g_det_cap * sizeof(double));
     }
+    if (g_detect == 3 && display) {
+        atkdsp_cfar_config ccfg;
+        atkdsp_cfar_defaults(&ccfg);
+        ccfg.k_frames = 2;
+        g_cfar = (atkdsp_cfar **)calloc((size_t)g_ns, sizeof(atkdsp_cfar *));
+        for (int i = 0; i < admitted; ++i)
+            if (g_st[i].nel == 1) g_cfar[i] = atkdsp_cfar_create(1024, &ccfg);
+    }
     struct rusage ru0, ru1;
     getrusage(RUSAGE_SELF, &ru0);
     const double w0 = now_s();
@@ -468,7 +527,9 @@ int main(int argc, char **argv) {
     for (long t = 0; t < NTHREADS; ++t) pthread_create(&th[t], NULL, submitter, (void *)t);
     if (g_ring)
         for (long t = 0; t < g_consumers; ++t) pthread_create(&rc[t], NULL, ring_consumer, (void *)t);
-    if (g_det_lat) pthread_create(&dth, NULL, g_detect == 2 ? line_detector_latest : line_detector, NULL);
+    if (g_det_lat)
+        pthread_create(&dth, NULL, g_detect == 3 ? line_cfar : g_detect == 2 ? line_detector_latest : line_detector,
+                       NULL);
     for (int t = 0; t < NTHREADS; ++t) pthread_join(th[t], NULL);
     g_producers_done = 1;
     if (g_ring)
@@ -509,6 +570,9 @@ int main(int argc, char **argv) {
            g_picked ? 1e3 * g_pick_sum / (double)g_picked : 0.0, 1e3 * g_pick_max,
            g_picked ? 1e6 * g_call_sum / (double)g_picked : 0.0, 1e3 * g_call_max, g_ring ? g_consumers : 0, g_ring ? g_ring_blocks : 0, 1e3 * g_pass_max, 1e3 * g_late_max,
            g_det_lat ? g_detect : 0, g_det_lines, g_det_missed, g_det_chans, dp50, dp99, dmax, cpus);
+    if (g_cfar)
+        for (int i = 0; i < g_ns; ++i) atkdsp_cfar_destroy(g_cfar[i]);
+    free(g_cfar);
     atkgpu_server_destroy(g_srv);
     for (int s = 0; s < admitted; ++s) {
         free(g_st[s].src);
