@@ -77,7 +77,11 @@ __global__ void __launch_bounds__(256) k_chain_unpack(ChainUnpackArgs a, float c
     const long long stride = (long long)gridDim.x * 256;
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < a.n; i += stride) {
         const cpx o = chain_conv<FMT>(raw, i, cre, cim, kk, ore, oim);
-        row[i] = o;
+        // streamed past the caches: the row is read back by the filterbank
+        // and display kernels only after every block is written
+        unsigned long long bits;
+        __builtin_memcpy(&bits, &o, 8);
+        __builtin_nontemporal_store(bits, (unsigned long long *)&row[i]);
         sr += (double)o.x;
         si += (double)o.y;
     }
