@@ -176,7 +176,11 @@ ptrdiff_t atkgpu_chain_run(atkgpu_chain *c, size_t n) {
                               NULL, 0, c->d_out, (m + 15) & ~(size_t)15, 0);
     if (!rc) rc = atkgpu_event_record(c->ev[2], c->stream);
     if (!rc && frames) {
-        int nfb = (int)((2048 + c->S - 1) / c->S);
+        /* about 512 display blocks in all: the 4096-point reduction holds two
+         * blocks per CU, so on 256 CUs every block is resident at once and
+         * each transforms a long run of frames with its twiddles in registers
+         * (measured 0.762 ms a step against 0.792 ms with 2048 blocks) */
+        int nfb = (int)((512 + c->S - 1) / c->S);
         if (nfb > c->nfb_max) nfb = c->nfb_max;
         if (nfb < 1) nfb = 1;
         rc = atkgpu_chain_display(c->stream, R + (keep - c->dlen), c->spitch, c->S, (size_t)frames, g->fft_hop,
