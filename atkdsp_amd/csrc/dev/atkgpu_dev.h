@@ -244,6 +244,13 @@ int atkgpu_disp_teams(unsigned N);          /* frames one block of the frames la
 int atkgpu_disp_frames(void *s, const void *blks, int nblk, unsigned N, const void *tw);
 int atkgpu_disp_lines(void *s, const void *blks, int nblk, unsigned N, int det);
 int atkgpu_pfb_group(unsigned M, unsigned os);      /* hops per block of the fused kernel */
+/* the matrix-core FP32 filterbank (pfb_mm.hpp): same arguments and output as
+ * atkgpu_pfb_fused; atkgpu_pfb_mm_k gives the tap rows per branch it runs for
+ * (M, os, kos), 0 when the shape is not covered */
+int atkgpu_pfb_mm_k(unsigned M, unsigned os, unsigned kos);
+int atkgpu_pfb_mm(void *s, const void *sbuf, size_t spitch, size_t nvalid, unsigned nel, size_t m,
+                  long long b0, unsigned M, unsigned os, unsigned kos, const float *taps, const void *tw,
+                  const void *cal, unsigned cal_nel, void *out, size_t opitch, size_t ocol);
 int atkgpu_pfb_fused_jobs(void *s, const void *jobs, int njobs, int nblocks, unsigned M, unsigned os,
                           unsigned kos, const void *tw);
 /* any mix of formats and prefix copies; max_grid > 0 caps the launch's blocks */

@@ -100,9 +100,10 @@ atkgpu_pfbx *atkgpu_pfbx_create(const atkgpu_pfbx_cfg *cfg, unsigned S) {
     if (!cfg || S < 1 || !(cfg->fs > 0) || cfg->block_max < 8 || (cfg->block_max & 7)) return NULL;
     const int bps = atkdsp_bytes_per_sample(cfg->fmt);
     const int kos = atkgpu_pfb_fused_kos(cfg->nchan, cfg->oversample, cfg->taps_per_poly);
-    if (!bps || !kos || cfg->taps_per_poly < 2 || cfg->form < ATKGPU_FORM_FP32 || cfg->form > ATKGPU_FORM_FP16)
+    if (!bps || !kos || cfg->taps_per_poly < 2 || cfg->form < ATKGPU_FORM_FP32 || cfg->form > ATKGPU_FORM_FP32MM)
         return NULL;
-    const int mx = cfg->form != ATKGPU_FORM_FP32;
+    const int mx = cfg->form != ATKGPU_FORM_FP32 && cfg->form != ATKGPU_FORM_FP32MM;
+    if (cfg->form == ATKGPU_FORM_FP32MM && !atkgpu_pfb_mm_k(cfg->nchan, cfg->oversample, (unsigned)kos)) return NULL;
     if (mx && !atkgpu_pfb_mx_supported(cfg->nchan, cfg->oversample, (unsigned)kos, cfg->fmt, cfg->form)) return NULL;
     if (atkgpu_init() != 0) return NULL;
     atkgpu_pfbx *v = (atkgpu_pfbx *)calloc(1, sizeof *v);
@@ -282,6 +283,9 @@ ptrdiff_t atkgpu_pfbx_run(atkgpu_pfbx *v, size_t n) {
             rc = atkgpu_pfb_mx(v->stream, base, v->spitch, (size_t)v->ov + avail, v->S, m, v->b, v->M, v->os,
                                (unsigned)v->kos, v->cfg.fmt, v->cfg.form, v->d_atab, v->d_tapsum, v->d_taps, zlo,
                                v->scale, v->d_offs, tw, NULL, 0, v->d_out, mp);
+        else if (v->cfg.form == ATKGPU_FORM_FP32MM)
+            rc = atkgpu_pfb_mm(v->stream, base, v->spitch, (size_t)v->ov + avail, v->S, m, v->b, v->M, v->os,
+                               (unsigned)v->kos, v->d_taps, tw, NULL, 0, v->d_out, mp, 0);
         else
             rc = atkgpu_pfb_fused(v->stream, base, v->spitch, (size_t)v->ov + avail, v->S, m, v->b, v->M, v->os,
                                   (unsigned)v->kos, v->d_taps, tw, NULL, 0, v->d_out, mp, 0);

@@ -208,12 +208,16 @@ ATKGPU_API int       atkgpu_server_detect_latest(atkgpu_server *v, const int *si
  * rotation and channel-major store are the FP32 path's. The offset is fixed
  * per stream (unpack_dc's offset argument). form ATKGPU_FORM_FP32 runs the
  * plain FP32 path (unpack, then the fused FP32 filterbank) on the same
- * plumbing, the form the others are measured against.
+ * plumbing, the form the others are measured against; ATKGPU_FORM_FP32MM is
+ * that path with the FP32 fold on the matrix cores (4x4x1 FP32 products,
+ * one rounding a tap, taps in ascending order: bit-identical to the FP32
+ * form for finite input; 256, 512 or 1024 channels at 2x, 256 or 512 at 1x,
+ * up to 16 taps/branch).
  * Supported shapes: 64 channels 1x with up to 8 taps/branch (every form and
  * format), 64 channels 1x up to 16 or 32 taps, 64 channels 2x up to 8 taps,
  * 128 channels 2x up to 8 taps (BF16 hi+lo for 8-bit, FP16 hi+lo for 12-bit). */
 enum { ATKGPU_FORM_FP32 = 0, ATKGPU_FORM_BF16X2 = 1, ATKGPU_FORM_BF16 = 2, ATKGPU_FORM_FP16X2 = 3,
-       ATKGPU_FORM_FP16 = 4 };
+       ATKGPU_FORM_FP16 = 4, ATKGPU_FORM_FP32MM = 5 };
 typedef struct {
     int fmt;                   /* ATKDSP_FMT_CU8, _CI8 or _CI16Q11 (FP32 form: any) */
     double fs;
