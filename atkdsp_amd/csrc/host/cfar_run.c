@@ -5,19 +5,24 @@
  * power, averages the last k_frames frames and runs the CA / OS / GO / SO test
  * of every bin against its reference cells (csrc/dev/cfar.hip), and returns
  * each frame's average, threshold, noise estimate, hits and power. The
- * tracker then runs on the host, frame by frame, as the header defines it:
- *   blobs        hit bins closer than merge_gap non-hit bins, min_bins wide;
- *   association  each blob to the track it overlaps most within merge_gap+1
- *                bins (ties to the older), a track may take several blobs; a
- *                blob near no track starts a pending one (beyond max_tracks
- *                it is dropped and counted);
- *   life         OPEN after open_m hit frames in the last open_n, forgotten
- *                when pending with none; CLOSE after close_k misses in a
- *                row; UPDATE when an open track's span grows; one event per
- *                track per frame, tracks visited in id order;
- *   times        with k_frames > 1 the first and last hit frames are refined
- *                by a single-frame band test at ATKDSP_CFAR_EDGE_PFA
- *                (T0_COARSE / T1_COARSE when it fails).
+ * tracker then follows, frame by frame, on the host. Its rules are the
+ * header's; its steps here are
+ *   runs       the frame's hit bins cut into runs wherever more than
+ *              merge_gap non-hit bins separate two hits; runs narrower than
+ *              min_bins are ignored; each run's excess-weighted bin sums, its
+ *              power and noise sums and its strongest bin are taken in bin order;
+ *   assign     each run to the track (among those alive when the frame came)
+ *              whose latest span it overlaps most, within merge_gap + 1 bins;
+ *              equal overlaps go to the older track;
+ *   spawn      a pending track for every run left over, in run order, up to
+ *              max_tracks (the rest are counted as dropped);
+ *   observe    a track's runs of the frame merged into one observation;
+ *   step       every track in id order: its hit history over open_n frames,
+ *              OPEN at open_m hits, forgotten when pending with none left,
+ *              CLOSE after close_k misses in a row, UPDATE when an open
+ *              track's span grows; its first and last times, refined with
+ *              k_frames > 1 by the single-frame band test at
+ *              ATKDSP_CFAR_EDGE_PFA (T0_COARSE / T1_COARSE when it fails).
  * The tables of reference cells (left nearest first, right nearest first,
  * then a side's shortfall borrowed from beyond the other side) are built on
  * the host when the mask changes. */
@@ -32,31 +37,40 @@ _Static_assert(sizeof(atkdsp_cfar_counters) == 88, "counters: 88 bytes, as the r
 
 #define SLAB_VALUES ((size_t)1 << 22)   /* frame values per device slab */
 
+enum { TRK_PENDING = 0, TRK_OPEN = 1 };
+
+/* a tracked emitter */
 typedef struct {
     uint32_t id;
-    int open;                 /* 0 pending, 1 open */
-    int fresh;                /* no frame processed since it started */
-    uint64_t window;          /* hit frames, newest in bit 0 */
-    int misses;
-    int32_t n_hits, cur_lo, cur_hi, lo, hi;   /* this frame's span, the union of spans */
-    double sum_be, sum_e;     /* excess-weighted bin sum and excess over the track's life */
-    double peak_x, peak_n;    /* the strongest averaged bin and its noise */
-    double band;              /* best frame's hit-bin power over noise */
+    int state;                /* TRK_PENDING or TRK_OPEN */
+    int unseen;               /* created this frame, no observation applied yet */
+    uint64_t hist;            /* hit frames over the last open_n, newest in bit 0 */
+    int miss_run;             /* frames in a row without a hit */
+    int32_t hits_total;
+    int32_t now_lo, now_hi;   /* the latest observation's span */
+    int32_t lo, hi;           /* the union of every observed span */
+    double wsum, esum;        /* sum of bin * excess and of excess, over the track's life */
+    double pk_x, pk_n;        /* the strongest averaged bin seen, and its noise */
+    double best_band;         /* the best frame's power / noise over its hit bins */
     uint32_t flags;
-    double t0;
-    int64_t last_hit, last_raw;
-    double t_hit_end, dt_hit, t_raw_end;
-    /* this frame's observation */
-    int seen;
-    int32_t obs_lo, obs_hi;
-    double obs_x, obs_n;
-} cfar_track;
+    double t_start;
+    int64_t f_hit, f_raw;     /* the last frame with a hit / whose single frame passed the band test */
+    double end_hit, dt_hit, end_raw;
+} cfar_trk;
 
+/* one frame's observation of a track, the merge of its runs */
+typedef struct {
+    int any;
+    int32_t lo, hi;
+    double xsum, nsum;
+} cfar_obs;
+
+/* a run of hit bins in one frame */
 typedef struct {
     int32_t lo, hi;
-    double sum_be, sum_e, sum_x, sum_n, peak_x, peak_n;
-    int track;
-} cfar_blob;
+    double wsum, esum, xsum, nsum, pk_x, pk_n;
+    int owner;                /* the track it belongs to, -1 before assignment */
+} cfar_run;
 
 struct atkdsp_cfar {
     atkdsp_cfar_config cfg;
@@ -91,10 +105,11 @@ struct atkdsp_cfar {
     uint8_t *last_hits;
     int has_last;
     double *beta;             /* per band width, NaN until needed */
-    cfar_track *tracks;
-    int ntracks;
+    cfar_trk *trk;            /* live tracks in id order */
+    cfar_obs *obs;            /* this frame's observation of trk[i] */
+    int ntrk;
     uint32_t next_id;
-    cfar_blob *blobs;
+    cfar_run *runs;
     atkdsp_cfar_counters st;
 };
 
@@ -183,8 +198,25 @@ void atkdsp_cfar_destroy(atkdsp_cfar *c) {
     atkgpu_hfree(c->X); atkgpu_hfree(c->thr); atkgpu_hfree(c->noise); atkgpu_hfree(c->raw); atkgpu_hfree(c->hits);
     free(c->ring); free(c->ring_t); free(c->ring_dt);
     free(c->last_X); free(c->last_thr); free(c->last_hits);
-    free(c->beta); free(c->tracks); free(c->blobs);
+    free(c->beta); free(c->trk); free(c->obs); free(c->runs);
     free(c);
+}
+
+/* What a configuration implies: the OS rank when left to the default (the
+ * ceiling of 3N/4), K = looks x frames averaged, the effective cell count,
+ * and the thresholds (GO and SO fall back to CA's where a side borrowed). */
+static int derive(atkdsp_cfar *c) {
+    atkdsp_cfar_config *g = &c->cfg;
+    c->N = 2 * g->n_ref;
+    if (g->mode == ATKDSP_CFAR_OS && !g->os_k) g->os_k = (3 * c->N + 3) / 4;
+    c->K = g->looks * (double)g->k_frames;
+    c->n_eff = atkdsp_cfar_n_eff(g->n_ref, g->rho2);
+    if (atkdsp_cfar_alpha(g->mode, g->n_ref, g->pfa, c->K, g->os_k, g->rho2, &c->alpha, &c->et) != ATKDSP_OK)
+        return 0;
+    c->alpha_ca = c->alpha;
+    if (g->mode == ATKDSP_CFAR_GO || g->mode == ATKDSP_CFAR_SO)
+        return atkdsp_cfar_alpha(ATKDSP_CFAR_CA, g->n_ref, g->pfa, c->K, 0, g->rho2, &c->alpha_ca, NULL) == ATKDSP_OK;
+    return 1;
 }
 
 atkdsp_cfar *atkdsp_cfar_create(size_t nbins, const atkdsp_cfar_config *cfg) {
@@ -194,23 +226,9 @@ atkdsp_cfar *atkdsp_cfar_create(size_t nbins, const atkdsp_cfar_config *cfg) {
     if (!c) return NULL;
     c->cfg = *cfg;
     c->nbins = nbins;
-    c->N = 2 * cfg->n_ref;
-    if (c->cfg.mode == ATKDSP_CFAR_OS && c->cfg.os_k == 0) c->cfg.os_k = (3 * c->N + 3) / 4;
-    c->K = c->cfg.looks * (double)c->cfg.k_frames;
-    c->n_eff = atkdsp_cfar_n_eff(c->cfg.n_ref, c->cfg.rho2);
-    if (atkdsp_cfar_alpha(c->cfg.mode, c->cfg.n_ref, c->cfg.pfa, c->K, c->cfg.os_k, c->cfg.rho2, &c->alpha,
-                          &c->et) != ATKDSP_OK) {
+    if (!derive(c)) {
         free(c);
         return NULL;
-    }
-    if (c->cfg.mode == ATKDSP_CFAR_GO || c->cfg.mode == ATKDSP_CFAR_SO) {
-        if (atkdsp_cfar_alpha(ATKDSP_CFAR_CA, c->cfg.n_ref, c->cfg.pfa, c->K, 0, c->cfg.rho2, &c->alpha_ca, NULL) !=
-            ATKDSP_OK) {
-            free(c);
-            return NULL;
-        }
-    } else {
-        c->alpha_ca = c->alpha;
     }
     const size_t k = (size_t)c->cfg.k_frames, nb = nbins;
     size_t slab = SLAB_VALUES / nb;
@@ -239,11 +257,12 @@ atkdsp_cfar *atkdsp_cfar_create(size_t nbins, const atkdsp_cfar_config *cfg) {
     c->last_thr = (double *)malloc(nb * sizeof(double));
     c->last_hits = (uint8_t *)malloc(nb);
     c->beta = (double *)malloc((nb + 1) * sizeof(double));
-    c->tracks = (cfar_track *)calloc((size_t)c->cfg.max_tracks, sizeof(cfar_track));
-    c->blobs = (cfar_blob *)calloc(nb, sizeof(cfar_blob));
+    c->trk = (cfar_trk *)calloc((size_t)c->cfg.max_tracks, sizeof(cfar_trk));
+    c->obs = (cfar_obs *)calloc((size_t)c->cfg.max_tracks, sizeof(cfar_obs));
+    c->runs = (cfar_run *)calloc(nb, sizeof(cfar_run));
     int ok = c->mask && c->ok && c->sided && c->reg && c->cells && c->X && c->thr && c->noise && (c->raw || k == 1) && c->hits &&
              c->ring && c->ring_t && c->ring_dt && c->last_X && c->last_thr && c->last_hits && c->beta &&
-             c->tracks && c->blobs && !atkgpu_stream_create(&c->stream);
+             c->trk && c->obs && c->runs && !atkgpu_stream_create(&c->stream);
     if (ok) {
         c->d_cells = (int32_t *)atkgpu_dmalloc(nb * (size_t)c->N * sizeof(int32_t));
         c->d_ok = (uint8_t *)atkgpu_dmalloc(nb);
@@ -274,7 +293,7 @@ atkdsp_cfar *atkdsp_cfar_create(size_t nbins, const atkdsp_cfar_config *cfg) {
 
 void atkdsp_cfar_reset(atkdsp_cfar *c) {
     if (!c) return;
-    c->ntracks = 0;
+    c->ntrk = 0;
     c->seen = 0;
     c->frame = 0;
     memset(c->ring, 0, (size_t)c->cfg.k_frames * c->nbins * sizeof(double));
@@ -326,50 +345,261 @@ static int band_pass(atkdsp_cfar *c, const double *pw, const double *noise, int3
     return nb > 0 && sp > edge_beta(c, nb) * sn;
 }
 
-static uint32_t track_flags(const atkdsp_cfar *c, const cfar_track *t) {
+/* the event flags of a track: its own (coarse times) plus EDGE when a bin of
+ * its span was tested with borrowed cells or not at all, MASK_ADJ when a
+ * masked bin lies within merge_gap + 1 of the span */
+static uint32_t span_flags(const atkdsp_cfar *c, const cfar_trk *t) {
     uint32_t f = t->flags;
-    for (int32_t b = t->lo; b <= t->hi; ++b)
-        if (!c->sided[b] || !c->ok[b]) { f |= ATKDSP_CFAR_EDGE; break; }
-    const long m = c->cfg.merge_gap + 1;
-    long a = (long)t->lo - m, z = (long)t->hi + m + 1;
-    if (a < 0) a = 0;
-    if (z > (long)c->nbins) z = (long)c->nbins;
-    for (long b = a; b < z; ++b)
-        if (c->mask[b]) { f |= ATKDSP_CFAR_MASK_ADJ; break; }
+    int edge = 0, near_mask = 0;
+    for (int32_t b = t->lo; b <= t->hi && !edge; ++b) edge = !c->ok[b] || !c->sided[b];
+    const long reach = (long)c->cfg.merge_gap + 1;
+    const long a = (long)t->lo - reach > 0 ? (long)t->lo - reach : 0;
+    const long z = (long)t->hi + reach < (long)c->nbins - 1 ? (long)t->hi + reach : (long)c->nbins - 1;
+    for (long b = a; b <= z && !near_mask; ++b) near_mask = c->mask[b] != 0;
+    if (edge) f |= ATKDSP_CFAR_EDGE;
+    if (near_mask) f |= ATKDSP_CFAR_MASK_ADJ;
     return f;
 }
 
-static void emit(const atkdsp_cfar *c, const cfar_track *t, int type, double t1, uint32_t extra,
-                 atkdsp_cfar_event *e) {
+static double to_db(double p) { return 10.0 * log10(p); }
+
+static void make_event(const atkdsp_cfar *c, const cfar_trk *t, int type, double t1, uint32_t extra,
+                       atkdsp_cfar_event *e) {
     memset(e, 0, sizeof *e);
     e->type = type;
     e->id = t->id;
-    e->flags = track_flags(c, t) | extra;
+    e->flags = span_flags(c, t) | extra;
     e->lo_bin = t->lo;
     e->hi_bin = t->hi;
-    e->n_hits = t->n_hits;
-    e->t0 = t->t0;
+    e->n_hits = t->hits_total;
+    e->t0 = t->t_start;
     e->t1 = t1;
-    e->fc_bin = t->sum_e > 0.0 ? t->sum_be / t->sum_e : 0.5 * ((double)t->lo + (double)t->hi);
-    e->level_db = t->peak_x > 0.0 ? (float)(10.0 * log10(t->peak_x)) : -INFINITY;
-    e->noise_db = t->peak_n > 0.0 ? (float)(10.0 * log10(t->peak_n)) : -INFINITY;
-    e->snr_db = t->peak_n > 0.0 ? e->level_db - e->noise_db : INFINITY;
-    e->band_snr_db = t->band > 0.0 ? (float)(10.0 * log10(t->band)) : -INFINITY;
+    /* the excess-weighted centre, or the span's middle when nothing exceeded */
+    e->fc_bin = t->esum > 0.0 ? t->wsum / t->esum : 0.5 * ((double)t->lo + (double)t->hi);
+    const float level = t->pk_x > 0.0 ? (float)to_db(t->pk_x) : -INFINITY;
+    const float noise = t->pk_n > 0.0 ? (float)to_db(t->pk_n) : -INFINITY;
+    e->level_db = level;
+    e->noise_db = noise;
+    e->snr_db = t->pk_n > 0.0 ? level - noise : INFINITY;
+    e->band_snr_db = t->best_band > 0.0 ? (float)to_db(t->best_band) : -INFINITY;
 }
 
-static void emit_close(atkdsp_cfar *c, cfar_track *t, uint32_t extra, atkdsp_cfar_event *e) {
+/* CLOSE: the end time is the last hit frame's end; averaged over k frames it
+ * is the last raw frame that passed the band test when that frame belongs to
+ * the last hit's averaging window, else half the window before the hit end
+ * (T1_COARSE) */
+static void close_event(atkdsp_cfar *c, cfar_trk *t, uint32_t extra, atkdsp_cfar_event *e) {
     const int k = c->cfg.k_frames;
-    double t1;
-    if (k == 1) {
-        t1 = t->t_hit_end;
-    } else if (t->last_raw >= t->last_hit - (k - 1)) {
-        t1 = t->t_raw_end;
-    } else {
-        t1 = t->t_hit_end - 0.5 * (double)(k - 1) * t->dt_hit;
-        t->flags |= ATKDSP_CFAR_T1_COARSE;
+    double t1 = t->end_hit;
+    if (k > 1) {
+        if (t->f_raw >= t->f_hit - (k - 1)) {
+            t1 = t->end_raw;
+        } else {
+            t1 = t->end_hit - 0.5 * (double)(k - 1) * t->dt_hit;
+            t->flags |= ATKDSP_CFAR_T1_COARSE;
+        }
     }
     c->st.closed++;
-    emit(c, t, ATKDSP_CFAR_CLOSE, t1, extra, e);
+    make_event(c, t, ATKDSP_CFAR_CLOSE, t1, extra, e);
+}
+
+/* Close the run [lo, last] of one frame: its statistics over its hit bins in
+ * bin order; runs narrower than min_bins are not kept. */
+static void add_run(atkdsp_cfar *c, size_t *n, long lo, long last, const uint8_t *hits, const double *X,
+                    const double *noise) {
+    if (last - lo + 1 < c->cfg.min_bins) return;
+    cfar_run *r = &c->runs[(*n)++];
+    r->lo = (int32_t)lo;
+    r->hi = (int32_t)last;
+    r->wsum = r->esum = r->xsum = r->nsum = 0.0;
+    r->owner = -1;
+    long best = -1;
+    for (long b = lo; b <= last; ++b) {
+        if (!hits[b]) continue;
+        const double ex = X[b] > noise[b] ? X[b] - noise[b] : 0.0;
+        r->wsum += (double)b * ex;
+        r->esum += ex;
+        r->xsum += X[b];
+        r->nsum += noise[b];
+        if (best < 0 || X[b] > X[best]) best = b;
+    }
+    r->pk_x = X[best];
+    r->pk_n = noise[best];
+}
+
+/* Runs of hit bins in one frame, in c->runs: one pass over the hit bins; a
+ * hit more than merge_gap non-hit bins after the previous one starts a new
+ * run. Returns the number kept. */
+static size_t frame_runs(atkdsp_cfar *c, const uint8_t *hits, const double *X, const double *noise) {
+    const long nb = (long)c->nbins, gap = c->cfg.merge_gap;
+    size_t n = 0;
+    long lo = -1, last = -1;
+    long i = 0;
+    while (i < nb) {
+        if (i + 8 <= nb) {                       /* a word of non-hit bins at a time */
+            uint64_t w;
+            memcpy(&w, hits + i, 8);
+            if (!w) {
+                i += 8;
+                continue;
+            }
+        }
+        if (hits[i]) {
+            if (lo >= 0 && i - last - 1 > gap) {
+                add_run(c, &n, lo, last, hits, X, noise);
+                lo = -1;
+            }
+            if (lo < 0) lo = i;
+            last = i;
+        }
+        ++i;
+    }
+    if (lo >= 0) add_run(c, &n, lo, last, hits, X, noise);
+    return n;
+}
+
+/* the overlap of a run with a track's latest span (may be zero or negative
+ * for a run that is merely within reach) */
+static long overlap(const cfar_run *r, const cfar_trk *t) {
+    const long hi = r->hi < t->now_hi ? r->hi : t->now_hi;
+    const long lo = r->lo > t->now_lo ? r->lo : t->now_lo;
+    return hi - lo + 1;
+}
+
+/* Each run to the live track it overlaps most (within reach), the older one
+ * on equal overlap; then a new pending track for each run left over. */
+static void assign_runs(atkdsp_cfar *c, size_t nrun, double dt) {
+    const int alive = c->ntrk;
+    const long reach = (long)c->cfg.merge_gap + 1;
+    for (size_t q = 0; q < nrun; ++q) {
+        cfar_run *r = &c->runs[q];
+        long best = 0;
+        for (int t = 0; t < alive; ++t) {
+            const cfar_trk *tr = &c->trk[t];
+            if ((long)r->lo - (long)tr->now_hi > reach || (long)tr->now_lo - (long)r->hi > reach) continue;
+            const long ov = overlap(r, tr);
+            if (r->owner < 0 || ov > best) {
+                r->owner = t;
+                best = ov;
+            }
+        }
+    }
+    for (size_t q = 0; q < nrun; ++q) {
+        cfar_run *r = &c->runs[q];
+        if (r->owner >= 0) continue;
+        if (c->ntrk == c->cfg.max_tracks) {
+            c->st.dropped++;
+            continue;
+        }
+        cfar_trk *tr = &c->trk[c->ntrk];
+        memset(tr, 0, sizeof *tr);
+        tr->id = c->next_id;
+        c->next_id = c->next_id == UINT32_MAX ? 1 : c->next_id + 1;
+        tr->lo = tr->now_lo = r->lo;
+        tr->hi = tr->now_hi = r->hi;
+        tr->pk_x = -1.0;
+        tr->f_hit = tr->f_raw = -1;
+        tr->dt_hit = dt;
+        tr->unseen = 1;
+        r->owner = c->ntrk++;
+    }
+}
+
+/* A track's runs of this frame, in run order, into its observation and its
+ * life-long sums. */
+static void observe(atkdsp_cfar *c, size_t nrun) {
+    memset(c->obs, 0, (size_t)c->ntrk * sizeof *c->obs);
+    for (size_t q = 0; q < nrun; ++q) {
+        const cfar_run *r = &c->runs[q];
+        if (r->owner < 0) continue;
+        cfar_obs *o = &c->obs[r->owner];
+        cfar_trk *tr = &c->trk[r->owner];
+        if (!o->any) {
+            o->any = 1;
+            o->lo = r->lo;
+            o->hi = r->hi;
+        } else {
+            o->lo = r->lo < o->lo ? r->lo : o->lo;
+            o->hi = r->hi > o->hi ? r->hi : o->hi;
+        }
+        o->xsum += r->xsum;
+        o->nsum += r->nsum;
+        tr->wsum += r->wsum;
+        tr->esum += r->esum;
+        if (r->pk_x > tr->pk_x) {
+            tr->pk_x = r->pk_x;
+            tr->pk_n = r->pk_n;
+        }
+    }
+}
+
+/* The start time of a track seen for the first time in frame F: that frame's
+ * start with one frame averaged; otherwise the oldest raw frame of the
+ * averaging window whose band passes, or half the window before (T0_COARSE). */
+static double start_time(atkdsp_cfar *c, cfar_trk *tr, const double *noise, int64_t F, double tf, double dt) {
+    const int k = c->cfg.k_frames;
+    if (k == 1) return tf;
+    for (int age = k - 1; age >= 0; --age) {
+        const size_t slot = (size_t)(((F - age) % k + k) % k);
+        if (band_pass(c, c->ring + slot * c->nbins, noise, tr->now_lo, tr->now_hi)) return c->ring_t[slot];
+    }
+    tr->flags |= ATKDSP_CFAR_T0_COARSE;
+    return tf - 0.5 * (double)(k - 1) * dt;
+}
+
+/* One frame of one track; returns 1 to keep it, 0 when it ends. An event, if
+ * any, goes to *e and *emitted is set. */
+static int step_track(atkdsp_cfar *c, cfar_trk *tr, const cfar_obs *o, const double *noise, const double *pw,
+                      int64_t F, double tf, double dt, atkdsp_cfar_event *e, int *emitted) {
+    const uint64_t keep_bits = c->cfg.open_n >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << c->cfg.open_n) - 1);
+    int grew = 0;
+    *emitted = 0;
+    tr->hist = (tr->hist << 1) & keep_bits;
+    if (o->any) {
+        tr->hist |= 1u;
+        tr->miss_run = 0;
+        tr->hits_total++;
+        tr->now_lo = o->lo;
+        tr->now_hi = o->hi;
+        grew = !tr->unseen && (o->lo < tr->lo || o->hi > tr->hi);
+        tr->lo = o->lo < tr->lo ? o->lo : tr->lo;
+        tr->hi = o->hi > tr->hi ? o->hi : tr->hi;
+        if (o->nsum > 0.0 && o->xsum / o->nsum > tr->best_band) tr->best_band = o->xsum / o->nsum;
+        tr->f_hit = F;
+        tr->end_hit = tf + dt;
+        tr->dt_hit = dt;
+        if (tr->unseen) {
+            tr->unseen = 0;
+            tr->t_start = start_time(c, tr, noise, F, tf, dt);
+        }
+    } else {
+        tr->miss_run++;
+    }
+    if (pw && band_pass(c, pw, noise, tr->now_lo, tr->now_hi)) {
+        tr->f_raw = F;
+        tr->end_raw = tf + dt;
+    }
+    if (tr->state == TRK_PENDING) {
+        if (__builtin_popcountll(tr->hist) >= c->cfg.open_m) {
+            tr->state = TRK_OPEN;
+            c->st.opened++;
+            make_event(c, tr, ATKDSP_CFAR_OPEN, NAN, 0, e);
+            *emitted = 1;
+            return 1;
+        }
+        return tr->hist != 0;                     /* a pending track with no hit left is forgotten */
+    }
+    if (tr->miss_run >= c->cfg.close_k) {
+        close_event(c, tr, 0, e);
+        *emitted = 1;
+        return 0;
+    }
+    if (grew) {
+        c->st.updates++;
+        make_event(c, tr, ATKDSP_CFAR_UPDATE, NAN, 0, e);
+        *emitted = 1;
+    }
+    return 1;
 }
 
 /* one tested frame: X, noise, hits and pw (this frame's power; NULL with one
@@ -378,166 +608,22 @@ static void emit_close(atkdsp_cfar *c, cfar_track *t, uint32_t extra, atkdsp_cfa
  * events written */
 static size_t track_frame(atkdsp_cfar *c, const double *X, const double *noise, const uint8_t *hits,
                           const double *pw, double tf, double dt, atkdsp_cfar_event *ev) {
-    const size_t nb = c->nbins;
-    const int k = c->cfg.k_frames;
     const int64_t F = c->frame - 1;                          /* this frame's number */
-    /* blobs */
-    const long gap = c->cfg.merge_gap;
-    size_t nblob = 0;
-    for (long i = 0; i < (long)nb;) {
-        if (!hits[i]) {
-            ++i;
-            while (i + 8 <= (long)nb) {                         /* whole zero words at a time */
-                uint64_t w;
-                memcpy(&w, hits + i, 8);
-                if (w) break;
-                i += 8;
-            }
-            continue;
-        }
-        long hi = i, prev = i;
-        for (long j = i + 1; j < (long)nb && j - prev - 1 <= gap; ++j)
-            if (hits[j]) { hi = j; prev = j; }
-        const long lo = i;
-        i = hi + 1;
-        if (hi - lo + 1 < c->cfg.min_bins) continue;
-        cfar_blob *o = &c->blobs[nblob++];
-        memset(o, 0, sizeof *o);
-        o->lo = (int32_t)lo;
-        o->hi = (int32_t)hi;
-        long pk = -1;
-        for (long b = lo; b <= hi; ++b) {
-            if (!hits[b]) continue;
-            const double ex = X[b] - noise[b] > 0.0 ? X[b] - noise[b] : 0.0;
-            o->sum_be += (double)b * ex;
-            o->sum_e += ex;
-            o->sum_x += X[b];
-            o->sum_n += noise[b];
-            if (pk < 0 || X[b] > X[pk]) pk = b;
-        }
-        o->peak_x = X[pk];
-        o->peak_n = noise[pk];
-        o->track = -1;
-    }
-    /* each blob to the existing track it overlaps most (ties: the older) */
-    const int existing = c->ntracks;
-    const long reach = gap + 1;
-    for (size_t q = 0; q < nblob; ++q) {
-        cfar_blob *o = &c->blobs[q];
-        long best_ov = 0;
-        for (int t = 0; t < existing; ++t) {
-            const cfar_track *tr = &c->tracks[t];
-            if ((long)o->lo > (long)tr->cur_hi + reach || (long)o->hi < (long)tr->cur_lo - reach) continue;
-            const long ov = (long)(o->hi < tr->cur_hi ? o->hi : tr->cur_hi) -
-                            (long)(o->lo > tr->cur_lo ? o->lo : tr->cur_lo) + 1;
-            if (o->track < 0 || ov > best_ov) { o->track = t; best_ov = ov; }
-        }
-    }
-    /* blobs near no track start pending ones */
-    for (size_t q = 0; q < nblob; ++q) {
-        cfar_blob *o = &c->blobs[q];
-        if (o->track >= 0) continue;
-        if (c->ntracks >= c->cfg.max_tracks) { c->st.dropped++; continue; }
-        cfar_track *tr = &c->tracks[c->ntracks];
-        memset(tr, 0, sizeof *tr);
-        tr->id = c->next_id;
-        c->next_id = c->next_id + 1 ? c->next_id + 1 : 1;
-        tr->cur_lo = tr->lo = o->lo;
-        tr->cur_hi = tr->hi = o->hi;
-        tr->peak_x = -1.0;
-        tr->last_hit = -1;
-        tr->last_raw = -1;
-        tr->dt_hit = dt;
-        tr->fresh = 1;
-        o->track = c->ntracks++;
-    }
-    /* each track's observation: its blobs in blob order */
-    for (size_t q = 0; q < nblob; ++q) {
-        const cfar_blob *o = &c->blobs[q];
-        if (o->track < 0) continue;
-        cfar_track *tr = &c->tracks[o->track];
-        if (!tr->seen) {
-            tr->seen = 1;
-            tr->obs_lo = o->lo;
-            tr->obs_hi = o->hi;
-            tr->obs_x = 0.0;
-            tr->obs_n = 0.0;
-        } else {
-            if (o->lo < tr->obs_lo) tr->obs_lo = o->lo;
-            if (o->hi > tr->obs_hi) tr->obs_hi = o->hi;
-        }
-        tr->sum_be += o->sum_be;
-        tr->sum_e += o->sum_e;
-        tr->obs_x += o->sum_x;
-        tr->obs_n += o->sum_n;
-        if (o->peak_x > tr->peak_x) { tr->peak_x = o->peak_x; tr->peak_n = o->peak_n; }
-    }
-    /* every track, in id order: its window, its state, its event */
-    const uint64_t wmask = c->cfg.open_n >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << c->cfg.open_n) - 1);
+    const size_t nrun = frame_runs(c, hits, X, noise);
+    assign_runs(c, nrun, dt);
+    observe(c, nrun);
     size_t nev = 0;
-    int kept = 0;
-    for (int t = 0; t < c->ntracks; ++t) {
-        cfar_track *tr = &c->tracks[t];
-        int grew = 0;
-        if (tr->seen) {
-            const int32_t lo = tr->obs_lo, hi = tr->obs_hi;
-            tr->window = ((tr->window << 1) | 1u) & wmask;
-            tr->misses = 0;
-            tr->n_hits++;
-            tr->cur_lo = lo;
-            tr->cur_hi = hi;
-            if (!tr->fresh) grew = lo < tr->lo || hi > tr->hi;
-            if (lo < tr->lo) tr->lo = lo;
-            if (hi > tr->hi) tr->hi = hi;
-            if (tr->obs_n > 0.0 && tr->obs_x / tr->obs_n > tr->band) tr->band = tr->obs_x / tr->obs_n;
-            tr->last_hit = F;
-            tr->t_hit_end = tf + dt;
-            tr->dt_hit = dt;
-            if (tr->fresh) {
-                tr->fresh = 0;
-                if (k == 1) {
-                    tr->t0 = tf;
-                } else {
-                    /* the earliest raw frame of the averaging window whose band passes */
-                    int found = 0;
-                    for (int r = 0; r < k && !found; ++r) {
-                        const size_t s = (size_t)((F + 1 + r) % k);
-                        if (band_pass(c, c->ring + s * nb, noise, lo, hi)) { tr->t0 = c->ring_t[s]; found = 1; }
-                    }
-                    if (!found) {
-                        tr->t0 = tf - 0.5 * (double)(k - 1) * dt;
-                        tr->flags |= ATKDSP_CFAR_T0_COARSE;
-                    }
-                }
-            }
-        } else {
-            tr->window = (tr->window << 1) & wmask;
-            tr->misses++;
+    int live = 0;
+    for (int t = 0; t < c->ntrk; ++t) {
+        int emitted;
+        const int keep = step_track(c, &c->trk[t], &c->obs[t], noise, pw, F, tf, dt, &ev[nev], &emitted);
+        nev += (size_t)emitted;
+        if (keep) {
+            if (live != t) c->trk[live] = c->trk[t];
+            ++live;
         }
-        tr->seen = 0;
-        if (k > 1 && band_pass(c, pw, noise, tr->cur_lo, tr->cur_hi)) {
-            tr->last_raw = F;
-            tr->t_raw_end = tf + dt;
-        }
-        if (!tr->open) {
-            if (__builtin_popcountll(tr->window) >= c->cfg.open_m) {
-                tr->open = 1;
-                c->st.opened++;
-                emit(c, tr, ATKDSP_CFAR_OPEN, NAN, 0, &ev[nev++]);
-            } else if (tr->window == 0) {
-                continue;                                     /* forgotten */
-            }
-        } else if (tr->misses >= c->cfg.close_k) {
-            emit_close(c, tr, 0, &ev[nev++]);
-            continue;
-        } else if (grew) {
-            c->st.updates++;
-            emit(c, tr, ATKDSP_CFAR_UPDATE, NAN, 0, &ev[nev++]);
-        }
-        if (kept != t) c->tracks[kept] = *tr;
-        ++kept;
     }
-    c->ntracks = kept;
+    c->ntrk = live;
     return nev;
 }
 
@@ -665,13 +751,13 @@ ptrdiff_t atk_cfar_process_staged(atkdsp_cfar *c, size_t n, double frame_t0, dou
 
 ptrdiff_t atkdsp_cfar_flush(atkdsp_cfar *c, atkdsp_cfar_event *ev, size_t cap) {
     if (!c) return ATKDSP_E_ARG;
-    size_t open = 0;
-    for (int t = 0; t < c->ntracks; ++t) open += c->tracks[t].open == 1;
-    if (open && (!ev || cap < open)) return ATKDSP_E_CAP;
+    size_t nopen = 0;
+    for (int t = 0; t < c->ntrk; ++t) nopen += c->trk[t].state == TRK_OPEN;
+    if (nopen && (!ev || cap < nopen)) return ATKDSP_E_CAP;
     size_t nev = 0;
-    for (int t = 0; t < c->ntracks; ++t)
-        if (c->tracks[t].open) emit_close(c, &c->tracks[t], ATKDSP_CFAR_FLUSHED, &ev[nev++]);
-    c->ntracks = 0;
+    for (int t = 0; t < c->ntrk; ++t)
+        if (c->trk[t].state == TRK_OPEN) close_event(c, &c->trk[t], ATKDSP_CFAR_FLUSHED, &ev[nev++]);
+    c->ntrk = 0;
     return (ptrdiff_t)nev;
 }
 
@@ -688,34 +774,37 @@ int atkdsp_cfar_last(const atkdsp_cfar *c, float *avg_db, float *thr_db, uint8_t
 
 int atkdsp_cfar_info(const atkdsp_cfar *c, double *alpha, double *alpha_ca, double *et, atkdsp_cfar_config *cfg) {
     if (!c) return ATKDSP_E_ARG;
-    if (alpha) *alpha = c->alpha;
-    if (alpha_ca) *alpha_ca = c->alpha_ca;
-    if (et) *et = c->et;
-    if (cfg) *cfg = c->cfg;
+    const double v[3] = {c->alpha, c->alpha_ca, c->et};
+    double *dst[3] = {alpha, alpha_ca, et};
+    for (int i = 0; i < 3; ++i)
+        if (dst[i]) *dst[i] = v[i];
+    if (cfg) *cfg = c->cfg;                     /* with the OS rank resolved */
     return ATKDSP_OK;
 }
 
+/* A one-line summary: the statistic, the cells, the false-alarm design, the
+ * tracker's rules and the handle's state. */
 int atkdsp_cfar_describe(const atkdsp_cfar *c, char *buf, size_t cap) {
     if (!c || !buf || !cap) return ATKDSP_E_ARG;
-    static const char *names[4] = {"CA", "OS", "GO", "SO"};
-    char rank[48] = "", eff[64] = "";
-    if (c->cfg.mode == ATKDSP_CFAR_OS) snprintf(rank, sizeof rank, " (rank %d of %d)", c->cfg.os_k, c->N);
-    if (c->n_eff != (double)c->N) snprintf(eff, sizeof eff, " (%.1f effective, window-correlated)", c->n_eff);
+    static const char *const stat_name[4] = {"cell-averaging", "ordered-statistic", "greatest-of", "smallest-of"};
+    const atkdsp_cfar_config *g = &c->cfg;
     size_t masked = 0;
     for (size_t i = 0; i < c->nbins; ++i) masked += c->mask[i] != 0;
+    char rank[40] = "";
+    if (g->mode == ATKDSP_CFAR_OS) snprintf(rank, sizeof rank, ", rank %d", g->os_k);
     snprintf(buf, cap,
-             "CFAR %s%s: %d+%d reference cells%s, %d guard each side, Pfa %.3g per cell per frame, K = %g look(s) "
-             "x %d frame(s), alpha %.4g (%.2f dB); open %d of %d, close after %d misses, merge gap %d, min %d "
-             "bin(s); %zu bins, %zu masked, tracks %d/%d",
-             names[c->cfg.mode], rank, c->cfg.n_ref, c->cfg.n_ref, eff, c->cfg.n_guard, c->cfg.pfa, c->cfg.looks,
-             c->cfg.k_frames, c->alpha, 10.0 * log10(c->alpha), c->cfg.open_m, c->cfg.open_n, c->cfg.close_k,
-             c->cfg.merge_gap, c->cfg.min_bins, c->nbins, masked, c->ntracks, c->cfg.max_tracks);
+             "cfar[%s%s] bins=%zu masked=%zu ref=2x%d guard=%d n_eff=%.2f pfa=%.3g looks=%g frames=%d "
+             "alpha=%.6g (%+.2f dB) open=%d/%d close=%d gap=%d min=%d tracks=%d/%d",
+             stat_name[g->mode], rank, c->nbins, masked, g->n_ref, g->n_guard, c->n_eff, g->pfa, g->looks,
+             g->k_frames, c->alpha, 10.0 * log10(c->alpha), g->open_m, g->open_n, g->close_k, g->merge_gap,
+             g->min_bins, c->ntrk, g->max_tracks);
     return ATKDSP_OK;
 }
 
 int atkdsp_cfar_stats(const atkdsp_cfar *c, atkdsp_cfar_counters *st) {
     if (!c || !st) return ATKDSP_E_ARG;
-    *st = c->st;
-    st->tracks = (uint64_t)c->ntracks;
+    atkdsp_cfar_counters now = c->st;
+    now.tracks = (uint64_t)c->ntrk;             /* live tracks, pending or open */
+    *st = now;
     return ATKDSP_OK;
 }
