@@ -144,6 +144,15 @@ int atkgpu_chain_unpack(void *s, const void *raw, size_t raw_pitch, int fmt, siz
                         int update, unsigned S);
 /* every frame f < frames at row0 + s*spitch + f*fhop (N in 1024/2048/4096):
  * window, FFT, |X|^2/N^2, detector -> partial[S][nfb][N] -> line_db[S][N] */
+/* the chain's display reduction straight from the raw blocks (frames >= 1),
+ * unpacking each block into its row on the way: row0 is stream 0's frame 0,
+ * the block starts dlen samples after it; then the dB lines and the mean of
+ * every block (FP64 sums in mparts, [S][64][2]) into mean and, with update,
+ * the next block's offset */
+int atkgpu_chain_display_unpack(void *st, void *row0, size_t spitch, const void *raw, size_t raw_pitch, int fmt,
+                                size_t n, size_t dlen, float *off, unsigned S, size_t frames, size_t fhop, unsigned N,
+                                int nfb, const float *win, const void *tw, int det, float *partial, float *line_db,
+                                double *mparts, double *mean, int update);
 int atkgpu_chain_display(void *s, const void *row0, size_t spitch, unsigned S, size_t frames,
                          size_t fhop, unsigned N, int nfb, const float *win, const void *tw, int det,
                          float *partial, float *line_db);

@@ -8,12 +8,20 @@
 // tail (< nfft samples) just before the block. After a block the last keep
 // samples move to the other slab buffer.
 //
-//   k_chain_unpack   raw -> cf32 into the row (bit-exact unpack_dc, reference
-//                    src/convert.c:33-111), DC offset from the stream's
-//                    previous-block mean (device state), FP64 partial sums
+//   k_disp_unpack    the display reduction straight from the raw block: each
+//                    frame's samples are converted as they are loaded
+//                    (bit-exact unpack_dc, reference src/convert.c:33-111,
+//                    with the DC offset from the stream's previous-block mean,
+//                    device state); each converted sample is also written
+//                    into the row once, by the frame whose first hop holds it
+//                    (the last frame takes the rest of the block), and summed
+//                    in FP64 for the block mean; then window, team FFT,
+//                    |X|^2/n^2, per-bin detector in registers (reference
+//                    src/fft.c:177-191), partial lines
+//   k_chain_unpack   raw -> cf32 into the row with the FP64 partial sums, for
+//                    a block that completes no display frame
 //   k_chain_mean     per stream: FP64 mean -> next block's offset
-//   k_disp_reduce    frames: window, team FFT, |X|^2/n^2, per-bin detector in
-//                    registers (reference src/fft.c:177-191), partial lines
+//   k_disp_reduce    the display reduction over a cf32 row
 //   k_disp_final     partial lines in fixed order -> dB line (src/fft.c:286-292)
 //   k_disp_post      decimate_max (src/display.c:19-37) and the NaN-aware
 //                    stitch of the decimated line (src/detect.c:116-134)
@@ -193,8 +201,248 @@ __global__ void __launch_bounds__(256) k_disp_reduce(DispArgs a) {
     // frame offsets within a stream row fit 32 bits
     const int f0 = fb * a.fpb;
     const int f1 = min(f0 + a.fpb, (int)a.frames);
-    disp_frames<N, DET, 256>(a.row0 + (long long)s * a.spitch, (int)a.fhop, f0, f1, a.win, twl, lds, red,
-                        a.partial + ((long long)s * a.nfb + fb) * N);
+    DispRowSrc src{a.row0 + (long long)s * a.spitch, (int)a.fhop};
+    disp_frames<N, DET, 256>(src, f0, f1, a.win, twl, lds, red, a.partial + ((long long)s * a.nfb + fb) * N);
+}
+
+__global__ void k_disp_final(const float *__restrict__ partial, int nfb, int N, int det, long long frames,
+                             float *__restrict__ line_db);
+
+// ---- the display reduction from the raw block, unpacking as it goes ----------------------------
+// Row positions are counted from frame 0's first sample: [0, dlen) is the
+// kept cf32 tail of earlier blocks, [dlen, dlen + n) this block, raw sample i
+// at position dlen + i. Frame f owns the block samples of its first hop
+// [f fhop, (f+1) fhop); the last frame owns the rest of the block.
+// one raw sample at index i (32-bit) of a block, converted exactly as
+// k_chain_unpack does
+template <int FMT>
+__device__ __forceinline__ cpx raw_conv(const unsigned char *raw, unsigned i, float cre, float cim, float k,
+                                        float ore, float oim) {
+#pragma clang fp contract(off)
+    cpx o;
+    if constexpr (FMT == FMT_CU8) {
+        const uchar2 b = ((const uchar2 *)raw)[i];
+        o.x = ((float)b.x - cre) * k;
+        o.y = ((float)b.y - cim) * k;
+    } else if constexpr (FMT == FMT_CI8) {
+        const char2 b = ((const char2 *)raw)[i];
+        o.x = ((float)(signed char)b.x - cre) * k;
+        o.y = ((float)(signed char)b.y - cim) * k;
+    } else if constexpr (FMT == FMT_CI16) {
+        const short2 b = ((const short2 *)raw)[i];
+        o.x = ((float)b.x - cre) * k;
+        o.y = ((float)b.y - cim) * k;
+    } else {
+        const float2 b = ((const float2 *)raw)[i];
+        o.x = b.x - ore;
+        o.y = b.y - oim;
+    }
+    return o;
+}
+
+// HALF: fhop = N/2, so a frame inside the block (and not the last) owns
+// exactly its first PT/2 sample slots
+template <int FMT, bool HALF>
+struct DispRawSrc {
+    cpx *row;
+    const unsigned char *raw;
+    int fhop, dlen, last;
+    float cre, cim, kk, ore, oim;
+    double sr, si;
+    __device__ __forceinline__ cpx conv(int pos) const {
+        return raw_conv<FMT>(raw, (unsigned)(pos - dlen), cre, cim, kk, ore, oim);
+    }
+    template <int PT, int NT>
+    __device__ __forceinline__ void load(int f, int j, cpx *v) const {
+        const int p0 = f * fhop + j;
+        if (f * fhop >= dlen) {
+            // the whole frame is in this block: raw loads only
+            const unsigned i0 = (unsigned)(p0 - dlen);
+#pragma unroll
+            for (int r = 0; r < PT; ++r) v[r] = raw_conv<FMT>(raw, i0 + r * NT, cre, cim, kk, ore, oim);
+        } else {
+#pragma unroll
+            for (int r = 0; r < PT; ++r) {
+                const int pos = p0 + r * NT;
+                v[r] = pos < dlen ? row[pos] : conv(pos);
+            }
+        }
+    }
+    __device__ __forceinline__ void take(cpx *dst, cpx x) {
+        *dst = x;
+        sr += (double)x.x;
+        si += (double)x.y;
+    }
+    template <int PT, int NT>
+    __device__ __forceinline__ void own(int f, int j, const cpx *v) {
+        const int p0 = f * fhop + j;
+        if (HALF && f != last && f * fhop >= dlen) {
+            cpx *dst = row + (unsigned)p0;
+#pragma unroll
+            for (int r = 0; r < PT / 2; ++r) take(dst + r * NT, v[r]);
+            return;
+        }
+#pragma unroll
+        for (int r = 0; r < PT; ++r) {
+            const int p = j + r * NT, pos = p0 + r * NT;
+            if (pos >= dlen && (p < fhop || f == last)) take(row + pos, v[r]);
+        }
+    }
+};
+
+struct DispRawArgs {
+    cpx *row0;                  // stream s frame 0 at row0 + s*spitch
+    long long spitch;
+    const unsigned char *raw;   // stream s block at raw + s*raw_pitch
+    long long raw_pitch;
+    int dlen, n;                // kept tail before the block; block samples
+    const float *off;           // per stream offset (re, im); null = 0
+    float c0, kk;
+    long long frames, fhop;
+    int fpb, nfb;
+    const float *win;
+    const cpx *tw;
+    float *partial;             // [S][nfb][N] display order (fftshifted)
+    double *mparts;             // [S][nfb][2] FP64 sums of the samples each block wrote
+};
+
+template <int N, int DET, int FMT, bool HALF>
+__global__ void __launch_bounds__(256) k_disp_unpack(DispRawArgs a) {
+#pragma clang fp contract(off)
+    using D = DispShape<N>;
+    __shared__ cpx lds[D::TEAMS * D::TL];
+    __shared__ float red[D::TEAMS > 1 ? D::TEAMS * N : 1];
+    __shared__ cpx twl[N];
+    const int s = blockIdx.y, fb = blockIdx.x;
+    for (int i = threadIdx.x; i < N; i += 256) twl[i] = a.tw[i];
+    __syncthreads();
+    DispRawSrc<FMT, HALF> src;
+    src.row = a.row0 + (long long)s * a.spitch;
+    src.raw = a.raw + (long long)s * a.raw_pitch;
+    src.fhop = (int)a.fhop;
+    src.dlen = a.dlen;
+    src.last = (int)a.frames - 1;
+    src.kk = a.kk;
+    src.ore = a.off ? a.off[2 * s] : 0.0f;
+    src.oim = a.off ? a.off[2 * s + 1] : 0.0f;
+    // the folded constants exactly as src/convert.c:47, :57, :68 (IEEE division)
+    src.cre = a.c0 + src.ore / a.kk;
+    src.cim = a.c0 + src.oim / a.kk;
+    src.sr = 0.0;
+    src.si = 0.0;
+    const int f0 = fb * a.fpb;
+    const int f1 = min(f0 + a.fpb, (int)a.frames);
+    disp_frames<N, DET, 256>(src, f0, f1, a.win, twl, lds, red, a.partial + ((long long)s * a.nfb + fb) * N);
+    if (f1 == (int)a.frames) {
+        // the block's samples past the last frame's window
+        for (int pos = src.last * src.fhop + N + (int)threadIdx.x; pos < a.dlen + a.n; pos += 256) {
+            const cpx x = src.conv(pos);
+            src.row[pos] = x;
+            src.sr += (double)x.x;
+            src.si += (double)x.y;
+        }
+    }
+    // this block's FP64 sums, in a fixed order
+    __syncthreads();
+    double *sh = (double *)lds;
+    sh[threadIdx.x] = src.sr;
+    sh[256 + threadIdx.x] = src.si;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[threadIdx.x] += sh[threadIdx.x + w];
+            sh[256 + threadIdx.x] += sh[256 + threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double *mp = a.mparts + ((long long)s * a.nfb + fb) * 2;
+        mp[0] = sh[0];
+        mp[1] = sh[256];
+    }
+}
+
+template <int N, int FMT, bool HALF>
+static int launch_disp_unpack_h(const DispRawArgs &a, int det, unsigned S, hipStream_t st) {
+    const dim3 g((unsigned)a.nfb, S);
+    switch (det) {
+    case 0: hipLaunchKernelGGL((k_disp_unpack<N, 0, FMT, HALF>), g, dim3(256), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_disp_unpack<N, 1, FMT, HALF>), g, dim3(256), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_disp_unpack<N, 2, FMT, HALF>), g, dim3(256), 0, st, a); break;
+    }
+    return LAUNCH_OK();
+}
+
+template <int N, int FMT>
+static int launch_disp_unpack(const DispRawArgs &a, int det, unsigned S, hipStream_t st) {
+    return a.fhop * 2 == N ? launch_disp_unpack_h<N, FMT, true>(a, det, S, st)
+                           : launch_disp_unpack_h<N, FMT, false>(a, det, S, st);
+}
+
+template <int N>
+static int launch_disp_unpack_fmt(const DispRawArgs &a, int fmt, int det, unsigned S, hipStream_t st) {
+    switch (fmt) {
+    case FMT_CU8: return launch_disp_unpack<N, FMT_CU8>(a, det, S, st);
+    case FMT_CI8: return launch_disp_unpack<N, FMT_CI8>(a, det, S, st);
+    case FMT_CI16:
+    case FMT_CI16Q11: return launch_disp_unpack<N, FMT_CI16>(a, det, S, st);
+    case FMT_CF32: return launch_disp_unpack<N, FMT_CF32>(a, det, S, st);
+    }
+    return -1;
+}
+
+// The display reduction of `frames` frames straight from the raw blocks,
+// unpacking each block into its row on the way (frames >= 1), then the dB
+// line, and the per-stream mean -> next offset (update != 0) from the FP64
+// sums the display blocks formed. *nfb_used receives the frame blocks per
+// stream (the partial line and mean rows used).
+extern "C" int atkgpu_chain_display_unpack(void *st, void *row0, size_t spitch, const void *raw, size_t raw_pitch,
+                                           int fmt, size_t n, size_t dlen, float *off, unsigned S, size_t frames,
+                                           size_t fhop, unsigned N, int nfb, const float *win, const void *tw,
+                                           int det, float *partial, float *line_db, double *mparts, double *mean,
+                                           int update) {
+    if (!frames || !S || !n) return -1;
+    DispRawArgs a;
+    a.row0 = (cpx *)row0;
+    a.spitch = (long long)spitch;
+    a.raw = (const unsigned char *)raw;
+    a.raw_pitch = (long long)raw_pitch;
+    a.dlen = (int)dlen;
+    a.n = (int)n;
+    a.off = off;
+    a.c0 = 0.0f;
+    a.kk = 1.0f;
+    switch (fmt) {
+    case FMT_CU8: a.c0 = 127.5f; a.kk = 1.0f / 127.5f; break;
+    case FMT_CI8: a.kk = 1.0f / 128.0f; break;
+    case FMT_CI16: a.kk = 1.0f / 32768.0f; break;
+    case FMT_CI16Q11: a.kk = 1.0f / 2048.0f; break;
+    case FMT_CF32: break;
+    default: return -1;
+    }
+    a.frames = (long long)frames;
+    a.fhop = (long long)fhop;
+    a.fpb = (int)((frames + nfb - 1) / nfb);
+    a.nfb = (int)((frames + a.fpb - 1) / a.fpb);
+    a.win = win;
+    a.tw = (const cpx *)tw;
+    a.partial = partial;
+    a.mparts = mparts;
+    hipStream_t s = (hipStream_t)st;
+    int rc;
+    switch (N) {
+    case 1024: rc = launch_disp_unpack_fmt<1024>(a, fmt, det, S, s); break;
+    case 2048: rc = launch_disp_unpack_fmt<2048>(a, fmt, det, S, s); break;
+    case 4096: rc = launch_disp_unpack_fmt<4096>(a, fmt, det, S, s); break;
+    default: return -1;
+    }
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_disp_final, dim3((N + 255) / 256, S), dim3(256), 0, s, (const float *)partial, a.nfb,
+                       (int)N, det, (long long)frames, line_db);
+    if (LAUNCH_OK()) return -4;
+    hipLaunchKernelGGL(k_chain_mean, dim3((S + 63) / 64), dim3(64), 0, s, (const double *)mparts, a.nfb,
+                       (long long)n, off, mean, update, (int)S);
+    return LAUNCH_OK();
 }
 
 // per (stream, bin): combine the frame blocks in order, then dB (reduce's

@@ -50,15 +50,33 @@ __device__ __forceinline__ float disp_db(float v, float scale) {
     return 20.0f * log10f(sqrtf(v * scale) + 1e-12f);
 }
 
-// Frames [f0, f1) of one stream, frame f starting at row + f*fhop. Every thread
-// of the NTHR-thread block calls it (the team FFT synchronises the block).
+// A frame's samples from a cf32 row: frame f starts at row + f*fhop.
+struct DispRowSrc {
+    const cpx *row;
+    int fhop;
+    // samples j + r*NT (r < PT) of frame f
+    template <int PT, int NT>
+    __device__ __forceinline__ void load(int f, int j, cpx *v) const {
+        const cpx *fr = row + f * fhop + j;
+#pragma unroll
+        for (int r = 0; r < PT; ++r) v[r] = fr[r * NT];
+    }
+    template <int PT, int NT>
+    __device__ __forceinline__ void own(int, int, const cpx *) {}
+};
+
+// Frames [f0, f1) of one stream, a thread's samples j + r*NT of frame f being
+// src.load<PT, NT>(f, j, v). Every thread of the NTHR-thread block calls it
+// (the team FFT synchronises the block); src.own<PT, NT>(f, j, v) is called
+// once for each frame the block transforms (and for no other) with its
+// samples, so a source may write or sum them.
 // twl holds the N twiddles in LDS (loaded and synchronised by the caller);
 // lds holds TEAMS * TL and red TEAMS * N (TEAMS > 1) entries. The line goes to
 // out (global or LDS) in display order; with TEAMS > 1 it is complete when
 // the call returns, with TEAMS == 1 after the caller's barrier.
-template <int N, int DET, int NTHR = 256>
-__device__ __forceinline__ void disp_frames(const cpx *row, int fhop, int f0, int f1, const float *win,
-                                            const cpx *twl, cpx *lds, float *red, float *out) {
+template <int N, int DET, int NTHR = 256, class Src>
+__device__ __forceinline__ void disp_frames(Src &src, int f0, int f1, const float *win, const cpx *twl, cpx *lds,
+                                            float *red, float *out) {
 #pragma clang fp contract(off)
     using D = DispShape<N, NTHR>;
     constexpr int PT = D::PT, NT = D::NT, TEAMS = D::TEAMS, TL = D::TL;
@@ -81,21 +99,18 @@ __device__ __forceinline__ void disp_frames(const cpx *row, int fhop, int f0, in
     cpx nx[PT];
     {
         const int f = f0 + tm;
-        const cpx *fr = row + (f < f1 ? f : f0) * fhop + j;
-#pragma unroll
-        for (int r = 0; r < PT; ++r) nx[r] = fr[r * NT];
+        src.template load<PT, NT>(f < f1 ? f : f0, j, nx);
     }
     for (int fbase = f0; fbase < f1; fbase += TEAMS) {
         const int f = fbase + tm;
         const bool act = f < f1;
+        if (act) src.template own<PT, NT>(f, j, nx);
         cpx v[PT];
 #pragma unroll
         for (int r = 0; r < PT; ++r) v[r] = act ? cmake(nx[r].x * w[r], nx[r].y * w[r]) : cmake(0.f, 0.f);
         {
             const int fn = f + TEAMS;
-            const cpx *fr = row + (fn < f1 ? fn : f0) * fhop + j;
-#pragma unroll
-            for (int r = 0; r < PT; ++r) nx[r] = fr[r * NT];
+            src.template load<PT, NT>(fn < f1 ? fn : f0, j, nx);
         }
         if constexpr (TWR) team_fft_tw<N, PT>(v, lds + tm * TL, ttw, j);
         else team_fft<N, PT>(v, lds + tm * TL, twl, j);

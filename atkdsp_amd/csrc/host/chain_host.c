@@ -166,9 +166,25 @@ ptrdiff_t atkgpu_chain_run(atkgpu_chain *c, size_t n) {
     const size_t m = avail / c->hop;
     const size_t d = c->dlen + n;
     const long long frames = d >= g->nfft ? (long long)((d - g->nfft) / g->fft_hop + 1) : 0;
+    /* about 512 display blocks in all: the 4096-point reduction holds two
+     * blocks per CU, so on 256 CUs every block is resident at once and each
+     * transforms a long run of frames with its twiddles in registers
+     * (measured 0.762 ms a step against 0.792 ms with 2048 blocks) */
+    int nfb = (int)((512 + c->S - 1) / c->S);
+    if (nfb > c->nfb_max) nfb = c->nfb_max;
+    if (nfb < 1) nfb = 1;
     int rc = atkgpu_event_record(c->ev[0], c->stream);
-    if (!rc) rc = atkgpu_chain_unpack(c->stream, c->d_raw, g->block_max * c->bps, g->fmt, n, R + keep, c->spitch,
-                                      c->d_off, c->d_upart, c->uparts, c->d_mean, g->dc_mode == 1, c->S);
+    /* the display reads the raw blocks and unpacks them into the rows on the
+     * way (with the block mean); a block that completes no frame is
+     * unpacked on its own */
+    if (!rc && frames)
+        rc = atkgpu_chain_display_unpack(c->stream, R + (keep - c->dlen), c->spitch, c->d_raw, g->block_max * c->bps,
+                                         g->fmt, n, c->dlen, c->d_off, c->S, (size_t)frames, g->fft_hop, g->nfft, nfb,
+                                         c->d_win, atkgpu_fftplan_twiddles(c->plan_n), g->detector, c->d_partial,
+                                         c->d_line, c->d_upart, c->d_mean, g->dc_mode == 1);
+    else if (!rc)
+        rc = atkgpu_chain_unpack(c->stream, c->d_raw, g->block_max * c->bps, g->fmt, n, R + keep, c->spitch,
+                                 c->d_off, c->d_upart, c->uparts, c->d_mean, g->dc_mode == 1, c->S);
     if (!rc) rc = atkgpu_event_record(c->ev[1], c->stream);
     if (!rc && m)
         rc = atkgpu_pfb_fused(c->stream, R + (keep - c->ov - c->lc), c->spitch, (size_t)c->ov + avail, c->S, m,
@@ -176,20 +192,10 @@ ptrdiff_t atkgpu_chain_run(atkgpu_chain *c, size_t n) {
                               NULL, 0, c->d_out, (m + 15) & ~(size_t)15, 0);
     if (!rc) rc = atkgpu_event_record(c->ev[2], c->stream);
     if (!rc && frames) {
-        /* about 512 display blocks in all: the 4096-point reduction holds two
-         * blocks per CU, so on 256 CUs every block is resident at once and
-         * each transforms a long run of frames with its twiddles in registers
-         * (measured 0.762 ms a step against 0.792 ms with 2048 blocks) */
-        int nfb = (int)((512 + c->S - 1) / c->S);
-        if (nfb > c->nfb_max) nfb = c->nfb_max;
-        if (nfb < 1) nfb = 1;
-        rc = atkgpu_chain_display(c->stream, R + (keep - c->dlen), c->spitch, c->S, (size_t)frames, g->fft_hop,
-                                  g->nfft, nfb, c->d_win, atkgpu_fftplan_twiddles(c->plan_n), g->detector,
-                                  c->d_partial, c->d_line);
         const double seg_step = g->fs / (double)g->line_cols;
-        if (!rc) rc = atkgpu_chain_post(c->stream, c->d_line, c->S, g->nfft, g->line_cols, c->d_dec, c->d_trace,
-                                        g->trace_bins, g->fc0_hz - g->fs / 2, g->fc_step_hz, seg_step,
-                                        g->trace_lo_hz, g->trace_hz_per_bin);
+        rc = atkgpu_chain_post(c->stream, c->d_line, c->S, g->nfft, g->line_cols, c->d_dec, c->d_trace,
+                               g->trace_bins, g->fc0_hz - g->fs / 2, g->fc_step_hz, seg_step, g->trace_lo_hz,
+                               g->trace_hz_per_bin);
     }
     if (!rc) rc = atkgpu_chain_keep(c->stream, R + n, c->spitch, T, keep, c->S);
     if (!rc) rc = atkgpu_event_record(c->ev[3], c->stream);
@@ -211,9 +217,10 @@ int atkgpu_chain_sync(atkgpu_chain *c) {
 
 int atkgpu_chain_timing(atkgpu_chain *c, float *ms_run, float *ms_pfb, float *ms_disp) {
     if (!c || !c->ran) return ATKDSP_E_ARG;
+    /* ev0 | display with the unpack, block mean | ev1 | filterbank | ev2 | decimate, stitch, keep | ev3 */
     float a = 0, b = 0, d = 0;
     if (atkgpu_event_elapsed_ms(c->ev[0], c->ev[3], &a) || atkgpu_event_elapsed_ms(c->ev[1], c->ev[2], &b) ||
-        atkgpu_event_elapsed_ms(c->ev[2], c->ev[3], &d))
+        atkgpu_event_elapsed_ms(c->ev[0], c->ev[1], &d))
         return ATKDSP_E_FFT;
     if (ms_run) *ms_run = a;
     if (ms_pfb) *ms_pfb = b;

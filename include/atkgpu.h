@@ -36,7 +36,9 @@ ATKGPU_API int atkgpu_version(void);
  *   HBM) -> display line over every nfft frame at fft_hop (the < nfft tail is
  *   carried to the next block) -> decimate_max to line_cols -> NaN-aware
  *   max stitch into the stream's trace.
- * Stream s covers [fc0 + s*fc_step - fs/2, fc0 + s*fc_step + fs/2]. */
+ * The display reduction reads each raw block directly and unpacks it into the
+ * stream's row as it goes (with the block mean); the filterbank then reads
+ * the row. Stream s covers [fc0 + s*fc_step - fs/2, fc0 + s*fc_step + fs/2]. */
 typedef struct {
     int fmt;                    /* enum atkdsp_format of the raw samples   */
     double fs;
@@ -66,7 +68,7 @@ ATKGPU_API int           atkgpu_chain_upload(atkgpu_chain *c, unsigned s, const 
 ATKGPU_API ptrdiff_t     atkgpu_chain_run(atkgpu_chain *c, size_t n);
 ATKGPU_API int           atkgpu_chain_sync(atkgpu_chain *c);
 /* device times of the last run, ms: whole run, the filterbank kernel alone,
- * the display kernels alone */
+ * the display reduction with the unpack and block mean */
 ATKGPU_API int           atkgpu_chain_timing(atkgpu_chain *c, float *ms_run, float *ms_pfb, float *ms_disp);
 /* results of the last run, copied to host (synchronising) */
 ATKGPU_API ptrdiff_t     atkgpu_chain_read_pfb(atkgpu_chain *c, unsigned s, atkdsp_cf32 *out, size_t cap);
