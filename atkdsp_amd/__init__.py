@@ -207,7 +207,8 @@ def _declare(lib) -> None:
 
 
 _STATUS = {-1: "bad argument", -2: "out of memory", -3: "output buffer too small",
-           -4: "FFT backend or device failure", -7: "nothing to return yet"}
+           -4: "FFT backend or device failure", -6: "overwritten while it was read (overrun)",
+           -7: "nothing to return yet"}
 
 
 def _check(rc: int, what: str) -> int:

@@ -51,6 +51,7 @@
 #include <sys/prctl.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
 #include <time.h>
 #include "internal.h"
 #include "../../../include/atkgpu.h"
@@ -62,7 +63,11 @@ int atkgpu_pfb_prototype(double *h, unsigned nchan, unsigned K, double atten_db)
 #endif
 #define POLL_NS 10000          /* completion poll interval */
 #define QCAP 256               /* pending chunks per stream */
-#define LAT_CAP (1u << 22)
+/* chunk latencies: a histogram of log-spaced buckets, each 1/256 wider
+ * than the last (0.4 % apart), from 0.1 us to ~8 s; every chunk is counted */
+#define LAT_NB 4608
+#define LAT_LO_MS 1e-4
+#define LAT_LOG_STEP 0.00389864041565732   /* log(1 + 1/256) */
 #define NEL_MAX 16             /* elements of an array stream */
 #define NPLANS 7               /* FFT twiddles for 64..4096 points */
 /* blocks of the ingest launch (each strides over the batch's ingest blocks):
@@ -158,7 +163,8 @@ typedef struct {
 } bitem;
 
 typedef struct {
-    void *event;               /* the batch's compute is done */
+    void *event;               /* the batch's compute is done (polled by the completion thread) */
+    void *oevent;              /* the same point, for the dispatcher's stream waits only */
     void *uevent;              /* its ingest is done */
     void *devent;              /* its display lines are done */
     int n;
@@ -195,8 +201,9 @@ struct atkgpu_server {
     pthread_cond_t cv_work, cv_launch, cv_done, cv_space;   /* dispatcher, completion, waiters, submitters */
     pthread_t th_disp, th_comp;
     int running, failed;
-    double *lat, *lat_sort;
+    uint64_t *lat_hist;                          /* LAT_NB buckets */
     size_t nlat;
+    double lat_sum, lat_min, lat_max;
     long long chunks, samples, nbatches;
     double t_first, t_last;
 };
@@ -261,9 +268,8 @@ atkgpu_server *atkgpu_server_create(size_t hbm_budget, unsigned max_streams, uns
     v->stage_h = (unsigned char *)atkgpu_hmalloc_mapped(staging, (void **)&v->stage_d);
     v->rec_cap = (size_t)max_streams * QCAP + 16;
     v->recs = (stage_rec *)calloc(v->rec_cap, sizeof(stage_rec));
-    v->lat = (double *)malloc(LAT_CAP * sizeof(double));
-    v->lat_sort = (double *)malloc(LAT_CAP * sizeof(double));
-    int ok = v->st && v->stage_h && v->recs && v->lat && v->lat_sort && !atkgpu_stream_create(&v->stream) &&
+    v->lat_hist = (uint64_t *)calloc(LAT_NB, sizeof(uint64_t));
+    int ok = v->st && v->stage_h && v->recs && v->lat_hist && !atkgpu_stream_create(&v->stream) &&
              !atkgpu_stream_create(&v->ustream) && !atkgpu_stream_create(&v->dstream) &&
              !atkgpu_stream_create(&v->qstream) &&
              (v->q_scratch = (float *)atkgpu_dmalloc(DISP_COLS_MAX * sizeof(float))) &&
@@ -272,7 +278,8 @@ atkgpu_server *atkgpu_server_create(size_t hbm_budget, unsigned max_streams, uns
     for (int i = 0; ok && i < NPLANS; ++i) ok = (v->plans[i] = atkgpu_fftplan_create(64u << i)) != NULL;
     for (int i = 0; ok && i < NINFLIGHT; ++i) {
         batch_t *b = &v->batch[i];
-        ok = !atkgpu_event_create_order(&b->event) && !atkgpu_event_create_order(&b->uevent) &&
+        ok = !atkgpu_event_create_order(&b->event) && !atkgpu_event_create_order(&b->oevent) &&
+             !atkgpu_event_create_order(&b->uevent) &&
              !atkgpu_event_create_order(&b->devent) &&
              (b->items = (bitem *)calloc(max_batch, sizeof(bitem))) &&
              (b->desc_h = atkgpu_hmalloc_mapped(desc_bytes(), &b->desc_d)) &&
@@ -333,6 +340,7 @@ void atkgpu_server_destroy(atkgpu_server *v) {
         if (v->st[i].used) stream_free(&v->st[i]);
     for (int i = 0; i < NINFLIGHT; ++i) {
         atkgpu_event_destroy(v->batch[i].event);
+        atkgpu_event_destroy(v->batch[i].oevent);
         atkgpu_event_destroy(v->batch[i].uevent);
         atkgpu_event_destroy(v->batch[i].devent);
         atkgpu_dfree(v->batch[i].dP);
@@ -349,8 +357,7 @@ void atkgpu_server_destroy(atkgpu_server *v) {
     atkgpu_hfree(v->qs_h);
     atkgpu_hfree(v->stage_h);
     free(v->recs);
-    free(v->lat);
-    free(v->lat_sort);
+    free(v->lat_hist);
     free(v->st);
     free(v);
 }
@@ -391,7 +398,9 @@ int atkgpu_server_add_stream(atkgpu_server *v, const atkgpu_stream_cfg *cfg) {
         return ATKDSP_E_ARG;
     /* output rows padded to 16 hops (128 bytes): whole sectors per segment */
     const size_t m_max = round16((cfg->chunk_max + hop) / hop + 1);
-    const size_t row_len = (size_t)keep + cfg->chunk_max;
+    /* rows padded to 8 samples (64 bytes): each element's row then starts
+     * aligned, whatever chunk_max is (the ingest's word stores need 16) */
+    const size_t row_len = ((size_t)keep + cfg->chunk_max + 7) & ~(size_t)7;
     size_t bytes = 2 * (size_t)nel * row_len * sizeof(atkdsp_cf32) + (size_t)kos * hop * sizeof(float) +
                    (size_t)cfg->out_slots * nel * M * m_max * sizeof(atkdsp_cf32);
     if (cfg->coherent)
@@ -686,10 +695,12 @@ static int launch_batch(atkgpu_server *v, batch_t *bt, unsigned k) {
             for (int t = 0; t < j.nblk; ++t) { j.blk0 = t; uj[nu++] = j; }
         }
     }
-    /* batch k's ingest overwrites the rows batch k-2 read */
+    /* batch k's ingest overwrites the rows batch k-2 read. The wait uses
+     * that batch's ordering event, which only this thread records and waits
+     * on; the completion thread polls the other one. */
     void *ust = INGEST_STREAM ? v->ustream : v->stream;
     if (!rc && INGEST_STREAM && k >= 2)
-        rc = atkgpu_stream_wait_event(v->ustream, v->batch[(k - 2) % NINFLIGHT].event);
+        rc = atkgpu_stream_wait_event(v->ustream, v->batch[(k - 2) % NINFLIGHT].oevent);
     if (!rc && nu) rc = atkgpu_unpack_jobs(ust, DEV(uj), nu, nu, 1, INGEST_GRID);
     if (!rc) rc = atkgpu_event_record(bt->uevent, ust);
     if (!rc && INGEST_STREAM) rc = atkgpu_stream_wait_event(v->stream, bt->uevent);
@@ -792,6 +803,7 @@ static int launch_batch(atkgpu_server *v, batch_t *bt, unsigned k) {
 #undef DEV
     if (!rc && nb && DISPLAY_STREAM) rc = atkgpu_event_record(bt->devent, v->dstream);
     if (!rc && nb && DISPLAY_STREAM) rc = atkgpu_stream_wait_event(v->stream, bt->devent);
+    if (!rc) rc = atkgpu_event_record(bt->oevent, v->stream);
     if (!rc) rc = atkgpu_event_record(bt->event, v->stream);
     return rc;
 }
@@ -933,6 +945,37 @@ static int poll_event(void *ev) {
     }
 }
 
+static int lat_bucket(double ms) {
+    if (!(ms > LAT_LO_MS)) return 0;
+    const double b = log(ms / LAT_LO_MS) / LAT_LOG_STEP + 1.0;
+    return b < (double)(LAT_NB - 1) ? (int)b : LAT_NB - 1;
+}
+
+/* the middle of bucket b (geometric), ms */
+static double lat_value(int b) { return b == 0 ? LAT_LO_MS : LAT_LO_MS * exp(((double)b - 0.5) * LAT_LOG_STEP); }
+
+static void lat_add(atkgpu_server *v, double ms) {
+    v->lat_hist[lat_bucket(ms)]++;
+    if (!v->nlat || ms < v->lat_min) v->lat_min = ms;
+    if (!v->nlat || ms > v->lat_max) v->lat_max = ms;
+    v->lat_sum += ms;
+    v->nlat++;
+}
+
+/* the latency at sorted position q (0 .. n-1) of the recorded ones, within
+ * the histogram's resolution and never outside [min, max] */
+static double lat_at(const atkgpu_server *v, size_t q) {
+    uint64_t seen = 0;
+    for (int b = 0; b < LAT_NB; ++b) {
+        seen += v->lat_hist[b];
+        if (seen > q) {
+            const double x = lat_value(b);
+            return x < v->lat_min ? v->lat_min : x > v->lat_max ? v->lat_max : x;
+        }
+    }
+    return v->lat_max;
+}
+
 static void *comp_main(void *arg) {
     atkgpu_server *v = (atkgpu_server *)arg;
     prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);      /* wake within ~1 us of the poll interval */
@@ -951,7 +994,7 @@ static void *comp_main(void *arg) {
             sstream *s = &v->st[it->sid];
             s->completed++;
             if (it->dN && it->lines_after > s->lines_done) s->lines_done = it->lines_after;
-            if (v->nlat < LAT_CAP) v->lat[v->nlat++] = (t - it->c.t_submit) * 1e3;
+            lat_add(v, (t - it->c.t_submit) * 1e3);
             v->chunks++;
             v->samples += (long long)it->c.n * it->nel;
             stage_release(v, it->c.off);
@@ -993,7 +1036,13 @@ ptrdiff_t atkgpu_server_read(atkgpu_server *v, int sid, long long ticket, atkdsp
     /* [nel][M] rows of m hops, from device rows of m_max */
     rc = atkgpu_d2h_2d_sync(out, m * sizeof(atkdsp_cf32), src, m_max * sizeof(atkdsp_cf32), m * sizeof(atkdsp_cf32),
                             (size_t)nel * M);
-    return rc ? ATKDSP_E_FFT : (ptrdiff_t)m;
+    if (rc) return ATKDSP_E_FFT;
+    /* the copy ran beside the compute stream: if a chunk that reuses the slot
+     * was dispatched meanwhile, what was copied may be torn */
+    pthread_mutex_lock(&v->mu);
+    const int overrun = ticket + (long long)s->cfg.out_slots < s->dispatched;
+    pthread_mutex_unlock(&v->mu);
+    return overrun ? ATKDSP_E_OVERRUN : (ptrdiff_t)m;
 }
 
 int atkgpu_server_read_cov(atkgpu_server *v, int sid, atkdsp_cf32 *R, uint32_t *counts) {
@@ -1054,7 +1103,12 @@ int atkgpu_server_read_line(atkgpu_server *v, int sid, long long line, float *ou
     if (failed) return ATKDSP_E_FFT;
     if (gone) return ATKDSP_E_ARG;
     if (cap < len) return ATKDSP_E_CAP;
-    return atkgpu_d2h_sync(out, src, len * sizeof(float)) ? ATKDSP_E_FFT : ATKDSP_OK;
+    if (atkgpu_d2h_sync(out, src, len * sizeof(float))) return ATKDSP_E_FFT;
+    /* a line dispatched meanwhile may have reused the ring slot during the copy */
+    pthread_mutex_lock(&v->mu);
+    const int overrun = line + (long long)s->dslots < s->lines_disp;
+    pthread_mutex_unlock(&v->mu);
+    return overrun ? ATKDSP_E_OVERRUN : ATKDSP_OK;
 }
 
 /* channels of a served line, found on the device where the line lives
@@ -1098,6 +1152,12 @@ ptrdiff_t atkgpu_server_detect(atkgpu_server *v, int sid, long long line, unsign
     }
     pthread_mutex_unlock(&v->qmu);
     if (rc) return ATKDSP_E_FFT;
+    /* the query ran beside the display stream: the line's slot must not have
+     * been reused by a line dispatched meanwhile */
+    pthread_mutex_lock(&v->mu);
+    const int overrun = line + (long long)s->dslots < s->lines_disp;
+    pthread_mutex_unlock(&v->mu);
+    if (overrun) return ATKDSP_E_OVERRUN;
     if (found > out_cap) return ATKDSP_E_CAP;
     atk_sort_channels(out, kept);
     return (ptrdiff_t)kept;
@@ -1159,6 +1219,7 @@ int atkgpu_server_detect_latest(atkgpu_server *v, const int *sids, unsigned n, f
     char *dbase = (char *)v->qs_d, *hbase = (char *)v->qs_h;
 #define QDEV(p) ((void *)(dbase + ((char *)(p) - hbase)))
     unsigned slot[4096];
+    long long line_of[4096];
     if (n > 4096) return ATKDSP_E_ARG;
     pthread_mutex_lock(&v->qmu);
     pthread_mutex_lock(&v->mu);
@@ -1175,7 +1236,12 @@ int atkgpu_server_detect_latest(atkgpu_server *v, const int *sids, unsigned n, f
         if (lines) lines[i] = L;
         counts[i] = 0;
         if (L < 0) continue;
+        if (L + (long long)s->dslots < s->lines_disp) {       /* its slot is already being reused */
+            counts[i] = ATKDSP_E_OVERRUN;
+            continue;
+        }
         ptr_h[nl] = s->dring + (size_t)(L % s->dslots) * s->nel * cols;
+        line_of[nl] = L;
         slot[nl++] = i;
     }
     const int failed = v->failed;
@@ -1188,8 +1254,23 @@ int atkgpu_server_detect_latest(atkgpu_server *v, const int *sids, unsigned n, f
             atkgpu_sync(v->qstream))
             rc = ATKDSP_E_FFT;
     }
+    /* lines dispatched during the query may have reused a slot: those
+     * streams report ATKDSP_E_OVERRUN */
+    unsigned char stale[4096];
+    if (!rc) {
+        pthread_mutex_lock(&v->mu);
+        for (unsigned j = 0; j < nl; ++j) {
+            const sstream *s = &v->st[sids[slot[j]]];
+            stale[j] = line_of[j] + (long long)s->dslots < s->lines_disp;
+        }
+        pthread_mutex_unlock(&v->mu);
+    }
     for (unsigned j = 0; !rc && j < nl; ++j) {
         const unsigned i = slot[j];
+        if (stale[j]) {
+            counts[i] = ATKDSP_E_OVERRUN;
+            continue;
+        }
         const unsigned long long found = found_h[j];
         const size_t kept = found < kcap ? (size_t)found : kcap;
         atkdsp_channel *o = out + (size_t)i * per_cap;
@@ -1222,11 +1303,6 @@ int atkgpu_server_drain(atkgpu_server *v) {
     return rc;
 }
 
-static int cmp_d(const void *a, const void *b) {
-    const double x = *(const double *)a, y = *(const double *)b;
-    return x < y ? -1 : x > y;
-}
-
 int atkgpu_server_stats(atkgpu_server *v, atkgpu_serve_stats *out, int reset) {
     if (!v || !out) return ATKDSP_E_ARG;
     pthread_mutex_lock(&v->mu);
@@ -1239,18 +1315,18 @@ int atkgpu_server_stats(atkgpu_server *v, atkgpu_serve_stats *out, int reset) {
     for (unsigned i = 0; i < v->max_streams; ++i) out->streams += v->st[i].used ? 1u : 0u;
     out->busy_s = v->nlat ? v->t_last - v->t_first : 0.0;
     const size_t n = v->nlat;
+    out->lat_count = (long long)n;
+    out->lat_rel_res = exp(LAT_LOG_STEP) - 1.0;
     if (n) {
-        memcpy(v->lat_sort, v->lat, n * sizeof(double));
-        qsort(v->lat_sort, n, sizeof(double), cmp_d);
-        double sum = 0;
-        for (size_t i = 0; i < n; ++i) sum += v->lat_sort[i];
-        out->mean_ms = sum / (double)n;
-        out->p50_ms = v->lat_sort[(size_t)(0.50 * (double)(n - 1))];
-        out->p99_ms = v->lat_sort[(size_t)(0.99 * (double)(n - 1))];
-        out->p999_ms = v->lat_sort[(size_t)(0.999 * (double)(n - 1))];
-        out->max_ms = v->lat_sort[n - 1];
+        out->mean_ms = v->lat_sum / (double)n;
+        out->p50_ms = lat_at(v, (size_t)(0.50 * (double)(n - 1)));
+        out->p99_ms = lat_at(v, (size_t)(0.99 * (double)(n - 1)));
+        out->p999_ms = lat_at(v, (size_t)(0.999 * (double)(n - 1)));
+        out->max_ms = v->lat_max;
     }
     if (reset) {
+        memset(v->lat_hist, 0, LAT_NB * sizeof(uint64_t));
+        v->lat_sum = 0.0;
         v->nlat = 0;
         v->chunks = 0;
         v->samples = 0;

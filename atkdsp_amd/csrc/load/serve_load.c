@@ -559,17 +559,19 @@ int main(int argc, char **argv) {
     printf("{\"mix\": \"%s\", \"arrays\": %d, \"streams\": %d, \"refused\": %d, \"offered_msps\": %.2f, \"speed\": %.2f, \"seconds\": %.2f, "
            "\"chunks\": %lld, \"samples\": %lld, \"batches\": %lld, \"msamples_per_s\": %.2f, "
            "\"p50_ms\": %.4f, \"p99_ms\": %.4f, \"p999_ms\": %.4f, \"max_ms\": %.4f, \"mean_ms\": %.4f, "
+           "\"latency_chunks\": %lld, \"latency_resolution\": %.4f, "
            "\"hbm_used_gib\": %.3f, \"hbm_budget_gib\": %.1f, \"display\": %d, \"lines\": %lld, "
            "\"ring\": %d, \"ring_lost\": %lld, \"ring_pickup_mean_ms\": %.4f, \"ring_pickup_max_ms\": %.4f, "
            "\"ring_call_mean_us\": %.2f, \"ring_call_max_ms\": %.4f, \"consumers\": %d, \"ring_blocks\": %d, \"ring_pass_max_ms\": %.3f, \"arrival_late_max_ms\": %.3f, "
            "\"detect\": %d, \"detect_lines\": %lld, \"detect_missed\": %lld, \"detect_channels\": %lld, "
-           "\"detect_p50_ms\": %.4f, \"detect_p99_ms\": %.4f, \"detect_max_ms\": %.4f, \"host_cpus_busy\": %.2f}\n",
+           "\"detect_p50_ms\": %.4f, \"detect_p99_ms\": %.4f, \"detect_max_ms\": %.4f, \"detect_latency_calls\": %zu, "
+           "\"host_cpus_busy\": %.2f}\n",
            uniform ? "uniform" : "c4", arrays, admitted, refused, total_rate / 1e6, speed, seconds, st.chunks, st.samples, st.batches,
            st.busy_s > 0 ? st.samples / st.busy_s / 1e6 : 0.0, st.p50_ms, st.p99_ms, st.p999_ms, st.max_ms,
-           st.mean_ms, st.hbm_used / 1073741824.0, st.hbm_budget / 1073741824.0, display, lines, g_ring, g_lost,
+           st.mean_ms, st.lat_count, st.lat_rel_res, st.hbm_used / 1073741824.0, st.hbm_budget / 1073741824.0, display, lines, g_ring, g_lost,
            g_picked ? 1e3 * g_pick_sum / (double)g_picked : 0.0, 1e3 * g_pick_max,
            g_picked ? 1e6 * g_call_sum / (double)g_picked : 0.0, 1e3 * g_call_max, g_ring ? g_consumers : 0, g_ring ? g_ring_blocks : 0, 1e3 * g_pass_max, 1e3 * g_late_max,
-           g_det_lat ? g_detect : 0, g_det_lines, g_det_missed, g_det_chans, dp50, dp99, dmax, cpus);
+           g_det_lat ? g_detect : 0, g_det_lines, g_det_missed, g_det_chans, dp50, dp99, dmax, (size_t)g_det_n, cpus);
     if (g_cfar)
         for (int i = 0; i < g_ns; ++i) atkdsp_cfar_destroy(g_cfar[i]);
     free(g_cfar);

@@ -38,7 +38,8 @@ class _Stats(C.Structure):
     _fields_ = [("chunks", C.c_longlong), ("samples", C.c_longlong), ("batches", C.c_longlong),
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("p999_ms", C.c_double),
                 ("max_ms", C.c_double), ("mean_ms", C.c_double), ("busy_s", C.c_double),
-                ("hbm_used", C.c_size_t), ("hbm_budget", C.c_size_t), ("streams", C.c_uint)]
+                ("hbm_used", C.c_size_t), ("hbm_budget", C.c_size_t), ("streams", C.c_uint),
+                ("lat_count", C.c_longlong), ("lat_rel_res", C.c_double)]
 
 
 def _bind(lib) -> None:

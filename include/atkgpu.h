@@ -129,6 +129,9 @@ typedef struct {
     double busy_s;              /* first submit to last completion */
     size_t hbm_used, hbm_budget;
     unsigned streams;
+    long long lat_count;        /* chunks the latency figures cover: every completed chunk */
+    double lat_rel_res;         /* relative resolution of p50/p99/p99.9 (log-spaced histogram);
+                                   mean and max are exact */
 } atkgpu_serve_stats;
 
 typedef struct atkgpu_server atkgpu_server;
@@ -157,7 +160,9 @@ ATKGPU_API long long atkgpu_server_submit_ring(atkgpu_server *v, int sid, atkdsp
                                                atkdsp_ring_meta *meta, uint64_t *lost);
 ATKGPU_API int       atkgpu_server_wait(atkgpu_server *v, int sid, long long ticket);
 /* the chunk's filterbank output, [nchan][m] channel-major; returns m. Valid
- * until out_slots later chunks of the stream have been dispatched. */
+ * until out_slots later chunks of the stream have been dispatched:
+ * ATKDSP_E_ARG when that happened before the call, ATKDSP_E_OVERRUN (the copy
+ * may be torn) when it happened during the copy. */
 ATKGPU_API ptrdiff_t atkgpu_server_read(atkgpu_server *v, int sid, long long ticket, atkdsp_cf32 *out, size_t cap);
 ATKGPU_API int       atkgpu_server_drain(atkgpu_server *v);
 /* a coherent stream's W1 over every chunk submitted so far (waits for them):
@@ -169,7 +174,8 @@ ATKGPU_API int       atkgpu_server_stats(atkgpu_server *v, atkgpu_serve_stats *s
 /* display lines: how many are complete, and line L as [nel][disp_cols] dB
  * values (waits for the chunk that completes it; L must be completed by the
  * chunks submitted so far). Valid until disp_slots later lines of the stream
- * have been dispatched. */
+ * have been dispatched: ATKDSP_E_ARG when that happened before the call,
+ * ATKDSP_E_OVERRUN (the copy may be torn) when it happened during it. */
 ATKGPU_API long long atkgpu_server_lines(atkgpu_server *v, int sid);
 ATKGPU_API int       atkgpu_server_read_line(atkgpu_server *v, int sid, long long line, float *out, size_t cap);
 /* the line's consumer on the device: atkdsp_detect_channels over element
@@ -177,7 +183,8 @@ ATKGPU_API int       atkgpu_server_read_line(atkgpu_server *v, int sid, long lon
  * strongest first. floor_db NaN = the median of the 5-bin-smoothed line
  * (signal_id's floor), computed on the device; *floor_used (may be NULL)
  * gets the floor applied. ATKDSP_E_CAP when more than out_cap are found (out
- * then holds the first out_cap in bin order). */
+ * then holds the first out_cap in bin order); ATKDSP_E_OVERRUN when the
+ * line's ring slot was reused while the query ran. */
 ATKGPU_API ptrdiff_t atkgpu_server_detect(atkgpu_server *v, int sid, long long line, unsigned element,
                                           float floor_db, float threshold_db, int gap_bins, int min_run,
                                           atkdsp_channel *out, size_t out_cap, float *floor_used);

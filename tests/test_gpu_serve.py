@@ -66,7 +66,8 @@ def test_serving_matches_dropin_and_alone(gpu):
     which = list(range(len(SHAPES)))
     mix, st = _serve(gpu, streams, which)
     assert st["chunks"] == sum(len(s[6]) - 1 for s in streams)
-    assert st["p50_ms"] > 0 and st["p99_ms"] >= st["p50_ms"]
+    assert st["p50_ms"] > 0 and st["p99_ms"] >= st["p50_ms"] and st["max_ms"] >= st["p999_ms"] >= st["p99_ms"]
+    assert st["lat_count"] == st["chunks"]                  # every chunk is in the latency figures
     for i in which:
         fmt, f, M, K, off, raw, cuts = streams[i]
         x, _ = O.unpack_dc(raw, f, off)
@@ -152,3 +153,69 @@ def test_coherent_array_stream_in_the_mix(gpu):
     with pytest.raises(gpu.AtkDspError):
         srv.read_cov(sids[0])                                       # not an array
     srv.close()
+
+
+def test_array_stream_with_odd_chunk_max(gpu):
+    """An array stream whose chunk_max is odd (its element rows then need
+    padding to stay aligned) served beside a single stream: both equal the
+    drop-in filterbank, to the bit."""
+    from atkdsp_amd import synth
+    from atkdsp_amd.serve import Server
+    rng = np.random.default_rng(5)
+    nel, M, K, n, cmax = 3, 256, 8, 70_003, 20_001
+    els = synth.c5_elements(n=n, nel=nel, fs=20e6)
+    single = _streams(nsamp=70_003)[1]
+    srv = Server(hbm_budget=4 << 30, max_streams=4, max_batch=16, staging_bytes=64 << 20)
+    aid = srv.add_stream(fmt="cf32", fs=20e6, nchan=M, oversample=2, taps_per_poly=K, chunk_max=cmax,
+                         out_slots=64, nel=nel)
+    sid = srv.add_stream(fmt=single[0], fs=FS, nchan=single[2], oversample=2, taps_per_poly=single[3],
+                         offset=single[4], chunk_max=cmax, out_slots=64)
+    cuts = [0]
+    while cuts[-1] < n:
+        cuts.append(min(n, cuts[-1] + int(rng.integers(1, cmax + 1))))
+    ta, ts = [], []
+    bps = O.BPS[single[1]]
+    for c in range(len(cuts) - 1):
+        a, b = cuts[c], cuts[c + 1]
+        ta.append(srv.submit(aid, np.concatenate([e[a:b] for e in els])))
+        ts.append(srv.submit(sid, single[5][a * bps:b * bps]))
+    ya = np.concatenate([p for p in (srv.read(aid, t) for t in ta) if p.shape[2]], axis=2)
+    ys = np.concatenate([p for p in (srv.read(sid, t) for t in ts) if p.shape[1]], axis=1)
+    srv.close()
+    assert np.array_equal(ya, gpu.Pfb(20e6, M, 2, K, 80.0, max_nel=nel).process_n(els, nel))
+    x, _ = O.unpack_dc(single[5], single[1], single[4])
+    assert np.array_equal(ys, gpu.Pfb(FS, single[2], 2, single[3], 80.0).process(x))
+
+
+def test_a_read_is_exact_or_refused(gpu):
+    """With two output slots and later chunks already submitted, reading an
+    older chunk either returns exactly the drop-in's output for it or is
+    refused (its slot gone before the call, or reused during the copy):
+    never a torn copy."""
+    from atkdsp_amd.serve import Server
+    fmt, f, M, K, off, raw, cuts = _streams(seed=3, nsamp=200_000)[0]
+    bps = O.BPS[f]
+    srv = Server(hbm_budget=1 << 30, max_streams=2, max_batch=8, staging_bytes=64 << 20)
+    sid = srv.add_stream(fmt=fmt, fs=FS, nchan=M, oversample=2, taps_per_poly=K, offset=off, chunk_max=20_000,
+                         out_slots=2)
+    pfb = gpu.Pfb(FS, M, 2, K, 80.0)
+    exact = refused = 0
+    prev = None
+    for c in range(len(cuts) - 1):
+        seg = raw[cuts[c] * bps:cuts[c + 1] * bps]
+        t = srv.submit(sid, seg)
+        x, _ = O.unpack_dc(seg, f, off)
+        ref = pfb.process(x)
+        if prev is not None:
+            pt, pref = prev
+            try:
+                y = srv.read(sid, pt)
+            except gpu.AtkDspError as e:
+                assert "overrun" in str(e) or "bad argument" in str(e), e
+                refused += 1
+            else:
+                assert np.array_equal(y, pref), c
+                exact += 1
+        prev = (t, ref)
+    srv.close()
+    assert exact + refused == len(cuts) - 2 and exact > 0
