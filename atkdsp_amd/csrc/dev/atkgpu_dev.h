@@ -32,6 +32,8 @@ int         atkgpu_d2h(void *s, void *dst, const void *src, size_t bytes);
 int         atkgpu_d2d(void *s, void *dst, const void *src, size_t bytes);
 int         atkgpu_d2h_2d(void *s, void *dst, size_t dpitch, const void *src,
                           size_t spitch, size_t width, size_t height);
+int         atkgpu_d2d_2d(void *s, void *dst, size_t dpitch, const void *src, size_t spitch,
+                          size_t width, size_t height);
 int         atkgpu_fill_u32(void *s, void *dst, uint32_t v, size_t count);
 int         atkgpu_event_create(void **e);
 void        atkgpu_event_destroy(void *e);

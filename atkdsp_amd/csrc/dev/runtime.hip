@@ -81,6 +81,13 @@ extern "C" int atkgpu_d2h_2d(void *s, void *dst, size_t dpitch, const void *src,
                             (hipStream_t)s) == hipSuccess ? 0 : -4;
 }
 
+extern "C" int atkgpu_d2d_2d(void *s, void *dst, size_t dpitch, const void *src, size_t spitch,
+                             size_t width, size_t height) {
+    if (!width || !height) return 0;
+    return hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, hipMemcpyDeviceToDevice,
+                            (hipStream_t)s) == hipSuccess ? 0 : -4;
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)

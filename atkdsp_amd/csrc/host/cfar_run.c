@@ -724,7 +724,7 @@ ptrdiff_t atkdsp_cfar_process(atkdsp_cfar *c, const float *frames_db, size_t nfr
     size_t nev = 0;
     for (size_t f0 = 0; f0 < nframes; f0 += c->slab) {
         const size_t nf = nframes - f0 < c->slab ? nframes - f0 : c->slab;
-        if (atkgpu_h2d(c->stream, c->d_db, frames_db + f0 * nb, nf * nb * sizeof(float))) return ATKDSP_E_FFT;
+        if (atk_put(c->stream, c->d_db, frames_db + f0 * nb, nf * nb * sizeof(float))) return ATKDSP_E_FFT;
         const ptrdiff_t got = run_slab(c, nf, f0, frame_t0, frame_dt, ev + nev);
         if (got < 0) return got;
         nev += (size_t)got;

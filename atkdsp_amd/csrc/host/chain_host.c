@@ -154,7 +154,7 @@ void *atkgpu_chain_input(atkgpu_chain *c, unsigned s) {
 
 int atkgpu_chain_upload(atkgpu_chain *c, unsigned s, const void *raw, size_t nbytes) {
     if (!c || s >= c->S || !raw || nbytes > c->cfg.block_max * c->bps) return ATKDSP_E_ARG;
-    return atkgpu_h2d(c->stream, atkgpu_chain_input(c, s), raw, nbytes) ? ATKDSP_E_FFT : ATKDSP_OK;
+    return atk_put(c->stream, atkgpu_chain_input(c, s), raw, nbytes) ? ATKDSP_E_FFT : ATKDSP_OK;
 }
 
 ptrdiff_t atkgpu_chain_run(atkgpu_chain *c, size_t n) {

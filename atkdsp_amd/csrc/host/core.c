@@ -184,11 +184,11 @@ static ptrdiff_t unpack_run(const void *raw, size_t nbytes, int fmt, atkdsp_cf32
     for (size_t a = 0; a < n && !rc; a += pool->chunk) {
         const size_t c = n - a < pool->chunk ? n - a : pool->chunk;
         double msum[2];
-        rc = atkgpu_h2d(s->stream, s->d_a, (const char *)raw + a * (size_t)bps, c * (size_t)bps);
+        rc = atk_put(s->stream, s->d_a, (const char *)raw + a * (size_t)bps, c * (size_t)bps);
         if (!rc) rc = atkgpu_unpack(s->stream, s->d_a, fmt, c, cre, cim, k, off_re, off_im, s->d_b,
                                     d_parts, want_mean ? d_mean : NULL);
         if (!rc && pole) rc = atkgpu_dc_pole(s->stream, s->d_b, c, dc_alpha, d_state, (double *)s->d_c);
-        if (!rc) rc = atkgpu_d2h(s->stream, out + a, s->d_b, c * sizeof(atkdsp_cf32));
+        if (!rc) rc = atk_get(s->stream, out + a, s->d_b, c * sizeof(atkdsp_cf32));
         if (!rc && want_mean) rc = atkgpu_d2h(s->stream, msum, d_mean, sizeof msum);
         if (!rc) rc = atkgpu_sync(s->stream);
         if (!rc && want_mean) {
@@ -266,9 +266,9 @@ void atkdsp_db_to_pixels(const float *db, size_t n, float lo, float hi, const ui
     for (size_t a = 0; a < n; a += pool->chunk) {
         const size_t c = n - a < pool->chunk ? n - a : pool->chunk;
         if (atkgpu_h2d(s->stream, d_lut, lut256, 256 * sizeof(uint32_t)) ||
-            atkgpu_h2d(s->stream, s->d_a, db + a, c * sizeof(float)) ||
+            atk_put(s->stream, s->d_a, db + a, c * sizeof(float)) ||
             atkgpu_db_to_pixels(s->stream, (const float *)s->d_a, c, lo, hi, d_lut, (uint32_t *)s->d_b) ||
-            atkgpu_d2h(s->stream, out + a, s->d_b, c * sizeof(uint32_t)) || atkgpu_sync(s->stream))
+            atk_get(s->stream, out + a, s->d_b, c * sizeof(uint32_t)) || atkgpu_sync(s->stream))
             break;
     }
     atk_release(pool->slot, si);

@@ -91,9 +91,9 @@ int atkdsp_fft_exec(const atkdsp_fft *cp, const atkdsp_cf32 *in, atkdsp_cf32 *ou
     const int si = atk_claim(p->slot, NSLOT);
     atk_slot *s = &p->slot[si];
     const size_t n = p->n;
-    int rc = atkgpu_h2d(s->stream, s->d_a, in, n * sizeof(atkdsp_cf32));
+    int rc = atk_put(s->stream, s->d_a, in, n * sizeof(atkdsp_cf32));
     if (!rc) rc = atkgpu_fft_batch(s->stream, p->plan, s->d_a, n, NULL, 1, inverse, xbuf(p, s), scrbuf(p, s));
-    if (!rc) rc = atkgpu_d2h(s->stream, out, xbuf(p, s), n * sizeof(atkdsp_cf32));
+    if (!rc) rc = atk_get(s->stream, out, xbuf(p, s), n * sizeof(atkdsp_cf32));
     if (!rc) rc = atkgpu_sync(s->stream);
     atk_release(p->slot, si);
     return rc ? ATKDSP_E_FFT : ATKDSP_OK;
@@ -106,7 +106,7 @@ int atkdsp_power_db(const atkdsp_fft *cp, const atkdsp_cf32 *in, const float *wi
     atk_slot *s = &p->slot[si];
     const size_t n = p->n;
     planes q = slot_planes(s, n);
-    int rc = atkgpu_h2d(s->stream, s->d_a, in, n * sizeof(atkdsp_cf32));
+    int rc = atk_put(s->stream, s->d_a, in, n * sizeof(atkdsp_cf32));
     if (!rc && window) rc = atkgpu_h2d(s->stream, q.win, window, n * sizeof(float));
     if (!rc) rc = atkgpu_fft_batch(s->stream, p->plan, s->d_a, n, window ? q.win : NULL, 1, 0, xbuf(p, s),
                                    scrbuf(p, s));
@@ -129,7 +129,7 @@ static int frames_run(atkdsp_fft *p, atk_slot *s, const atkdsp_cf32 *in, size_t 
     for (long long f0 = 0; f0 < frames && !rc; f0 += (long long)F) {
         const size_t nf = (size_t)(frames - f0) < F ? (size_t)(frames - f0) : F;
         const size_t span = (nf - 1) * hop + n;
-        rc = atkgpu_h2d(s->stream, s->d_a, in + (size_t)f0 * hop, span * sizeof(atkdsp_cf32));
+        rc = atk_put(s->stream, s->d_a, in + (size_t)f0 * hop, span * sizeof(atkdsp_cf32));
         if (!rc) rc = atkgpu_fft_batch(s->stream, p->plan, s->d_a, hop, d_win, nf, 0, xbuf(p, s), scrbuf(p, s));
         if (!rc) {
             if (stats) rc = atkgpu_spec_stats_accum(s->stream, xbuf(p, s), n, nf, f0 == 0, q.mx, q.mn, q.su, q.s2);

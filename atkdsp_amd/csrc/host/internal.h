@@ -38,6 +38,16 @@ typedef struct {
 } atk_pool;
 atk_pool *atk_pool_get(void);
 
+/* Copies between caller-owned host memory and the device on `stream`
+ * (xfer_host.c): large ones through pinned staging. atk_put returns once src
+ * has been read (the device copy lands in stream order); atk_get and
+ * atk_get_rows return once dst is written (after the stream's earlier work).
+ * atk_get_rows reads `rows` packed rows of `width` bytes and writes them
+ * `dpitch` bytes apart. Nonzero on a device error. */
+int atk_put(void *stream, void *dst, const void *src, size_t bytes);
+int atk_get(void *stream, void *dst, const void *src, size_t bytes);
+int atk_get_rows(void *stream, void *dst, size_t dpitch, const void *src, size_t width, size_t rows);
+
 /* channel records strongest first, with the reference's comparator and qsort */
 void atk_sort_channels(atkdsp_channel *c, size_t n);
 

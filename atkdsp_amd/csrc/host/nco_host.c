@@ -51,9 +51,9 @@ void atkdsp_nco_mix(atkdsp_nco *nco, const atkdsp_cf32 *in, atkdsp_cf32 *out, si
     int rc = 0;
     for (size_t a = 0; a < n && !rc; a += pool->chunk) {
         const size_t c = n - a < pool->chunk ? n - a : pool->chunk;
-        rc = atkgpu_h2d(s->stream, s->d_a, in + a, c * sizeof(atkdsp_cf32));
+        rc = atk_put(s->stream, s->d_a, in + a, c * sizeof(atkdsp_cf32));
         if (!rc) rc = atkgpu_nco_mix(s->stream, s->d_a, s->d_b, c, a, ph0, step);
-        if (!rc) rc = atkgpu_d2h(s->stream, out + a, s->d_b, c * sizeof(atkdsp_cf32));
+        if (!rc) rc = atk_get(s->stream, out + a, s->d_b, c * sizeof(atkdsp_cf32));
         if (!rc) rc = atkgpu_sync(s->stream);
     }
     atk_release(pool->slot, si);
@@ -112,7 +112,7 @@ void *atkgpu_nco_output(atkgpu_nco *v) { return v ? v->d_out : NULL; }
 
 int atkgpu_nco_upload(atkgpu_nco *v, const atkdsp_cf32 *in, size_t n) {
     if (!v || !in || n > v->chunk_max) return ATKDSP_E_ARG;
-    return atkgpu_h2d(v->stream, v->d_in, in, n * sizeof(atkdsp_cf32)) ? ATKDSP_E_FFT : ATKDSP_OK;
+    return atk_put(v->stream, v->d_in, in, n * sizeof(atkdsp_cf32)) ? ATKDSP_E_FFT : ATKDSP_OK;
 }
 
 int atkgpu_nco_run(atkgpu_nco *v, size_t n) {
@@ -134,7 +134,7 @@ int atkgpu_nco_sync(atkgpu_nco *v) {
 
 ptrdiff_t atkgpu_nco_read(atkgpu_nco *v, atkdsp_cf32 *out, size_t n) {
     if (!v || !out || n > v->chunk_max) return ATKDSP_E_ARG;
-    int rc = atkgpu_d2h(v->stream, out, v->d_out, n * sizeof(atkdsp_cf32));
+    int rc = atk_get(v->stream, out, v->d_out, n * sizeof(atkdsp_cf32));
     if (!rc) rc = atkgpu_sync(v->stream);
     return rc ? ATKDSP_E_FFT : (ptrdiff_t)n;
 }

@@ -261,9 +261,9 @@ ptrdiff_t atkdsp_arb_resampler_process(atkdsp_arb_resampler *r, const atkdsp_cf3
             k = rs_count(&r->rs, c);
         }
         if (k > r->ocap) return ATKDSP_E_FFT;
-        int rc = atkgpu_h2d(r->stream, r->d_in, in + pos, c * sizeof(atkdsp_cf32));
+        int rc = atk_put(r->stream, r->d_in, in + pos, c * sizeof(atkdsp_cf32));
         if (!rc) rc = rs_run(&r->rs, r->stream, r->d_in, 0, c, 1, r->d_out, 0, k);
-        if (!rc && k) rc = atkgpu_d2h(r->stream, out + done, r->d_out, k * sizeof(atkdsp_cf32));
+        if (!rc && k) rc = atk_get(r->stream, out + done, r->d_out, k * sizeof(atkdsp_cf32));
         if (!rc) rc = atkgpu_sync(r->stream);
         if (rc) return ATKDSP_E_FFT;
         done += k;
@@ -605,7 +605,7 @@ static int osx_sub(atkdsp_osx *x, const atkdsp_cf32 *const *ins, unsigned nel, s
     for (unsigned e = 0; e < nel && !rc; ++e) {
         atkdsp_cf32 *se = S + (size_t)e * x->spitch;
         if (x->lc) rc = atkgpu_h2d(x->stream, se + ov, x->lo + (size_t)e * hop, x->lc * sizeof(atkdsp_cf32));
-        if (!rc && c) rc = atkgpu_h2d(x->stream, se + ov + x->lc, ins[e] + pos, c * sizeof(atkdsp_cf32));
+        if (!rc && c) rc = atk_put(x->stream, se + ov + x->lc, ins[e] + pos, c * sizeof(atkdsp_cf32));
         /* every block of the element: nfft samples, hop apart */
         if (!rc) rc = atkgpu_fft_batch(x->stream, x->plan, se, hop, NULL, ms, 0, x->d_X + (size_t)e * x->mcap * nfft,
                                        x->d_scr);

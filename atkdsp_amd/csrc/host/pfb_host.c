@@ -238,7 +238,7 @@ static int pfb_sub(atkdsp_pfb *p, const atkdsp_cf32 *const *ins, unsigned nel, s
     for (unsigned e = 0; e < nel && !rc; ++e) {
         atkdsp_cf32 *se = S + (size_t)e * p->spitch;
         if (p->lc) rc = atkgpu_h2d(p->stream, se + ov, p->lo + (size_t)e * hop, p->lc * sizeof(atkdsp_cf32));
-        if (!rc && c) rc = atkgpu_h2d(p->stream, se + ov + p->lc, ins[e] + pos, c * sizeof(atkdsp_cf32));
+        if (!rc && c) rc = atk_put(p->stream, se + ov + p->lc, ins[e] + pos, c * sizeof(atkdsp_cf32));
     }
     if (p->kos) {
         if (!rc) rc = atkgpu_pfb_fused(p->stream, S, p->spitch, (size_t)ov + avail, nel, ms, (long long)p->b, M,
@@ -252,8 +252,16 @@ static int pfb_sub(atkdsp_pfb *p, const atkdsp_cf32 *const *ins, unsigned nel, s
                                               atkgpu_fftplan_twiddles(p->plan),
                                               p->cal_nel ? (const void *)p->d_cal : NULL, p->cal_nel, p->d_out);
     }
-    if (!rc) rc = atkgpu_d2h_2d(p->stream, out + i_done, m * sizeof(atkdsp_cf32), p->d_out,
-                                mp * sizeof(atkdsp_cf32), ms * sizeof(atkdsp_cf32), (size_t)nel * M);
+    /* the rows packed on the device first (at HBM speed): one packed
+     * device-to-host copy instead of nel * M short rows */
+    const atkdsp_cf32 *packed = p->d_out;
+    if (!rc && mp != ms) {
+        rc = atkgpu_d2d_2d(p->stream, p->d_fold, ms * sizeof(atkdsp_cf32), p->d_out, mp * sizeof(atkdsp_cf32),
+                           ms * sizeof(atkdsp_cf32), (size_t)nel * M);
+        packed = p->d_fold;
+    }
+    if (!rc) rc = atk_get_rows(p->stream, out + i_done, m * sizeof(atkdsp_cf32), packed, ms * sizeof(atkdsp_cf32),
+                               (size_t)nel * M);
     /* next history: s[ms hop, ms hop + ov) into the other slab buffer */
     atkdsp_cf32 *T = p->d_s[p->cur ^ 1];
     for (unsigned e = 0; e < nel && !rc; ++e)

@@ -210,7 +210,7 @@ void *atkgpu_pfbx_input(atkgpu_pfbx *v, unsigned s) {
 
 int atkgpu_pfbx_upload(atkgpu_pfbx *v, unsigned s, const void *raw, size_t nbytes) {
     if (!v || s >= v->S || !raw || nbytes > v->cfg.block_max * (size_t)v->bps) return ATKDSP_E_ARG;
-    return atkgpu_h2d(v->stream, atkgpu_pfbx_input(v, s), raw, nbytes) ? ATKDSP_E_FFT : ATKDSP_OK;
+    return atk_put(v->stream, atkgpu_pfbx_input(v, s), raw, nbytes) ? ATKDSP_E_FFT : ATKDSP_OK;
 }
 
 /* the descriptor set for (cur, n): unpack jobs (FP32 form) then copy jobs */
