@@ -50,6 +50,38 @@ int         atkgpu_d2h_2d_sync(void *dst, size_t dpitch, const void *src, size_t
 /* ---- unpack --------------------------------------------------------------- */
 /* number of double pairs in the mean partial buffer */
 #define ATKGPU_MEAN_PARTS 1024
+/* samples one pass of the pooled entry points moves (unpack forms its mean
+ * as FP64 sums per pass, added in pass order) */
+#define ATKGPU_POOL_CHUNK ((size_t)1 << 22)
+
+/* The folded constants of a format, formed in float exactly as
+ * src/convert.c:46-68 does (formats by the ABI's values: 0 cu8, 1 ci8,
+ * 2 ci16, 3 ci16q11, 4 cf32). */
+static inline void atkgpu_unpack_consts(int fmt, float off_re, float off_im, float *cre, float *cim, float *k) {
+    switch (fmt) {
+    case 0:
+        *k = 1.0f / 127.5f;
+        *cre = 127.5f + off_re / *k;
+        *cim = 127.5f + off_im / *k;
+        break;
+    case 1:
+        *k = 1.0f / 128.0f;
+        *cre = off_re / *k;
+        *cim = off_im / *k;
+        break;
+    case 2:
+    case 3:
+        *k = fmt == 2 ? (1.0f / 32768.0f) : (1.0f / 2048.0f);
+        *cre = off_re / *k;
+        *cim = off_im / *k;
+        break;
+    default:
+        *k = 1.0f;
+        *cre = 0.0f;
+        *cim = 0.0f;
+        break;
+    }
+}
 /* out[i] = ((float)raw - c) * k per component (cf32 input: x - off).
  * When sum2 (2 doubles, device) is non-NULL it receives the FP64 sums of out,
  * formed through partials (2*ATKGPU_MEAN_PARTS doubles) in a fixed order. */

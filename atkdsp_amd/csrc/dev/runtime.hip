@@ -9,9 +9,13 @@
 
 static char g_devname[128] = "";
 
+extern "C" int atkgpu_noalloc_setup(void);
+
 extern "C" int atkgpu_init(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return -4;
+    // the allocation-free entry point's scratch address on this device
+    if (atkgpu_noalloc_setup() != 0) return -4;
     if (!g_devname[0]) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, 0) == hipSuccess)
@@ -25,7 +29,10 @@ extern "C" const char *atkgpu_device_name(void) { return g_devname; }
 extern "C" void *atkgpu_dmalloc(size_t bytes) {
     void *p = nullptr;
     if (bytes == 0) bytes = 256;
-    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();            // the caller handles NULL; no error is left for a launch check
+        return nullptr;
+    }
     return p;
 }
 
@@ -36,7 +43,10 @@ extern "C" void atkgpu_dfree(void *p) {
 extern "C" void *atkgpu_hmalloc(size_t bytes) {
     void *p = nullptr;
     if (bytes == 0) bytes = 256;
-    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
     return p;
 }
 

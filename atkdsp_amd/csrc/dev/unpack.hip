@@ -7,8 +7,11 @@
 // bit-identical to the reference. Contraction is off in this file: nothing
 // here may become an FMA.
 #include <hip/hip_runtime.h>
+#include <atomic>
+#include <mutex>
 #include "atkgpu_dev.h"
 #include "dev_common.hpp"
+#include "atkgpu.h"
 
 #pragma clang fp contract(off)
 
@@ -240,4 +243,97 @@ extern "C" int atkgpu_dc_pole(void *s, void *x, size_t n, float alpha, float *st
     hipLaunchKernelGGL(k_pole_apply, dim3(g), dim3(256), 0, st, (cf *)x, n, alpha,
                        (const double *)start, state);
     return LAUNCH_OK();
+}
+
+// ---- unpack with the block mean into device buffers, allocating nothing -------------
+// The scratch of the mean: the FP64 partials of one pass's blocks (reused pass
+// after pass, in stream order) and the FP64 sums of each pass. One copy per
+// GPU, resolved to its device address once per device and held until the
+// process ends. Every slot a call reads, the same call wrote first.
+#define NOALLOC_PASSES 64
+static_assert((size_t)NOALLOC_PASSES * ATKGPU_POOL_CHUNK == ATKGPU_NOALLOC_MAX_SAMPLES,
+              "the header's maximum is the scratch's pass count times the pass length");
+static __device__ __align__(256) double scratch[2 * ATKGPU_MEAN_PARTS + 2 * NOALLOC_PASSES];
+
+#define NOALLOC_MAX_DEV 64
+static std::atomic<double *> g_scratch_at[NOALLOC_MAX_DEV];
+static std::mutex g_scratch_mu;
+
+// the scratch's device address on the current device (resolved on first use
+// there); nullptr when it cannot be had
+static double *scratch_here() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEV) return nullptr;
+    double *p = g_scratch_at[dev].load(std::memory_order_acquire);
+    if (p) return p;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    p = g_scratch_at[dev].load(std::memory_order_relaxed);
+    if (!p) {
+        void *q = nullptr;
+        if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) {
+            (void)hipGetLastError();        // leave no error for the next launch check
+            return nullptr;
+        }
+        p = (double *)q;
+        g_scratch_at[dev].store(p, std::memory_order_release);
+    }
+    return p;
+}
+
+// called by the library's init on the current device
+extern "C" int atkgpu_noalloc_setup(void) { return scratch_here() ? 0 : -4; }
+
+// the mean from the pass sums, added in pass order from zero, then scaled and
+// offset as the pooled entry point does on the host (no contraction here)
+__global__ void k_mean_from_passes(const double *__restrict__ sums, int passes, unsigned long long n, float ore,
+                                   float oim, double *__restrict__ mean) {
+    if (threadIdx.x != 0) return;
+    if (n == 0) {
+        mean[0] = (double)ore;
+        mean[1] = (double)oim;
+        return;
+    }
+    double sr = 0.0, si = 0.0;
+    for (int j = 0; j < passes; ++j) {
+        sr += sums[2 * j];
+        si += sums[2 * j + 1];
+    }
+    const double inv = 1.0 / (double)n;
+    mean[0] = sr * inv + (double)ore;
+    mean[1] = si * inv + (double)oim;
+}
+
+extern "C" ptrdiff_t atkgpu_unpack_dc_noalloc(void *stream, const void *raw, size_t nbytes, int fmt,
+                                              atkdsp_cf32 *out, size_t out_cap, float off_re, float off_im,
+                                              double *mean) {
+    if (!raw || !out) return ATKDSP_E_ARG;
+    size_t bps = 0;
+    switch (fmt) {
+    case FMT_CU8: case FMT_CI8: bps = 2; break;
+    case FMT_CI16: case FMT_CI16Q11: bps = 4; break;
+    case FMT_CF32: bps = 8; break;
+    default: return ATKDSP_E_ARG;
+    }
+    size_t n = nbytes / bps;
+    if (n > out_cap) n = out_cap;
+    if (n > ATKGPU_NOALLOC_MAX_SAMPLES) return ATKDSP_E_CAP;
+    double *scr = scratch_here();
+    if (!scr) return ATKDSP_E_FFT;
+    double *parts = scr, *sums = scr + 2 * ATKGPU_MEAN_PARTS;
+    float cre, cim, k;
+    atkgpu_unpack_consts(fmt, off_re, off_im, &cre, &cim, &k);
+    const size_t passes = (n + ATKGPU_POOL_CHUNK - 1) / ATKGPU_POOL_CHUNK;
+    for (size_t j = 0; j < passes; ++j) {
+        const size_t a = j * ATKGPU_POOL_CHUNK;
+        const size_t c = n - a < ATKGPU_POOL_CHUNK ? n - a : ATKGPU_POOL_CHUNK;
+        if (atkgpu_unpack(stream, (const char *)raw + a * bps, fmt, c, cre, cim, k, off_re, off_im, out + a,
+                          parts, mean ? sums + 2 * j : nullptr))
+            return ATKDSP_E_FFT;
+    }
+    if (mean) {
+        hipLaunchKernelGGL(k_mean_from_passes, dim3(1), dim3(64), 0, (hipStream_t)stream, sums, (int)passes,
+                           (unsigned long long)n, off_re, off_im, mean);
+        if (LAUNCH_OK()) return ATKDSP_E_FFT;
+    }
+    return (ptrdiff_t)n;
 }

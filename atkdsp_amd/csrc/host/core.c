@@ -100,7 +100,7 @@ int atk_claim(atk_slot *slots, int n) {
 void atk_release(atk_slot *slots, int i) { atomic_store(&slots[i].busy, 0); }
 
 /* ---- the library-wide pool -------------------------------------------------------- */
-#define POOL_CHUNK ((size_t)1 << 22)           /* samples a slot moves per pass */
+#define POOL_CHUNK ATKGPU_POOL_CHUNK           /* samples a slot moves per pass */
 
 static atk_pool g_pool;
 static atk_pool *g_pool_ok;
@@ -127,31 +127,6 @@ atk_pool *atk_pool_get(void) {
 }
 
 /* ---- unpack ------------------------------------------------------------------------ */
-/* The folded constants, formed in float exactly as src/convert.c:46-68 does. */
-static void unpack_consts(int fmt, float off_re, float off_im, float *cre, float *cim, float *k) {
-    switch (fmt) {
-    case ATKDSP_FMT_CU8:
-        *k = 1.0f / 127.5f;
-        *cre = 127.5f + off_re / *k;
-        *cim = 127.5f + off_im / *k;
-        break;
-    case ATKDSP_FMT_CI8:
-        *k = 1.0f / 128.0f;
-        *cre = off_re / *k;
-        *cim = off_im / *k;
-        break;
-    case ATKDSP_FMT_CI16:
-    case ATKDSP_FMT_CI16Q11:
-        *k = fmt == ATKDSP_FMT_CI16 ? (1.0f / 32768.0f) : (1.0f / 2048.0f);
-        *cre = off_re / *k;
-        *cim = off_im / *k;
-        break;
-    default:
-        *k = 1.0f; *cre = 0.0f; *cim = 0.0f;
-        break;
-    }
-}
-
 static ptrdiff_t unpack_run(const void *raw, size_t nbytes, int fmt, atkdsp_cf32 *out, size_t out_cap,
                             float off_re, float off_im, double *mean_re, double *mean_im,
                             float dc_alpha, atkdsp_cf32 *dc_state) {
@@ -172,7 +147,7 @@ static ptrdiff_t unpack_run(const void *raw, size_t nbytes, int fmt, atkdsp_cf32
     atk_pool *pool = atk_pool_get();
     if (!pool) return ATKDSP_E_FFT;
     float cre, cim, k;
-    unpack_consts(fmt, off_re, off_im, &cre, &cim, &k);
+    atkgpu_unpack_consts(fmt, off_re, off_im, &cre, &cim, &k);
     const int si = atk_claim(pool->slot, pool->nslot);
     atk_slot *s = &pool->slot[si];
     double *d_parts = (double *)s->d_small;                       /* 2*PARTS doubles */

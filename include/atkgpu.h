@@ -280,6 +280,27 @@ ATKGPU_API int         atkgpu_nco_phase(atkgpu_nco *v, double *phase, long long 
 /* the phase after each of the first log_max runs */
 ATKGPU_API ptrdiff_t   atkgpu_nco_log(atkgpu_nco *v, double *phases, size_t cap);
 
+/* ---- unpack with the block mean, allocating nothing ---------------------------
+ * No counterpart in the reference: the device-buffer form of atkdsp_unpack_dc
+ * (include/atkdsp.h) for callers that cannot allocate or synchronize, such as
+ * code under HIP graph capture and tight serving loops. raw, out and mean are
+ * device pointers: out receives min(nbytes / bytes_per_sample, out_cap)
+ * samples, bit-identical to atkdsp_unpack_dc's, and mean (2 doubles, or NULL)
+ * the mean of the samples before the offset came out, bit-identical to its
+ * mean_re / mean_im. Work is queued on `stream` (NULL: the null stream) and
+ * not waited for; the return value is the sample count.
+ *
+ * The mean's scratch (the FP64 block partials of one pass and the sums of up
+ * to 64 passes of 2^22 samples, 17 KiB) is one static device array per GPU,
+ * held until the process ends: calls that overlap on one GPU race on it, so
+ * serialize them (one stream, or an event between). More than
+ * ATKGPU_NOALLOC_MAX_SAMPLES samples: ATKDSP_E_CAP, nothing queued. No
+ * hipMalloc, hipFree, memset or host copy is made. */
+#define ATKGPU_NOALLOC_MAX_SAMPLES ((size_t)64 << 22)
+ATKGPU_API ptrdiff_t atkgpu_unpack_dc_noalloc(void *stream, const void *raw, size_t nbytes, int fmt,
+                                               atkdsp_cf32 *out, size_t out_cap, float off_re, float off_im,
+                                               double *mean);
+
 #ifdef __cplusplus
 }
 #endif
