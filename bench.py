@@ -483,6 +483,9 @@ def main() -> int:
                     help="skip the precision-form comparison and the NCO phase run")
     ap.add_argument("--cfar-only", action="store_true",
                     help="only time the CFAR detector and tracker (and the reference C beside it) and exit")
+    ap.add_argument("--only", choices=["filterbank_forms_wideband", "precision_form", "drop_in", "osx_extract",
+                                       "nco_phase_run", "coherent_array"],
+                    help="only run this one measured section (for profiling it alone), print it and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB -> 1 GiB copy kernel 5 times (the reference point for "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -503,6 +506,16 @@ def main() -> int:
         return 2
     import atkdsp_amd
     from atkdsp_amd.chain import Chain
+    if args.only:
+        if args.only in ("filterbank_forms_wideband", "drop_in"):
+            from atkdsp_amd import synth
+            base = np.frombuffer(synth.c2_bytes(n=N_BLOCK), "<i2").reshape(-1, 2)
+            res = filterbank_forms_c2(base) if args.only == "filterbank_forms_wideband" else drop_in(base)
+        else:
+            res = {"precision_form": precision_forms, "osx_extract": osx_extract, "nco_phase_run": nco_phase_run,
+                   "coherent_array": coherent_array}[args.only]()
+        print(json.dumps({args.only: res}))
+        return 0
     if args.cfar_only:
         r, why = _ref_library()
         print(json.dumps({"cfar": cfar_frames(), "cpu_baseline_cfar": cpu_baseline_cfar(r) if r else why}))
