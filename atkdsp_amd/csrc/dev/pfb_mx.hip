@@ -123,7 +123,10 @@ extern "C" int atkgpu_pfb_mx(void *s, const void *raw, size_t spitch, size_t nva
     a.cal_nel = (int)cal_nel;
     a.out = (cpx *)out;
     a.opitch = (long long)opitch;
-    const long long ngroups = ((long long)m + G - 1) / G;
+    // groups start on multiples of 16/os hops (absolute): the first begins
+    // b0 mod (16/os) hops before hop 0
+    const long long c0 = b0 % (long long)(16 / os);
+    const long long ngroups = ((long long)m + c0 + G - 1) / G;
     // about two resident blocks' worth of groups per CU at a time
     long long gpb = ((long long)nel * ngroups + 2047) / 2048;
     if (gpb < 1) gpb = 1;

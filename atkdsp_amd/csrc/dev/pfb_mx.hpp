@@ -156,6 +156,15 @@ __global__ void __launch_bounds__(512, 4) k_pfb_mx(PfbMxArgs a) {
     for (int i = tid; i < M; i += NTHR) twl[i] = a.tw[i];
     const cpx *cal = (a.cal && e < a.cal_nel) ? a.cal + (long long)e * M : nullptr;
     const bool flip_base = (a.b0 & 1) != 0;
+    // Groups are anchored to the absolute hop index: a group starts on a
+    // multiple of HA, so a hop's offset in its tile (and with it the places
+    // of its taps along the product's k dimension, which fix the order the
+    // matrix unit sums them in) depends on the hop alone, not on where the
+    // run started. The run's first group starts c0 hops before hop 0; those
+    // hops are computed and not stored, and their rows before the run's
+    // first window are staged as zeros (they reach no stored hop).
+    const int c0 = (int)(a.b0 % HA);
+    auto gstart = [&](int gi) { return gi * G - c0; };
 
     // the padded tap rows and the tap sums of every column
     {
@@ -190,14 +199,17 @@ __global__ void __launch_bounds__(512, 4) k_pfb_mx(PfbMxArgs a) {
             const int jb = idx / HOP, u = idx % HOP;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int si = min((i0 + 8 * jb + q) * HOP + u, nlast);
-                pre[it][q] = (idx < S::NRB * HOP) ? mx_load<FMT>(raw + (long long)si * BPS) : 0u;
+                const int row = i0 + 8 * jb + q;
+                const int si = min(row * HOP + u, nlast);
+                pre[it][q] = (idx < S::NRB * HOP && row >= 0) ? mx_load<FMT>(raw + (long long)si * BPS) : 0u;
             }
         }
     };
     auto stage = [&]() {
         uint32_t *lx32 = (uint32_t *)lx;
-        // only groups at the stream's very start reach back before its first sample
+        // only groups at the stream's very start reach back before its first
+        // sample, and a run's first group before its first window: every row
+        // there (row * HOP + u < 0 <= zlo) is staged as zero
         const bool zcheck = pre_i0 * HOP < a.zlo;
 #pragma unroll
         for (int it = 0; it < S::NIT; ++it) {
@@ -226,16 +238,16 @@ __global__ void __launch_bounds__(512, 4) k_pfb_mx(PfbMxArgs a) {
             }
         }
     };
-    if (tile * G < m) {
-        load_group(tile * G);
+    if (gstart(tile) < m) {
+        load_group(gstart(tile));
         stage();
-        const int nx = (tile + ntile) * G;
+        const int nx = gstart(tile + ntile);
         if (1 < a.groups_per_block && nx < m) load_group(nx);
     }
     __syncthreads();
 
     for (int grp = 0; grp < a.groups_per_block; ++grp) {
-        const int i0 = (tile + grp * ntile) * G;
+        const int i0 = gstart(tile + grp * ntile);
         if (i0 >= m) break;
         // ---- fold on MFMA, one column at a time ----
         // (a lane's output rows 4g..4g+3 share one branch: orr)
@@ -289,9 +301,9 @@ __global__ void __launch_bounds__(512, 4) k_pfb_mx(PfbMxArgs a) {
         __syncthreads();
         // the rows are free: stage the next group (overlapping this group's
         // FFT and store), then start the loads of the one after
-        if (grp + 1 < a.groups_per_block && (tile + (grp + 1) * ntile) * G < m) {
+        if (grp + 1 < a.groups_per_block && gstart(tile + (grp + 1) * ntile) < m) {
             stage();
-            const int nx = (tile + (grp + 2) * ntile) * G;
+            const int nx = gstart(tile + (grp + 2) * ntile);
             if (grp + 2 < a.groups_per_block && nx < m) load_group(nx);
         }
         // ---- FFT, rotation, cal (as the FP32 kernel) ----
@@ -328,7 +340,7 @@ __global__ void __launch_bounds__(512, 4) k_pfb_mx(PfbMxArgs a) {
                     cpx *ok = oute + k * opitch + i0;
 #pragma unroll
                     for (int p = lp; p < G; p += S::LPC)
-                        if (i0 + p < m) ok[p] = lds[p * ROW + k];
+                        if (i0 + p >= 0 && i0 + p < m) ok[p] = lds[p * ROW + k];
                 }
             }
         }

@@ -72,29 +72,44 @@ def test_forms_on_c1_against_fp64(gpu):
     assert rows["fp16x2"]["rel_full_scale"] < rows["fp16"]["rel_full_scale"] / 10
 
 
+MX_CONTINUITY = [
+    ("bf16x2", "cu8", C1),
+    ("fp16x2", "cu8", C1),
+    ("bf16", "cu8", C1),
+    ("bf16x2", "cu8", dict(fs=2.4e6, nchan=64, oversample=2, taps_per_poly=8, atten_db=80.0)),
+    ("fp16x2", "ci16q11", dict(fs=10e6, nchan=128, oversample=2, taps_per_poly=8, atten_db=80.0)),
+]
+
+
 @pytest.mark.parametrize("blk", [1, 7, 999, 4096])
-def test_block_continuity(gpu, blk):
-    """Blocks of 1, 7, 999 and 4096 samples give the one-call result: history,
-    leftover and hop parity carry across runs, the zero history only at the start."""
+@pytest.mark.parametrize("form,fmt,shape", MX_CONTINUITY, ids=lambda v: v if isinstance(v, str) else None)
+def test_block_continuity(gpu, blk, form, fmt, shape):
+    """Blocks of 1, 7, 999 and 4096 samples give the one-call result bit for
+    bit: history, leftover and hop parity carry across runs, the zero history
+    only at the start, and every hop sits at the same place in its matrix tile
+    (groups start on absolute hop multiples) whatever the block boundaries."""
     from atkdsp_amd import synth
     from atkdsp_amd.pfbx import PfbX
+    bps = {"cu8": 2, "ci16q11": 4}[fmt]
     nsamp = 6000 if blk < 100 else 60_000
-    raw = synth.c1_bytes()[: 2 * nsamp]
+    raw = (synth.c1_bytes() if fmt == "cu8"
+           else synth.quantize(synth.wideband_iq(31, nsamp, shape["fs"]), O.CI16Q11))[: bps * nsamp]
     off = 0.01 - 0.02j
-    _, y64 = _oracle(raw, O.CU8, off, **C1)
-    one = PfbX(form="bf16x2", fmt="cu8", block_max=8192, offset=off, **C1)
-    whole = np.concatenate([one.process([raw[a:a + 2 * 8192]], 2)[0] for a in range(0, len(raw), 2 * 8192)], 1)
-    px = PfbX(form="bf16x2", fmt="cu8", block_max=((blk + 7) // 8) * 8, offset=off, **C1)
+    one = PfbX(form=form, fmt=fmt, block_max=nsamp, offset=off, **shape)
+    whole = one.process([raw], bps)[0]
+    px = PfbX(form=form, fmt=fmt, block_max=((blk + 7) // 8) * 8, offset=off, **shape)
     parts = []
     for a in range(0, nsamp, blk):
         b = min(nsamp, a + blk)
-        px.upload(0, raw[2 * a:2 * b])
+        px.upload(0, raw[bps * a:bps * b])
         if px.run(b - a):
             parts.append(px.read(0))
     y = np.concatenate(parts, 1)
-    assert y.shape == y64.shape == whole.shape
-    assert _errors(y, y64)[0] < BOUND["bf16x2"]
-    assert float(np.max(np.abs(y - whole)) / np.max(np.abs(whole))) < 2e-6
+    assert y.shape == whole.shape
+    assert np.array_equal(y.view(np.uint32), whole.view(np.uint32))
+    if shape is C1:
+        _, y64 = _oracle(raw, O.CU8, off, **C1)
+        assert _errors(y, y64)[0] < BOUND[form]
 
 
 @pytest.mark.parametrize("fmt,nchan,os_,K,form", [
