@@ -176,9 +176,9 @@ int atkgpu_pfb_fused(void *s, const void *sbuf, size_t spitch, size_t nvalid, un
 int atkgpu_chain_unpack(void *s, const void *raw, size_t raw_pitch, int fmt, size_t n, void *row0,
                         size_t spitch, float *off, double *partials, int parts, double *mean,
                         int update, unsigned S);
-/* every frame f < frames at row0 + s*spitch + f*fhop (N in 1024/2048/4096):
- * window, FFT, |X|^2/N^2, detector -> partial[S][nfb][N] -> line_db[S][N] */
-/* the chain's display reduction straight from the raw blocks (frames >= 1),
+/* the chain's display reduction straight from the raw blocks (frames >= 1,
+ * N in 1024/2048/4096; every frame f at row0 + s*spitch + f*fhop: window,
+ * FFT, |X|^2/N^2, detector -> partial[S][nfb][N] -> line_db[S][N]),
  * unpacking each block into its row on the way: row0 is stream 0's frame 0,
  * the block starts dlen samples after it; then the dB lines and the mean of
  * every block (FP64 sums in mparts, [S][64][2]) into mean and, with update,
@@ -187,9 +187,6 @@ int atkgpu_chain_display_unpack(void *st, void *row0, size_t spitch, const void 
                                 size_t n, size_t dlen, float *off, unsigned S, size_t frames, size_t fhop, unsigned N,
                                 int nfb, const float *win, const void *tw, int det, float *partial, float *line_db,
                                 double *mparts, double *mean, int update);
-int atkgpu_chain_display(void *s, const void *row0, size_t spitch, unsigned S, size_t frames,
-                         size_t fhop, unsigned N, int nfb, const float *win, const void *tw, int det,
-                         float *partial, float *line_db);
 /* decimate line_db[S][N] to dec[S][cols]; stitch into trace[S][nout] (NULL: none)
  * with stream s's segment starting at seg_lo0 + s*seg_lo_step */
 int atkgpu_chain_post(void *s, const float *line_db, unsigned S, unsigned N, unsigned cols, float *dec,

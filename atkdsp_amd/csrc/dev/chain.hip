@@ -21,7 +21,6 @@
 //   k_chain_unpack   raw -> cf32 into the row with the FP64 partial sums, for
 //                    a block that completes no display frame
 //   k_chain_mean     per stream: FP64 mean -> next block's offset
-//   k_disp_reduce    the display reduction over a cf32 row
 //   k_disp_final     partial lines in fixed order -> dB line (src/fft.c:286-292)
 //   k_disp_post      decimate_max (src/display.c:19-37) and the NaN-aware
 //                    stitch of the decimated line (src/detect.c:116-134)
@@ -174,35 +173,6 @@ extern "C" int atkgpu_chain_unpack(void *st, const void *raw, size_t raw_pitch, 
     hipLaunchKernelGGL(k_chain_mean, dim3((S + 63) / 64), dim3(64), 0, s, (const double *)partials, parts,
                        (long long)n, off, mean, update, (int)S);
     return LAUNCH_OK();
-}
-
-// ---- the display reduction --------------------------------------------------------------
-struct DispArgs {
-    const cpx *row0;            // stream s frame 0 at row0 + s*spitch
-    long long spitch;
-    long long frames, fhop;
-    int fpb;                    // frames per block
-    int nfb;                    // frame blocks per stream
-    const float *win;           // N or null
-    const cpx *tw;              // N twiddles
-    int det;
-    float *partial;             // [S][nfb][N] display order (fftshifted)
-};
-
-template <int N, int DET>
-__global__ void __launch_bounds__(256) k_disp_reduce(DispArgs a) {
-    using D = DispShape<N>;
-    __shared__ cpx lds[D::TEAMS * D::TL];
-    __shared__ float red[D::TEAMS > 1 ? D::TEAMS * N : 1];
-    __shared__ cpx twl[N];                 // twiddles: every stage reads them from LDS
-    const int s = blockIdx.y, fb = blockIdx.x;
-    for (int i = threadIdx.x; i < N; i += 256) twl[i] = a.tw[i];
-    __syncthreads();
-    // frame offsets within a stream row fit 32 bits
-    const int f0 = fb * a.fpb;
-    const int f1 = min(f0 + a.fpb, (int)a.frames);
-    DispRowSrc src{a.row0 + (long long)s * a.spitch, (int)a.fhop};
-    disp_frames<N, DET, 256>(src, f0, f1, a.win, twl, lds, red, a.partial + ((long long)s * a.nfb + fb) * N);
 }
 
 __global__ void k_disp_final(const float *__restrict__ partial, int nfb, int N, int det, long long frames,
@@ -506,47 +476,6 @@ k_disp_post(const float *__restrict__ line_db, int N, int cols, float *__restric
         }
         t[k] = cur;
     }
-}
-
-template <int N>
-static int launch_disp(const DispArgs &a, unsigned S, hipStream_t s) {
-    const dim3 g((unsigned)a.nfb, S);
-    switch (a.det) {
-    case 0: hipLaunchKernelGGL((k_disp_reduce<N, 0>), g, dim3(256), 0, s, a); break;
-    case 1: hipLaunchKernelGGL((k_disp_reduce<N, 1>), g, dim3(256), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_disp_reduce<N, 2>), g, dim3(256), 0, s, a); break;
-    }
-    return LAUNCH_OK();
-}
-
-extern "C" int atkgpu_chain_display(void *st, const void *row0, size_t spitch, unsigned S, size_t frames,
-                                    size_t fhop, unsigned N, int nfb, const float *win, const void *tw, int det,
-                                    float *partial, float *line_db) {
-    if (!frames || !S) return 0;
-    DispArgs a;
-    a.row0 = (const cpx *)row0;
-    a.spitch = (long long)spitch;
-    a.frames = (long long)frames;
-    a.fhop = (long long)fhop;
-    a.nfb = nfb;
-    a.fpb = (int)((frames + nfb - 1) / nfb);
-    a.nfb = (int)((frames + a.fpb - 1) / a.fpb);
-    a.win = win;
-    a.tw = (const cpx *)tw;
-    a.det = det;
-    a.partial = partial;
-    hipStream_t s = (hipStream_t)st;
-    int rc;
-    switch (N) {
-    case 1024: rc = launch_disp<1024>(a, S, s); break;
-    case 2048: rc = launch_disp<2048>(a, S, s); break;
-    case 4096: rc = launch_disp<4096>(a, S, s); break;
-    default: return -1;
-    }
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_disp_final, dim3((N + 255) / 256, S), dim3(256), 0, s, (const float *)partial, a.nfb,
-                       (int)N, det, (long long)frames, line_db);
-    return LAUNCH_OK();
 }
 
 extern "C" int atkgpu_chain_post(void *st, const float *line_db, unsigned S, unsigned N, unsigned cols, float *dec,

@@ -50,21 +50,6 @@ __device__ __forceinline__ float disp_db(float v, float scale) {
     return 20.0f * log10f(sqrtf(v * scale) + 1e-12f);
 }
 
-// A frame's samples from a cf32 row: frame f starts at row + f*fhop.
-struct DispRowSrc {
-    const cpx *row;
-    int fhop;
-    // samples j + r*NT (r < PT) of frame f
-    template <int PT, int NT>
-    __device__ __forceinline__ void load(int f, int j, cpx *v) const {
-        const cpx *fr = row + f * fhop + j;
-#pragma unroll
-        for (int r = 0; r < PT; ++r) v[r] = fr[r * NT];
-    }
-    template <int PT, int NT>
-    __device__ __forceinline__ void own(int, int, const cpx *) {}
-};
-
 // Frames [f0, f1) of one stream, a thread's samples j + r*NT of frame f being
 // src.load<PT, NT>(f, j, v). Every thread of the NTHR-thread block calls it
 // (the team FFT synchronises the block); src.own<PT, NT>(f, j, v) is called
