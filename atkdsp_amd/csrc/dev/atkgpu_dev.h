@@ -50,9 +50,11 @@ int         atkgpu_d2h_2d_sync(void *dst, size_t dpitch, const void *src, size_t
 /* ---- unpack --------------------------------------------------------------- */
 /* number of double pairs in the mean partial buffer */
 #define ATKGPU_MEAN_PARTS 1024
-/* samples one pass of the pooled entry points moves (unpack forms its mean
- * as FP64 sums per pass, added in pass order) */
+/* samples one pass of the pooled entry points moves */
 #define ATKGPU_POOL_CHUNK ((size_t)1 << 22)
+/* unpack forms its mean as FP64 sums over passes of this many samples (one
+ * launch each), added in pass order */
+#define ATKGPU_MEAN_PASS ((size_t)1 << 19)
 
 /* The folded constants of a format, formed in float exactly as
  * src/convert.c:46-68 does (formats by the ABI's values: 0 cu8, 1 ci8,

@@ -150,7 +150,10 @@ extern "C" int atkgpu_event_elapsed_ms(void *e0, void *e1, float *ms) {
 extern "C" int atkgpu_event_query(void *e) {
     const hipError_t r = hipEventQuery((hipEvent_t)e);
     if (r == hipSuccess) return 1;
-    if (r == hipErrorNotReady) return 0;
+    if (r == hipErrorNotReady) {
+        (void)hipGetLastError();            // "not yet" is no error for a later launch check
+        return 0;
+    }
     return -4;
 }
 

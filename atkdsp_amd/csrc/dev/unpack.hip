@@ -247,11 +247,12 @@ extern "C" int atkgpu_dc_pole(void *s, void *x, size_t n, float alpha, float *st
 
 // ---- unpack with the block mean into device buffers, allocating nothing -------------
 // The scratch of the mean: the FP64 partials of one pass's blocks (reused pass
-// after pass, in stream order) and the FP64 sums of each pass. One copy per
-// GPU, resolved to its device address once per device and held until the
-// process ends. Every slot a call reads, the same call wrote first.
-#define NOALLOC_PASSES 64
-static_assert((size_t)NOALLOC_PASSES * ATKGPU_POOL_CHUNK == ATKGPU_NOALLOC_MAX_SAMPLES,
+// after pass, in stream order) and the FP64 sums of each pass (a pass is
+// ATKGPU_MEAN_PASS samples, as in atkdsp_unpack_dc). One copy per GPU,
+// resolved to its device address once per device and held until the process
+// ends. Every slot a call reads, the same call wrote first.
+#define NOALLOC_PASSES 512
+static_assert((size_t)NOALLOC_PASSES * ATKGPU_MEAN_PASS == ATKGPU_NOALLOC_MAX_SAMPLES,
               "the header's maximum is the scratch's pass count times the pass length");
 static __device__ __align__(256) double scratch[2 * ATKGPU_MEAN_PARTS + 2 * NOALLOC_PASSES];
 
@@ -322,10 +323,10 @@ extern "C" ptrdiff_t atkgpu_unpack_dc_noalloc(void *stream, const void *raw, siz
     double *parts = scr, *sums = scr + 2 * ATKGPU_MEAN_PARTS;
     float cre, cim, k;
     atkgpu_unpack_consts(fmt, off_re, off_im, &cre, &cim, &k);
-    const size_t passes = (n + ATKGPU_POOL_CHUNK - 1) / ATKGPU_POOL_CHUNK;
+    const size_t passes = (n + ATKGPU_MEAN_PASS - 1) / ATKGPU_MEAN_PASS;
     for (size_t j = 0; j < passes; ++j) {
-        const size_t a = j * ATKGPU_POOL_CHUNK;
-        const size_t c = n - a < ATKGPU_POOL_CHUNK ? n - a : ATKGPU_POOL_CHUNK;
+        const size_t a = j * ATKGPU_MEAN_PASS;
+        const size_t c = n - a < ATKGPU_MEAN_PASS ? n - a : ATKGPU_MEAN_PASS;
         if (atkgpu_unpack(stream, (const char *)raw + a * bps, fmt, c, cre, cim, k, off_re, off_im, out + a,
                           parts, mean ? sums + 2 * j : nullptr))
             return ATKDSP_E_FFT;

@@ -81,6 +81,11 @@ int atk_slot_init(atk_slot *s, size_t a, size_t b, size_t c) {
 
 void atk_slot_free(atk_slot *s) {
     if (s->stream) atkgpu_sync(s->stream);
+    if (s->stream2) {
+        atkgpu_sync(s->stream2);
+        atkgpu_stream_destroy(s->stream2);
+        s->stream2 = NULL;
+    }
     atkgpu_dfree(s->d_a); atkgpu_dfree(s->d_b); atkgpu_dfree(s->d_c); atkgpu_dfree(s->d_small);
     if (s->stream) atkgpu_stream_destroy(s->stream);
     s->d_a = s->d_b = s->d_c = s->d_small = NULL;
@@ -113,7 +118,8 @@ static void pool_init(void) {
     for (int i = 0; i < g_pool.nslot; ++i) {
         /* a: raw / input block, b: converted block, c: pole scratch */
         if (atk_slot_init(&g_pool.slot[i], POOL_CHUNK * 8, POOL_CHUNK * 8,
-                          (4 * (POOL_CHUNK / 256 + 1) + 16) * sizeof(double))) {
+                          (4 * (POOL_CHUNK / 256 + 1) + 16) * sizeof(double)) ||
+            atkgpu_stream_create(&g_pool.slot[i].stream2)) {
             for (int j = 0; j <= i; ++j) atk_slot_free(&g_pool.slot[j]);
             return;
         }
@@ -127,6 +133,22 @@ atk_pool *atk_pool_get(void) {
 }
 
 /* ---- unpack ------------------------------------------------------------------------ */
+/* one piece of a pipelined unpack: items [i0, i0 + cnt) of the slot's
+ * buffers; the piece is one pass of the mean (its sums at pass i0 / pass) */
+typedef struct {
+    const atk_slot *s;
+    int fmt, bps;
+    float cre, cim, k, off_re, off_im;
+    double *parts, *sums;
+} unpack_piece_ctx;
+
+static int unpack_piece(void *vc, void *st, size_t i0, size_t cnt) {
+    const unpack_piece_ctx *c = (const unpack_piece_ctx *)vc;
+    return atkgpu_unpack(st, (const char *)c->s->d_a + i0 * (size_t)c->bps, c->fmt, cnt, c->cre, c->cim, c->k,
+                         c->off_re, c->off_im, (atkdsp_cf32 *)c->s->d_b + i0, c->parts,
+                         c->sums ? c->sums + 2 * (i0 / ATKGPU_MEAN_PASS) : NULL);
+}
+
 static ptrdiff_t unpack_run(const void *raw, size_t nbytes, int fmt, atkdsp_cf32 *out, size_t out_cap,
                             float off_re, float off_im, double *mean_re, double *mean_im,
                             float dc_alpha, atkdsp_cf32 *dc_state) {
@@ -150,30 +172,45 @@ static ptrdiff_t unpack_run(const void *raw, size_t nbytes, int fmt, atkdsp_cf32
     atkgpu_unpack_consts(fmt, off_re, off_im, &cre, &cim, &k);
     const int si = atk_claim(pool->slot, pool->nslot);
     atk_slot *s = &pool->slot[si];
+    /* small scratch: the block partials, the pole state, the pass sums */
     double *d_parts = (double *)s->d_small;                       /* 2*PARTS doubles */
-    double *d_mean = d_parts + 2 * ATKGPU_MEAN_PARTS;             /* 2 per chunk     */
-    float *d_state = (float *)(d_mean + 2);                       /* pole state      */
+    float *d_state = (float *)(d_parts + 2 * ATKGPU_MEAN_PARTS);  /* pole state      */
+    double *d_sums = d_parts + 2 * ATKGPU_MEAN_PARTS + 2;         /* 2 per pass      */
     double sum_re = 0.0, sum_im = 0.0;
     int rc = 0;
-    if (pole) rc = atkgpu_h2d(s->stream, d_state, dc_state, sizeof(atkdsp_cf32));
-    for (size_t a = 0; a < n && !rc; a += pool->chunk) {
-        const size_t c = n - a < pool->chunk ? n - a : pool->chunk;
-        double msum[2];
-        rc = atk_put(s->stream, s->d_a, (const char *)raw + a * (size_t)bps, c * (size_t)bps);
-        if (!rc) rc = atkgpu_unpack(s->stream, s->d_a, fmt, c, cre, cim, k, off_re, off_im, s->d_b,
-                                    d_parts, want_mean ? d_mean : NULL);
-        if (!rc && pole) rc = atkgpu_dc_pole(s->stream, s->d_b, c, dc_alpha, d_state, (double *)s->d_c);
-        if (!rc) rc = atk_get(s->stream, out + a, s->d_b, c * sizeof(atkdsp_cf32));
-        if (!rc && want_mean) rc = atkgpu_d2h(s->stream, msum, d_mean, sizeof msum);
-        if (!rc) rc = atkgpu_sync(s->stream);
-        if (!rc && want_mean) {
-            sum_re += msum[0];
-            sum_im += msum[1];
+    if (pole) {
+        /* the one-pole DC block runs over whole chunks (its scan's chunking
+         * fixes its rounding); no mean is asked for with it */
+        rc = atkgpu_h2d(s->stream, d_state, dc_state, sizeof(atkdsp_cf32));
+        for (size_t a = 0; a < n && !rc; a += pool->chunk) {
+            const size_t c = n - a < pool->chunk ? n - a : pool->chunk;
+            rc = atk_put(s->stream, s->d_a, (const char *)raw + a * (size_t)bps, c * (size_t)bps);
+            if (!rc) rc = atkgpu_unpack(s->stream, s->d_a, fmt, c, cre, cim, k, off_re, off_im, s->d_b, d_parts, NULL);
+            if (!rc) rc = atkgpu_dc_pole(s->stream, s->d_b, c, dc_alpha, d_state, (double *)s->d_c);
+            if (!rc) rc = atk_get(s->stream, out + a, s->d_b, c * sizeof(atkdsp_cf32));
+            if (!rc) rc = atkgpu_sync(s->stream);
         }
-    }
-    if (!rc && pole) {
-        rc = atkgpu_d2h(s->stream, dc_state, d_state, sizeof(atkdsp_cf32));
+        if (!rc) rc = atkgpu_d2h(s->stream, dc_state, d_state, sizeof(atkdsp_cf32));
         if (!rc) rc = atkgpu_sync(s->stream);
+    } else {
+        /* pipelined: pieces of one mean pass, each copied in, unpacked and
+         * copied back while the next goes in */
+        unpack_piece_ctx ctx = {s, fmt, bps, cre, cim, k, off_re, off_im, d_parts, want_mean ? d_sums : NULL};
+        for (size_t a = 0; a < n && !rc; a += pool->chunk) {
+            const size_t c = n - a < pool->chunk ? n - a : pool->chunk;
+            const size_t np = (c + ATKGPU_MEAN_PASS - 1) / ATKGPU_MEAN_PASS;
+            rc = atk_pipe(s->stream, s->stream2, (const char *)raw + a * (size_t)bps, (size_t)bps, s->d_a, out + a,
+                          sizeof(atkdsp_cf32), s->d_b, c, ATKGPU_MEAN_PASS, unpack_piece, &ctx);
+            if (!rc && want_mean) {
+                double msum[2 * (ATKGPU_POOL_CHUNK / ATKGPU_MEAN_PASS)];
+                rc = atkgpu_d2h(s->stream, msum, d_sums, 2 * np * sizeof(double));
+                if (!rc) rc = atkgpu_sync(s->stream);
+                for (size_t j = 0; j < np && !rc; ++j) {
+                    sum_re += msum[2 * j];
+                    sum_im += msum[2 * j + 1];
+                }
+            }
+        }
     }
     atk_release(pool->slot, si);
     if (rc) return rc;

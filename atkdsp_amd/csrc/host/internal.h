@@ -17,6 +17,7 @@
 typedef struct {
     atomic_int busy;
     void *stream;
+    void *stream2;             /* the library pool's slots: the return copies of atk_pipe */
     void *d_a, *d_b, *d_c;     /* general device buffers */
     size_t bytes_a, bytes_b, bytes_c;
     void *d_small;             /* 1 MiB of small device scratch (lines, sums, luts) */
@@ -47,6 +48,17 @@ atk_pool *atk_pool_get(void);
 int atk_put(void *stream, void *dst, const void *src, size_t bytes);
 int atk_get(void *stream, void *dst, const void *src, size_t bytes);
 int atk_get_rows(void *stream, void *dst, size_t dpitch, const void *src, size_t width, size_t rows);
+
+/* Elementwise work over n items of caller memory, pipelined: piece by piece
+ * (at most `piece` items), the input bytes go to d_in on `s`, run(ctx, s, i0,
+ * cnt) queues the piece's kernels on `s` (items [i0, i0 + cnt): d_in + i0 *
+ * in_bpi in, d_out + i0 * out_bpi out), and the output comes back on `s2`, so
+ * one piece's copy out overlaps the next piece's copy in. d_in and d_out hold
+ * all n items. Returns once out holds every output byte; nonzero on an error
+ * (from run, or a device error). */
+typedef int (*atk_run_fn)(void *ctx, void *s, size_t i0, size_t cnt);
+int atk_pipe(void *s, void *s2, const void *in, size_t in_bpi, void *d_in, void *out, size_t out_bpi,
+             const void *d_out, size_t n, size_t piece, atk_run_fn run, void *ctx);
 
 /* channel records strongest first, with the reference's comparator and qsort */
 void atk_sort_channels(atkdsp_channel *c, size_t n);

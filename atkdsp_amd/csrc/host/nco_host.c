@@ -36,6 +36,21 @@ void atkdsp_nco_init(atkdsp_nco *nco, double freq_hz, double sample_rate) {
     atkdsp_nco_set_freq(nco, freq_hz, sample_rate);
 }
 
+/* one piece of a pipelined mix: samples [i0, i0 + cnt) of the slot's buffers,
+ * at absolute index base + i0 */
+#define NCO_PIECE ((size_t)1 << 19)
+typedef struct {
+    const atk_slot *s;
+    size_t base;
+    double ph0, step;
+} nco_piece_ctx;
+
+static int nco_piece(void *vc, void *st, size_t i0, size_t cnt) {
+    const nco_piece_ctx *c = (const nco_piece_ctx *)vc;
+    return atkgpu_nco_mix(st, (const atkdsp_cf32 *)c->s->d_a + i0, (atkdsp_cf32 *)c->s->d_b + i0, cnt, c->base + i0,
+                          c->ph0, c->step);
+}
+
 void atkdsp_nco_mix(atkdsp_nco *nco, const atkdsp_cf32 *in, atkdsp_cf32 *out, size_t n) {
     if (!nco || !in || !out) return;
     if (nco->step == 0.0) {
@@ -49,12 +64,14 @@ void atkdsp_nco_mix(atkdsp_nco *nco, const atkdsp_cf32 *in, atkdsp_cf32 *out, si
     const int si = atk_claim(pool->slot, pool->nslot);
     atk_slot *s = &pool->slot[si];
     int rc = 0;
+    /* each sample's phase comes from its absolute index, so the pieces of
+     * the pipelined copy cannot change a value; in place (out == in) is safe:
+     * a piece's output is written only after its input was read */
     for (size_t a = 0; a < n && !rc; a += pool->chunk) {
         const size_t c = n - a < pool->chunk ? n - a : pool->chunk;
-        rc = atk_put(s->stream, s->d_a, in + a, c * sizeof(atkdsp_cf32));
-        if (!rc) rc = atkgpu_nco_mix(s->stream, s->d_a, s->d_b, c, a, ph0, step);
-        if (!rc) rc = atk_get(s->stream, out + a, s->d_b, c * sizeof(atkdsp_cf32));
-        if (!rc) rc = atkgpu_sync(s->stream);
+        nco_piece_ctx ctx = {s, a, ph0, step};
+        rc = atk_pipe(s->stream, s->stream2, in + a, sizeof(atkdsp_cf32), s->d_a, out + a, sizeof(atkdsp_cf32), s->d_b,
+                      c, NCO_PIECE, nco_piece, &ctx);
     }
     atk_release(pool->slot, si);
     if (!rc) nco->phase = host_wrap(ph0 + step * (double)n);

@@ -139,6 +139,7 @@ typedef struct {
     char *buf[XF_NBUF];
     void *ev[XF_NBUF];
     int armed[XF_NBUF];                   /* ev[b] recorded since last wait  */
+    void *kev[2];                         /* atk_pipe: a piece's kernels done */
 } xf_engine;
 
 static xf_engine g_eng[XF_NENG];
@@ -149,6 +150,8 @@ static int engine_make(xf_engine *e) {
         if (!e->buf[b] || atkgpu_event_create_order(&e->ev[b])) return -1;
         e->armed[b] = 0;
     }
+    for (int i = 0; i < 2; ++i)
+        if (atkgpu_event_create_order(&e->kev[i])) return -1;
     return 1;
 }
 
@@ -291,4 +294,71 @@ int atk_get_rows(void *stream, void *dst, size_t dpitch, const void *src, size_t
 
 int atk_get(void *stream, void *dst, const void *src, size_t bytes) {
     return atk_get_rows(stream, dst, bytes, src, bytes, 1);
+}
+
+/* ---- pipelined elementwise work ---------------------------------------------------- */
+/* Pieces k = 0, 1, ...: the input goes through pinned piece k%2 to d_in on s;
+ * the piece's kernels run on s; s2 waits for them (kev) and copies the output
+ * into pinned piece 2 + k%2; the host copies piece k-1's output out meanwhile.
+ * Without staging (small copies, or no engine), the same pieces run between
+ * one copy in and one copy out, so the results do not depend on the route. */
+int atk_pipe(void *s, void *s2, const void *in, size_t in_bpi, void *d_in, void *out, size_t out_bpi,
+             const void *d_out, size_t n, size_t piece, atk_run_fn run, void *ctx) {
+    if (!n) return 0;
+    if (!piece) piece = n;
+    const size_t big = in_bpi > out_bpi ? in_bpi : out_bpi;
+    const size_t np = (n + piece - 1) / piece;
+    xf_engine *e = NULL;
+    if (s2 && piece * big <= XF_PIECE && n * (in_bpi + out_bpi) >= XF_STAGE_MIN && np > 1) e = engine_claim();
+    if (!e) {
+        int rc = atk_put(s, d_in, in, n * in_bpi);
+        for (size_t k = 0; k < np && !rc; ++k) {
+            const size_t i0 = k * piece, cnt = n - i0 < piece ? n - i0 : piece;
+            rc = run(ctx, s, i0, cnt);
+        }
+        if (!rc) rc = atk_get(s, out, d_out, n * out_bpi);
+        return rc;
+    }
+    int rc = 0;
+    size_t drained = 0;                   /* pieces whose output is in `out` */
+    for (size_t k = 0; k < np && !rc; ++k) {
+        const size_t i0 = k * piece, cnt = n - i0 < piece ? n - i0 : piece;
+        const int bi = (int)(k & 1), bo = 2 + (int)(k & 1);
+        /* in */
+        rc = piece_wait(e, bi);
+        if (rc) break;
+        const xf_job ji = {e->buf[bi], (const char *)in + i0 * in_bpi, cnt * in_bpi, cnt * in_bpi, cnt * in_bpi, 1};
+        team_copy(&ji);
+        rc = atkgpu_h2d(s, (char *)d_in + i0 * in_bpi, e->buf[bi], cnt * in_bpi);
+        if (!rc) rc = piece_arm(e, bi, s);
+        /* the piece's kernels, then its output on s2 */
+        if (!rc) rc = run(ctx, s, i0, cnt);
+        if (!rc) rc = atkgpu_event_record(e->kev[k & 1], s);
+        if (!rc) rc = atkgpu_stream_wait_event(s2, e->kev[k & 1]);
+        if (!rc) rc = piece_wait(e, bo);          /* piece k-2's output, drained already */
+        if (!rc) rc = atkgpu_d2h(s2, e->buf[bo], (const char *)d_out + i0 * out_bpi, cnt * out_bpi);
+        if (!rc) rc = piece_arm(e, bo, s2);
+        /* the previous piece's output to the caller */
+        if (!rc && k > 0) {
+            const size_t p0 = (k - 1) * piece, pc = n - p0 < piece ? n - p0 : piece;
+            const int pb = 2 + (int)((k - 1) & 1);
+            rc = piece_wait(e, pb);
+            if (!rc) {
+                const xf_job jo = {(char *)out + p0 * out_bpi, e->buf[pb], pc * out_bpi, pc * out_bpi, pc * out_bpi, 1};
+                team_copy(&jo);
+                drained = k;
+            }
+        }
+    }
+    if (!rc && drained < np) {
+        const size_t k = np - 1, p0 = k * piece, pc = n - p0;
+        const int pb = 2 + (int)(k & 1);
+        rc = piece_wait(e, pb);
+        if (!rc) {
+            const xf_job jo = {(char *)out + p0 * out_bpi, e->buf[pb], pc * out_bpi, pc * out_bpi, pc * out_bpi, 1};
+            team_copy(&jo);
+        }
+    }
+    engine_release(e);
+    return rc;
 }

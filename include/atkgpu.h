@@ -291,12 +291,12 @@ ATKGPU_API ptrdiff_t   atkgpu_nco_log(atkgpu_nco *v, double *phases, size_t cap)
  * not waited for; the return value is the sample count.
  *
  * The mean's scratch (the FP64 block partials of one pass and the sums of up
- * to 64 passes of 2^22 samples, 17 KiB) is one static device array per GPU,
+ * to 512 passes of 2^19 samples, 24 KiB) is one static device array per GPU,
  * held until the process ends: calls that overlap on one GPU race on it, so
  * serialize them (one stream, or an event between). More than
  * ATKGPU_NOALLOC_MAX_SAMPLES samples: ATKDSP_E_CAP, nothing queued. No
  * hipMalloc, hipFree, memset or host copy is made. */
-#define ATKGPU_NOALLOC_MAX_SAMPLES ((size_t)64 << 22)
+#define ATKGPU_NOALLOC_MAX_SAMPLES ((size_t)512 << 19)
 ATKGPU_API ptrdiff_t atkgpu_unpack_dc_noalloc(void *stream, const void *raw, size_t nbytes, int fmt,
                                                atkdsp_cf32 *out, size_t out_cap, float off_re, float off_im,
                                                double *mean);
