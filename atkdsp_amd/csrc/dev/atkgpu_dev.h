@@ -109,13 +109,16 @@ int atkgpu_fft_batch(void *s, const atkgpu_fftplan *p, const void *in, size_t in
 
 /* ---- spectrum lines ----------------------------------------------------------- */
 /* acc planes (device): reduce uses float line[n]; stats uses float max, min and
- * double sum, sum2 (see spectrum.hip). first != 0 starts a fresh accumulation. */
+ * double sum, sum2 (see spectrum.hip). first != 0 starts a fresh accumulation.
+ * part: device scratch of the frame groups' partials, at least
+ * atkgpu_spec_part_entries(n) doubles. */
+size_t atkgpu_spec_part_entries(size_t n);
 int atkgpu_spec_accum(void *s, const void *X, size_t n, size_t frames, int detector,
-                      int first, float *line);
+                      int first, float *line, void *part);
 int atkgpu_spec_reduce_db(void *s, const float *line, size_t n, int detector,
                           long long frames, float *out_db);
 int atkgpu_spec_stats_accum(void *s, const void *X, size_t n, size_t frames, int first,
-                            float *mx, float *mn, double *su, double *s2);
+                            float *mx, float *mn, double *su, double *s2, void *part);
 int atkgpu_spec_stats_db(void *s, size_t n, long long frames, const float *mx,
                          const float *mn, const double *su, const double *s2,
                          float *max_db, float *avg_db, float *min_db, float *sk);

@@ -11,7 +11,7 @@
 #include "internal.h"
 
 #define NSLOT 4
-#define CHUNK_SAMPLES ((size_t)1 << 20)
+#define CHUNK_SAMPLES ((size_t)1 << 21)
 
 struct atkdsp_fft {
     size_t n;
@@ -22,8 +22,12 @@ struct atkdsp_fft {
 };
 
 /* slot buffers: a = input samples, b = transforms, c = line planes
- * (float line, float max, float min, double sum, double sum2, window, 4 dB lines) */
-static size_t planes_bytes(size_t n) { return n * (5 * sizeof(float) + 2 * sizeof(double) + 4 * sizeof(float)) + 4096; }
+ * (double sum, double sum2, float line, max, min, window, 4 dB lines) and then
+ * the frame groups' partials */
+static size_t planes_head(size_t n) {
+    return ((n * (2 * sizeof(double) + 8 * sizeof(float))) + 255) & ~(size_t)255;
+}
+static size_t planes_bytes(size_t n) { return planes_head(n) + atkgpu_spec_part_entries(n) * sizeof(double); }
 
 atkdsp_fft *atkdsp_fft_create(size_t n) {
     if (n < 2) return NULL;
@@ -61,6 +65,7 @@ size_t atkdsp_fft_length(const atkdsp_fft *p) { return p ? p->n : 0; }
 typedef struct {
     float *line, *mx, *mn, *win, *db0, *db1, *db2, *db3;
     double *su, *s2;
+    void *part;
 } planes;
 
 static planes slot_planes(const atk_slot *s, size_t n) {
@@ -77,6 +82,7 @@ static planes slot_planes(const atk_slot *s, size_t n) {
     q.db1 = f + 5 * n;
     q.db2 = f + 6 * n;
     q.db3 = f + 7 * n;
+    q.part = (char *)s->d_c + planes_head(n);
     return q;
 }
 
@@ -132,8 +138,9 @@ static int frames_run(atkdsp_fft *p, atk_slot *s, const atkdsp_cf32 *in, size_t 
         rc = atk_put(s->stream, s->d_a, in + (size_t)f0 * hop, span * sizeof(atkdsp_cf32));
         if (!rc) rc = atkgpu_fft_batch(s->stream, p->plan, s->d_a, hop, d_win, nf, 0, xbuf(p, s), scrbuf(p, s));
         if (!rc) {
-            if (stats) rc = atkgpu_spec_stats_accum(s->stream, xbuf(p, s), n, nf, f0 == 0, q.mx, q.mn, q.su, q.s2);
-            else rc = atkgpu_spec_accum(s->stream, xbuf(p, s), n, nf, detector, f0 == 0, q.line);
+            if (stats) rc = atkgpu_spec_stats_accum(s->stream, xbuf(p, s), n, nf, f0 == 0, q.mx, q.mn, q.su, q.s2,
+                                                    q.part);
+            else rc = atkgpu_spec_accum(s->stream, xbuf(p, s), n, nf, detector, f0 == 0, q.line, q.part);
         }
         /* the next chunk's copy reuses the input buffer: the stream orders it */
     }
