@@ -39,11 +39,14 @@ struct PfbShape {
     // hops a subset folds: at least enough that every FFT team has a hop
     static constexpr int GP = SUB * GP0 >= TEAMS ? GP0 : TEAMS / SUB;
     static constexpr int G = SUB * GP;                       // hops per group
-    // an LDS row holds one hop: the FFT's padded layout (lpad) during the
-    // transform, natural bin order after it; the odd stride spreads the
-    // store phase's reads of several hops over distinct banks
-    static constexpr int ROW = lpad_size<M>() | 1;
     static constexpr int LPC = G >= 64 ? 64 : G;             // store lanes per channel
+    // an LDS row holds one hop: the FFT's padded layout (lpad) during the
+    // transform, natural bin order after it. In the store phase half a
+    // wavefront reads LPC' = min(LPC, 32) hops x 32/LPC' channels, 8-byte
+    // elements at p*ROW + k; a row stride of 32/LPC' (mod 32) elements puts
+    // those 32 reads on 32 distinct bank pairs
+    static constexpr int RMOD = 32 / (LPC < 32 ? LPC : 32);
+    static constexpr int ROW = lpad_size<M>() + ((RMOD - lpad_size<M>() % 32) % 32 + 32) % 32;
     static_assert(NT <= 256 && G % TEAMS == 0, "shape");
 };
 
