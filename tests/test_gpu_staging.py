@@ -96,3 +96,19 @@ def test_threads_share_the_staging_engines(gpu):
                 assert a == b
             else:
                 assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32)), t
+
+
+def test_filterbank_rows_longer_than_a_piece(gpu):
+    """Four channels, hop 4: one slab's output row holds over a million hops
+    (8 MiB, more than a staging piece), so the copy back cuts each row into
+    pieces and lands them in the caller's rows of the whole call. It equals
+    the same input run in blocks small enough to bypass staging."""
+    n = 5_000_003
+    rng = np.random.default_rng(9)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64) * np.float32(0.3)
+    whole = gpu.Pfb(1e6, 4, 1, 4, 60.0).process(x)
+    p = gpu.Pfb(1e6, 4, 1, 4, 60.0)
+    blk = 100_003                            # 0.8 MB in, 0.8 MB out: not staged
+    y = np.concatenate([p.process(x[a:a + blk]) for a in range(0, n, blk)], axis=1)
+    assert whole.shape == y.shape == (4, n // 4)
+    assert np.array_equal(whole.view(np.uint32), y.view(np.uint32))
