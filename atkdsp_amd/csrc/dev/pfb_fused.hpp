@@ -81,6 +81,74 @@ struct PfbFusedArgs {
     int nblocks;           // real blocks (grid may be padded to a multiple of 8)
 };
 
+// One column pass of the fold: column u of hops first .. first+GP-1, whose
+// window rows start at sample base. Taps come in NCH chunks, each over a
+// register window of GP+TH-1 rows; every loop has a constant trip count and
+// no condition, so all register indices are compile-time. Per hop the taps
+// still run in ascending order (k ascending for each branch offset r).
+//   CLAMP false: every row read lies inside the valid samples. Rows and taps
+//   are read as buffer loads: the thread's one 32-bit byte offset in a VGPR,
+//   the row's constant offset in an SGPR, so a load costs no vector
+//   arithmetic (a clamped global load costs four instructions of address).
+//   CLAMP true (the groups at an element's end): rows past the valid samples
+//   only reach hops >= m (not stored) or zero padding taps, so the index is
+//   clamped to the last valid sample, not the value.
+template <int M, int OS, int KOS, bool CLAMP, class S = PfbShape<M, OS>>
+__device__ __forceinline__ void pfb_fold_pass(float (&ar)[S::GP][OS], float (&ai)[S::GP][OS], const cpx *s,
+                                              const float *taps, int base, int nlast, int u) {
+    constexpr int HOP = S::HOP, GP = S::GP;
+    constexpr int NCH = KOS >= 24 ? 4 : 2;
+    constexpr int TH = KOS / NCH;
+    // buffer descriptors (base, stride 0, size in bytes, the gfx9 format word)
+    const __amdgpu_buffer_rsrc_t xr =
+        __builtin_amdgcn_make_buffer_rsrc((void *)s, (short)0, (int)((unsigned)(nlast + 1) * 8u), 0x00020000);
+    const __amdgpu_buffer_rsrc_t hr =
+        __builtin_amdgcn_make_buffer_rsrc((void *)taps, (short)0, (int)(KOS * HOP * 4), 0x00020000);
+    const int xoff = base * 8, hoff = u * 4;
+#pragma unroll
+    for (int p = 0; p < GP; ++p)
+#pragma unroll
+        for (int r = 0; r < OS; ++r) { ar[p][r] = 0.f; ai[p][r] = 0.f; }
+    if constexpr (!CLAMP) {
+        // the whole window in flight at once: KOS taps, then KOS+GP-1 rows
+        float h[KOS];
+        cpx x[KOS + GP - 1];
+#pragma unroll
+        for (int t = 0; t < KOS; ++t)
+            h[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(hr, hoff, t * HOP * 4, 0));
+#pragma unroll
+        for (int j = 0; j < KOS + GP - 1; ++j)
+            x[j] = __builtin_bit_cast(cpx, __builtin_amdgcn_raw_buffer_load_b64(xr, xoff, j * HOP * 8, 0));
+#pragma unroll
+        for (int half = 0; half < NCH; ++half)
+#pragma unroll
+            for (int p = 0; p < GP; ++p)
+#pragma unroll
+                for (int t = half * TH; t < (half + 1) * TH; ++t) {
+                    ar[p][t % OS] = fmaf(h[t], x[p + t].x, ar[p][t % OS]);
+                    ai[p][t % OS] = fmaf(h[t], x[p + t].y, ai[p][t % OS]);
+                }
+    } else {
+#pragma unroll
+        for (int half = 0; half < NCH; ++half) {
+            const int t0 = half * TH;
+            cpx x[GP + TH - 1];
+            float h[TH];
+#pragma unroll
+            for (int jj = 0; jj < GP + TH - 1; ++jj) x[jj] = s[min(base + (t0 + jj) * HOP, nlast)];
+#pragma unroll
+            for (int t = 0; t < TH; ++t) h[t] = taps[(t0 + t) * HOP + u];
+#pragma unroll
+            for (int p = 0; p < GP; ++p)
+#pragma unroll
+                for (int t = 0; t < TH; ++t) {
+                    ar[p][(t0 + t) % OS] = fmaf(h[t], x[p + t].x, ar[p][(t0 + t) % OS]);
+                    ai[p][(t0 + t) % OS] = fmaf(h[t], x[p + t].y, ai[p][(t0 + t) % OS]);
+                }
+        }
+    }
+}
+
 template <int M, int OS, int KOS>
 #ifndef ATKGPU_PFB_WAVES
 #define ATKGPU_PFB_WAVES 2
@@ -149,40 +217,17 @@ __global__ void __launch_bounds__(256, ATKGPU_PFB_WAVES) k_pfb_fused(PfbFusedArg
         const int i0 = (tile + grp * ntile) * G;
         if (i0 >= m) break;
         // ---- fold ----
+        // the group reads rows up to (i0 + G + KOS - 1)*HOP - 1 (same for every block thread)
+        const bool inside = (long long)(i0 + G + KOS - 1) * HOP <= (long long)nlast + 1;
 #pragma unroll 1
         for (int c = 0; c < S::COLS; ++c) {
             const int u = ucol + c * 256;
             float ar[GP][OS], ai[GP][OS];
-#pragma unroll
-            for (int p = 0; p < GP; ++p)
-#pragma unroll
-                for (int r = 0; r < OS; ++r) { ar[p][r] = 0.f; ai[p][r] = 0.f; }
-            // rows past the valid samples only reach hops >= m (not stored) or
-            // zero padding taps, so the address is clamped, not the value
             const int base = (i0 + sub * GP) * HOP + u;
-            // taps in NCH chunks, each over a register window of GP+TH-1 rows;
-            // every loop has a constant trip count and no condition, so all
-            // register indices are compile-time. Per hop the taps still run
-            // in ascending order (k ascending for each branch offset r).
-            constexpr int NCH = KOS >= 24 ? 4 : 2;
-            constexpr int TH = KOS / NCH;
-#pragma unroll
-            for (int half = 0; half < NCH; ++half) {
-                const int t0 = half * TH;
-                cpx x[GP + TH - 1];
-                float h[TH];
-#pragma unroll
-                for (int jj = 0; jj < GP + TH - 1; ++jj) x[jj] = s[min(base + (t0 + jj) * HOP, nlast)];
-#pragma unroll
-                for (int t = 0; t < TH; ++t) h[t] = taps[(t0 + t) * HOP + u];
-#pragma unroll
-                for (int p = 0; p < GP; ++p)
-#pragma unroll
-                    for (int t = 0; t < TH; ++t) {
-                        ar[p][(t0 + t) % OS] = fmaf(h[t], x[p + t].x, ar[p][(t0 + t) % OS]);
-                        ai[p][(t0 + t) % OS] = fmaf(h[t], x[p + t].y, ai[p][(t0 + t) % OS]);
-                    }
-            }
+            if (inside)
+                pfb_fold_pass<M, OS, KOS, false>(ar, ai, s, taps, base, nlast, u);
+            else
+                pfb_fold_pass<M, OS, KOS, true>(ar, ai, s, taps, base, nlast, u);
 #pragma unroll
             for (int p = 0; p < GP; ++p)
 #pragma unroll
