@@ -81,21 +81,31 @@ struct PfbFusedArgs {
     int nblocks;           // real blocks (grid may be padded to a multiple of 8)
 };
 
+// The taps a thread folds with: KOS per column, the same for every group.
+// Where a thread's columns need at most 64 of them they stay in registers for
+// the whole block (HREG); otherwise each pass loads its own.
+template <int M, int OS, int KOS, class S = PfbShape<M, OS>>
+struct PfbTaps {
+    static constexpr bool HREG = S::COLS * KOS <= 64;
+    float h[HREG ? S::COLS : 1][HREG ? KOS : 1];
+};
+
 // One column pass of the fold: column u of hops first .. first+GP-1, whose
-// window rows start at sample base. Taps come in NCH chunks, each over a
-// register window of GP+TH-1 rows; every loop has a constant trip count and
-// no condition, so all register indices are compile-time. Per hop the taps
-// still run in ascending order (k ascending for each branch offset r).
-//   CLAMP false: every row read lies inside the valid samples. Rows and taps
-//   are read as buffer loads: the thread's one 32-bit byte offset in a VGPR,
-//   the row's constant offset in an SGPR, so a load costs no vector
-//   arithmetic (a clamped global load costs four instructions of address).
+// window rows start at sample base. Per hop the taps run in ascending order
+// (k ascending for each branch offset r). Rows and taps are buffer loads: the
+// thread's one 32-bit byte offset in a VGPR, a row's constant offset in an
+// SGPR, so a load costs no vector arithmetic (a clamped 64-bit global address
+// costs four instructions).
+//   CLAMP false: every row read lies inside the valid samples, and the whole
+//   window is put in flight at once (KOS+GP-1 rows, and the taps unless they
+//   are held in hk).
 //   CLAMP true (the groups at an element's end): rows past the valid samples
-//   only reach hops >= m (not stored) or zero padding taps, so the index is
-//   clamped to the last valid sample, not the value.
-template <int M, int OS, int KOS, bool CLAMP, class S = PfbShape<M, OS>>
+//   only reach hops >= m (not stored) or zero padding taps, so the row index
+//   is clamped to the last valid sample, not the value; the window comes in
+//   NCH chunks of TH taps.
+template <int M, int OS, int KOS, bool CLAMP, bool HREG, class S = PfbShape<M, OS>>
 __device__ __forceinline__ void pfb_fold_pass(float (&ar)[S::GP][OS], float (&ai)[S::GP][OS], const cpx *s,
-                                              const float *taps, int base, int nlast, int u) {
+                                              const float *taps, const float *hk, int base, int nlast, int u) {
     constexpr int HOP = S::HOP, GP = S::GP;
     constexpr int NCH = KOS >= 24 ? 4 : 2;
     constexpr int TH = KOS / NCH;
@@ -105,17 +115,21 @@ __device__ __forceinline__ void pfb_fold_pass(float (&ar)[S::GP][OS], float (&ai
     const __amdgpu_buffer_rsrc_t hr =
         __builtin_amdgcn_make_buffer_rsrc((void *)taps, (short)0, (int)(KOS * HOP * 4), 0x00020000);
     const int xoff = base * 8, hoff = u * 4;
+    auto tap = [&](int t) -> float {
+        if constexpr (HREG)
+            return hk[t];
+        else
+            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(hr, hoff, t * HOP * 4, 0));
+    };
 #pragma unroll
     for (int p = 0; p < GP; ++p)
 #pragma unroll
         for (int r = 0; r < OS; ++r) { ar[p][r] = 0.f; ai[p][r] = 0.f; }
     if constexpr (!CLAMP) {
-        // the whole window in flight at once: KOS taps, then KOS+GP-1 rows
         float h[KOS];
         cpx x[KOS + GP - 1];
 #pragma unroll
-        for (int t = 0; t < KOS; ++t)
-            h[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(hr, hoff, t * HOP * 4, 0));
+        for (int t = 0; t < KOS; ++t) h[t] = tap(t);
 #pragma unroll
         for (int j = 0; j < KOS + GP - 1; ++j)
             x[j] = __builtin_bit_cast(cpx, __builtin_amdgcn_raw_buffer_load_b64(xr, xoff, j * HOP * 8, 0));
@@ -135,9 +149,11 @@ __device__ __forceinline__ void pfb_fold_pass(float (&ar)[S::GP][OS], float (&ai
             cpx x[GP + TH - 1];
             float h[TH];
 #pragma unroll
-            for (int jj = 0; jj < GP + TH - 1; ++jj) x[jj] = s[min(base + (t0 + jj) * HOP, nlast)];
+            for (int jj = 0; jj < GP + TH - 1; ++jj)
+                x[jj] = __builtin_bit_cast(
+                    cpx, __builtin_amdgcn_raw_buffer_load_b64(xr, min(base + (t0 + jj) * HOP, nlast) * 8, 0, 0));
 #pragma unroll
-            for (int t = 0; t < TH; ++t) h[t] = taps[(t0 + t) * HOP + u];
+            for (int t = 0; t < TH; ++t) h[t] = tap(t0 + t);
 #pragma unroll
             for (int p = 0; p < GP; ++p)
 #pragma unroll
@@ -150,10 +166,7 @@ __device__ __forceinline__ void pfb_fold_pass(float (&ar)[S::GP][OS], float (&ai
 }
 
 template <int M, int OS, int KOS>
-#ifndef ATKGPU_PFB_WAVES
-#define ATKGPU_PFB_WAVES 2
-#endif
-__global__ void __launch_bounds__(256, ATKGPU_PFB_WAVES) k_pfb_fused(PfbFusedArgs a) {
+__global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
     using S = PfbShape<M, OS>;
     constexpr int HOP = S::HOP, G = S::G, GP = S::GP, ROW = S::ROW;
     __shared__ cpx lds[G * ROW];
@@ -206,8 +219,23 @@ __global__ void __launch_bounds__(256, ATKGPU_PFB_WAVES) k_pfb_fused(PfbFusedArg
     const cpx *cal = (a.cal && e < a.cal_nel) ? a.cal + (long long)e * M : nullptr;
     const bool flip_base = (b0 & 1) != 0;     // parity of the hop counter at hop 0
 
-    // The loops over column passes and FFT rounds stay rolled: the unrolled
-    // fold is long, and the kernel must fit the instruction cache.
+    // The taps of the thread's columns, where they fit in registers
+    using TP = PfbTaps<M, OS, KOS>;
+    TP tk;
+    if constexpr (TP::HREG) {
+        const __amdgpu_buffer_rsrc_t hr =
+            __builtin_amdgcn_make_buffer_rsrc((void *)taps, (short)0, (int)(KOS * HOP * 4), 0x00020000);
+#pragma unroll
+        for (int c = 0; c < S::COLS; ++c)
+#pragma unroll
+            for (int t = 0; t < KOS; ++t)
+                tk.h[c][t] = __builtin_bit_cast(
+                    float, __builtin_amdgcn_raw_buffer_load_b32(hr, (ucol + c * 256) * 4, t * HOP * 4, 0));
+    }
+
+    // The FFT rounds stay rolled, and so do the column passes where the taps
+    // are loaded per pass: the unrolled fold is long, and the kernel must fit
+    // the instruction cache.
     // Groups are dealt to an element's blocks in turn (group tile, then
     // tile + ntile, ...): blocks resident together on an XCD fold adjacent
     // groups, whose windows share KOS-1 rows, so those rows are read from HBM
@@ -219,20 +247,26 @@ __global__ void __launch_bounds__(256, ATKGPU_PFB_WAVES) k_pfb_fused(PfbFusedArg
         // ---- fold ----
         // the group reads rows up to (i0 + G + KOS - 1)*HOP - 1 (same for every block thread)
         const bool inside = (long long)(i0 + G + KOS - 1) * HOP <= (long long)nlast + 1;
-#pragma unroll 1
-        for (int c = 0; c < S::COLS; ++c) {
+        auto pass = [&](int c, const float *hk) {
             const int u = ucol + c * 256;
             float ar[GP][OS], ai[GP][OS];
             const int base = (i0 + sub * GP) * HOP + u;
             if (inside)
-                pfb_fold_pass<M, OS, KOS, false>(ar, ai, s, taps, base, nlast, u);
+                pfb_fold_pass<M, OS, KOS, false, TP::HREG>(ar, ai, s, taps, hk, base, nlast, u);
             else
-                pfb_fold_pass<M, OS, KOS, true>(ar, ai, s, taps, base, nlast, u);
+                pfb_fold_pass<M, OS, KOS, true, TP::HREG>(ar, ai, s, taps, hk, base, nlast, u);
 #pragma unroll
             for (int p = 0; p < GP; ++p)
 #pragma unroll
                 for (int r = 0; r < OS; ++r)
                     lds[(sub * GP + p) * ROW + lpad(r * HOP + u)] = cmake(ar[p][r], ai[p][r]);
+        };
+        if constexpr (TP::HREG) {
+#pragma unroll
+            for (int c = 0; c < S::COLS; ++c) pass(c, tk.h[c]);
+        } else {
+#pragma unroll 1
+            for (int c = 0; c < S::COLS; ++c) pass(c, nullptr);
         }
         __syncthreads();
         // ---- FFT, rotation, cal: team tm transforms hops tm, tm+TEAMS, ... ----
