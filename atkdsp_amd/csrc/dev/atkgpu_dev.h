@@ -174,6 +174,15 @@ int atkgpu_pfb_fused(void *s, const void *sbuf, size_t spitch, size_t nvalid, un
                      size_t m, long long b0, unsigned M, unsigned os, unsigned kos,
                      const float *taps, const void *tw, const void *cal, unsigned cal_nel,
                      void *out, size_t opitch, size_t ocol);
+/* The same kernel over raw rows (element e at raw + e*spitch samples of the
+ * format's size): each sample is converted as it is loaded, bit for bit the
+ * unpack with the fixed offset (off_re, off_im); window samples before zlo
+ * are the stream's zero history. For M 64 or 128, os 1 or 2, kos 8/16/32 and
+ * the cu8, ci8, ci16 and ci16q11 formats (atkgpu_pfb_fused_raw_supported). */
+int atkgpu_pfb_fused_raw_supported(unsigned M, unsigned os, unsigned kos, int fmt);
+int atkgpu_pfb_fused_raw(void *s, const void *raw, size_t spitch, size_t nvalid, unsigned nel, size_t m,
+                         long long b0, unsigned M, unsigned os, unsigned kos, int fmt, float off_re,
+                         float off_im, int zlo, const float *taps, const void *tw, void *out, size_t opitch);
 /* ---- chain kernels (chain.hip): S stream rows spitch samples apart ---- */
 /* raw (stream s at raw + s*raw_pitch bytes) -> cf32 at row0 + s*spitch, with the
  * per-stream offset off[2s..2s+1] (NULL: none); mean[2s..] = block mean before

@@ -16,6 +16,12 @@
 //
 // Taps are zero-padded from K*OS to KOS (an instantiated size): a zero tap
 // adds an exact zero, so results equal the unpadded sum for finite input.
+//
+// Input: cf32 rows (IN = PFB_IN_CF32), or raw receiver rows (IN = FMT_CU8,
+// FMT_CI8, FMT_CI16) converted as they are loaded, ((float)b - c') * k with
+// c' = c0 + off/k, the unpack's arithmetic (src/convert.c:33-111), so a raw
+// row gives the same fold inputs bit for bit as its unpacked cf32 row; window
+// samples before zlo (the stream's zero history) are zero.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "dev_common.hpp"
@@ -62,10 +68,12 @@ struct PfbJob {
     int pad_;
 };
 
+enum { PFB_IN_CF32 = -1 };
+
 struct PfbFusedArgs {
     const PfbJob *jobs;    // non-null: a ragged batch, one descriptor per block
     int njobs;
-    const cpx *s;          // stream rows, element e at s + e*spitch
+    const cpx *s;          // stream rows, element e at s + e*spitch (raw input: the raw rows, spitch samples)
     long long spitch;
     long long nvalid;      // valid samples in each element's stream
     long long m;           // hops to produce
@@ -79,7 +87,46 @@ struct PfbFusedArgs {
     int groups_per_block;
     int nblk_per_el;       // blocks per element
     int nblocks;           // real blocks (grid may be padded to a multiple of 8)
+    float c0, kk, off_re, off_im;   // raw input: the format's constants and the fixed offset
+    int zlo;               // raw input: window samples before the stream's first (zero history)
 };
+
+// Bytes a sample of input IN takes.
+template <int IN>
+__host__ __device__ constexpr int pfb_in_bps() {
+    return IN == PFB_IN_CF32 ? 8 : (IN == FMT_CU8 || IN == FMT_CI8) ? 2 : 4;
+}
+
+// One loaded raw sample word to the cf32 value the unpack makes of it.
+template <int IN>
+__device__ __forceinline__ cpx pfb_raw_conv(unsigned w, float cre, float cim, float kk) {
+#pragma clang fp contract(off)
+    float a, b;
+    if constexpr (IN == FMT_CU8) {
+        a = (float)(w & 0xffu);
+        b = (float)((w >> 8) & 0xffu);
+    } else if constexpr (IN == FMT_CI8) {
+        a = (float)(signed char)(w & 0xffu);
+        b = (float)(signed char)((w >> 8) & 0xffu);
+    } else {
+        a = (float)(short)(w & 0xffffu);
+        b = (float)(short)(w >> 16);
+    }
+    return cmake((a - cre) * kk, (b - cim) * kk);
+}
+
+// A sample of input IN at byte offset voff + soff of buffer r.
+template <int IN>
+__device__ __forceinline__ cpx pfb_load(__amdgpu_buffer_rsrc_t r, int voff, int soff, float cre, float cim,
+                                        float kk) {
+    if constexpr (IN == PFB_IN_CF32) {
+        return __builtin_bit_cast(cpx, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    } else if constexpr (pfb_in_bps<IN>() == 2) {
+        return pfb_raw_conv<IN>((unsigned)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0), cre, cim, kk);
+    } else {
+        return pfb_raw_conv<IN>((unsigned)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0), cre, cim, kk);
+    }
+}
 
 // The taps a thread folds with: KOS per column, the same for every group.
 // Where a thread's columns need at most 64 of them they stay in registers for
@@ -103,18 +150,23 @@ struct PfbTaps {
 //   only reach hops >= m (not stored) or zero padding taps, so the row index
 //   is clamped to the last valid sample, not the value; the window comes in
 //   NCH chunks of TH taps.
-template <int M, int OS, int KOS, bool CLAMP, bool HREG, class S = PfbShape<M, OS>>
-__device__ __forceinline__ void pfb_fold_pass(float (&ar)[S::GP][OS], float (&ai)[S::GP][OS], const cpx *s,
-                                              const float *taps, const float *hk, int base, int nlast, int u) {
+//   Raw input (IN != PFB_IN_CF32): the CLAMP path also zeroes the rows
+//   before zlo, the stream's zero history.
+template <int M, int OS, int KOS, bool CLAMP, bool HREG, int IN = PFB_IN_CF32, class S = PfbShape<M, OS>>
+__device__ __forceinline__ void pfb_fold_pass(float (&ar)[S::GP][OS], float (&ai)[S::GP][OS], const void *s,
+                                              const float *taps, const float *hk, int base, int nlast, int u,
+                                              float cre = 0.f, float cim = 0.f, float kk = 1.f, int zlo = 0) {
     constexpr int HOP = S::HOP, GP = S::GP;
     constexpr int NCH = KOS >= 24 ? 4 : 2;
     constexpr int TH = KOS / NCH;
+    constexpr int BPS = pfb_in_bps<IN>();
     // buffer descriptors (base, stride 0, size in bytes, the gfx9 format word)
     const __amdgpu_buffer_rsrc_t xr =
-        __builtin_amdgcn_make_buffer_rsrc((void *)s, (short)0, (int)((unsigned)(nlast + 1) * 8u), 0x00020000);
+        __builtin_amdgcn_make_buffer_rsrc((void *)s, (short)0, (int)((unsigned)(nlast + 1) * (unsigned)BPS),
+                                          0x00020000);
     const __amdgpu_buffer_rsrc_t hr =
         __builtin_amdgcn_make_buffer_rsrc((void *)taps, (short)0, (int)(KOS * HOP * 4), 0x00020000);
-    const int xoff = base * 8, hoff = u * 4;
+    const int xoff = base * BPS, hoff = u * 4;
     auto tap = [&](int t) -> float {
         if constexpr (HREG)
             return hk[t];
@@ -131,8 +183,7 @@ __device__ __forceinline__ void pfb_fold_pass(float (&ar)[S::GP][OS], float (&ai
 #pragma unroll
         for (int t = 0; t < KOS; ++t) h[t] = tap(t);
 #pragma unroll
-        for (int j = 0; j < KOS + GP - 1; ++j)
-            x[j] = __builtin_bit_cast(cpx, __builtin_amdgcn_raw_buffer_load_b64(xr, xoff, j * HOP * 8, 0));
+        for (int j = 0; j < KOS + GP - 1; ++j) x[j] = pfb_load<IN>(xr, xoff, j * HOP * BPS, cre, cim, kk);
 #pragma unroll
         for (int half = 0; half < NCH; ++half)
 #pragma unroll
@@ -149,9 +200,12 @@ __device__ __forceinline__ void pfb_fold_pass(float (&ar)[S::GP][OS], float (&ai
             cpx x[GP + TH - 1];
             float h[TH];
 #pragma unroll
-            for (int jj = 0; jj < GP + TH - 1; ++jj)
-                x[jj] = __builtin_bit_cast(
-                    cpx, __builtin_amdgcn_raw_buffer_load_b64(xr, min(base + (t0 + jj) * HOP, nlast) * 8, 0, 0));
+            for (int jj = 0; jj < GP + TH - 1; ++jj) {
+                const int row = base + (t0 + jj) * HOP;
+                x[jj] = pfb_load<IN>(xr, min(row, nlast) * BPS, 0, cre, cim, kk);
+                if constexpr (IN != PFB_IN_CF32)
+                    if (row < zlo) x[jj] = cmake(0.f, 0.f);
+            }
 #pragma unroll
             for (int t = 0; t < TH; ++t) h[t] = tap(t0 + t);
 #pragma unroll
@@ -165,7 +219,7 @@ __device__ __forceinline__ void pfb_fold_pass(float (&ar)[S::GP][OS], float (&ai
     }
 }
 
-template <int M, int OS, int KOS>
+template <int M, int OS, int KOS, int IN = PFB_IN_CF32>
 __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
     using S = PfbShape<M, OS>;
     constexpr int HOP = S::HOP, G = S::G, GP = S::GP, ROW = S::ROW;
@@ -178,7 +232,7 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
     const int lb = (int)(blockIdx.x % 8) * nb8 + (int)(blockIdx.x / 8);
     if (lb >= a.nblocks) return;
     int e, tile, nlast, m, opitch, gpb, ntile;
-    const cpx *s;
+    const void *s;
     cpx *oute;
     const float *taps;
     long long b0;
@@ -202,7 +256,7 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
         e = lb / a.nblk_per_el;
         tile = lb % a.nblk_per_el;
         // per-element bases in 64-bit once; every offset below fits 32 bits
-        s = a.s + (long long)e * a.spitch;
+        s = (const unsigned char *)a.s + (long long)e * a.spitch * pfb_in_bps<IN>();
         oute = a.out + (long long)e * M * a.opitch + a.ocol;
         taps = a.taps;
         b0 = a.b0;
@@ -218,6 +272,15 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
     for (int i = tid; i < M; i += 256) twl[i] = a.tw[i];
     const cpx *cal = (a.cal && e < a.cal_nel) ? a.cal + (long long)e * M : nullptr;
     const bool flip_base = (b0 & 1) != 0;     // parity of the hop counter at hop 0
+    // raw input: the folded constants as the unpack forms them (src/convert.c:47, :57, :68)
+    float cre = 0.f, cim = 0.f, kk = 1.f;
+    int zlo = 0;
+    if constexpr (IN != PFB_IN_CF32) {
+        kk = a.kk;
+        cre = a.c0 + a.off_re / a.kk;
+        cim = a.c0 + a.off_im / a.kk;
+        zlo = a.zlo;
+    }
 
     // The taps of the thread's columns, where they fit in registers
     using TP = PfbTaps<M, OS, KOS>;
@@ -246,15 +309,17 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
         if (i0 >= m) break;
         // ---- fold ----
         // the group reads rows up to (i0 + G + KOS - 1)*HOP - 1 (same for every block thread)
-        const bool inside = (long long)(i0 + G + KOS - 1) * HOP <= (long long)nlast + 1;
+        const bool inside = (long long)(i0 + G + KOS - 1) * HOP <= (long long)nlast + 1 && i0 * HOP >= zlo;
         auto pass = [&](int c, const float *hk) {
             const int u = ucol + c * 256;
             float ar[GP][OS], ai[GP][OS];
             const int base = (i0 + sub * GP) * HOP + u;
             if (inside)
-                pfb_fold_pass<M, OS, KOS, false, TP::HREG>(ar, ai, s, taps, hk, base, nlast, u);
+                pfb_fold_pass<M, OS, KOS, false, TP::HREG, IN>(ar, ai, s, taps, hk, base, nlast, u, cre, cim, kk,
+                                                               zlo);
             else
-                pfb_fold_pass<M, OS, KOS, true, TP::HREG>(ar, ai, s, taps, hk, base, nlast, u);
+                pfb_fold_pass<M, OS, KOS, true, TP::HREG, IN>(ar, ai, s, taps, hk, base, nlast, u, cre, cim, kk,
+                                                              zlo);
 #pragma unroll
             for (int p = 0; p < GP; ++p)
 #pragma unroll

@@ -8,6 +8,9 @@
  *   MX forms    raw block straight into raw rows (the kept raw history sits
  *               in front of it) -> the MFMA filterbank -> the kept raw
  *               prefix to the other row. No cf32 copy of the input exists.
+ *   FP32 raw    the MX forms' raw rows -> the fused FP32 filterbank, which
+ *               converts each sample as it loads it (bit-identical to the
+ *               FP32 form) -> the kept raw prefix to the other row.
  *
  * Streaming state is the reference's (src/channelizer.c:305-408): a row holds
  * keep = ov + hop samples of history in front of each new block; hop i's
@@ -24,12 +27,12 @@ int atkgpu_pfb_prototype(double *h, unsigned nchan, unsigned K, double atten_db)
 struct atkgpu_pfbx {
     atkgpu_pfbx_cfg cfg;
     unsigned S, M, os, K, hop, ov;
-    int kos, bps, mx;
+    int kos, bps, mx, rawrows;      /* rawrows: raw rows (MX forms, FP32 raw), else cf32 rows */
     size_t keep, spitch, m_cap;
     void *stream;
     void *ev[5];
     unsigned char *d_raw;      /* FP32 form: [S][block_max*bps] upload blocks */
-    unsigned char *d_rows[2];  /* FP32 form: cf32 rows; MX forms: raw rows; [S][spitch] samples */
+    unsigned char *d_rows[2];  /* FP32 form: cf32 rows; MX and FP32 raw forms: raw rows; [S][spitch] samples */
     int cur;
     float *d_taps, *d_tapsum;
     void *d_atab, *d_offs;
@@ -89,7 +92,7 @@ int atkgpu_pfbx_reset(atkgpu_pfbx *v) {
      * encode cu8's zero (127.5 counts): the matrix-core kernel is told how
      * many window samples precede the stream's start and treats them as 0 */
     int rc = 0;
-    const size_t row_bytes = (size_t)v->S * v->spitch * (v->mx ? (size_t)v->bps : sizeof(atkdsp_cf32));
+    const size_t row_bytes = (size_t)v->S * v->spitch * (v->rawrows ? (size_t)v->bps : sizeof(atkdsp_cf32));
     for (int i = 0; i < 2 && !rc; ++i) rc = atkgpu_fill_u32(v->stream, v->d_rows[i], 0u, row_bytes / 4);
     if (!rc && v->cfg.coherent) rc = atkgpu_pfb_cov_reset(v->stream, v->d_R, v->d_cnt, v->S, v->M, NULL);
     if (!rc) rc = atkgpu_sync(v->stream);
@@ -100,9 +103,14 @@ atkgpu_pfbx *atkgpu_pfbx_create(const atkgpu_pfbx_cfg *cfg, unsigned S) {
     if (!cfg || S < 1 || !(cfg->fs > 0) || cfg->block_max < 8 || (cfg->block_max & 7)) return NULL;
     const int bps = atkdsp_bytes_per_sample(cfg->fmt);
     const int kos = atkgpu_pfb_fused_kos(cfg->nchan, cfg->oversample, cfg->taps_per_poly);
-    if (!bps || !kos || cfg->taps_per_poly < 2 || cfg->form < ATKGPU_FORM_FP32 || cfg->form > ATKGPU_FORM_FP32MM)
+    if (!bps || !kos || cfg->taps_per_poly < 2 || cfg->form < ATKGPU_FORM_FP32 || cfg->form > ATKGPU_FORM_FP32RAW)
         return NULL;
-    const int mx = cfg->form != ATKGPU_FORM_FP32 && cfg->form != ATKGPU_FORM_FP32MM;
+    const int mx = cfg->form != ATKGPU_FORM_FP32 && cfg->form != ATKGPU_FORM_FP32MM &&
+                   cfg->form != ATKGPU_FORM_FP32RAW;
+    const int rawrows = mx || cfg->form == ATKGPU_FORM_FP32RAW;
+    if (cfg->form == ATKGPU_FORM_FP32RAW &&
+        !atkgpu_pfb_fused_raw_supported(cfg->nchan, cfg->oversample, (unsigned)kos, cfg->fmt))
+        return NULL;
     if (cfg->form == ATKGPU_FORM_FP32MM && !atkgpu_pfb_mm_k(cfg->nchan, cfg->oversample, (unsigned)kos)) return NULL;
     if (mx && !atkgpu_pfb_mx_supported(cfg->nchan, cfg->oversample, (unsigned)kos, cfg->fmt, cfg->form)) return NULL;
     if (atkgpu_init() != 0) return NULL;
@@ -118,6 +126,7 @@ atkgpu_pfbx *atkgpu_pfbx_create(const atkgpu_pfbx_cfg *cfg, unsigned S) {
     v->kos = kos;
     v->bps = bps;
     v->mx = mx;
+    v->rawrows = rawrows;
     v->keep = (size_t)v->ov + v->hop;                  /* M*K: a multiple of 64 */
     v->spitch = v->keep + cfg->block_max;
     v->m_cap = round16((cfg->block_max + v->hop) / v->hop + 1);
@@ -131,15 +140,15 @@ atkgpu_pfbx *atkgpu_pfbx_create(const atkgpu_pfbx_cfg *cfg, unsigned S) {
         v->d_mean = (float *)atkgpu_dmalloc((size_t)v->M * sizeof(float));
         ok = v->d_R && v->d_cnt && v->d_pwr && v->d_mean;
     }
-    const size_t row_bytes = (size_t)S * v->spitch * (mx ? (size_t)bps : sizeof(atkdsp_cf32));
+    const size_t row_bytes = (size_t)S * v->spitch * (rawrows ? (size_t)bps : sizeof(atkdsp_cf32));
     if (ok) {
         v->d_rows[0] = (unsigned char *)atkgpu_dmalloc(row_bytes);
         v->d_rows[1] = (unsigned char *)atkgpu_dmalloc(row_bytes);
-        if (!mx) v->d_raw = (unsigned char *)atkgpu_dmalloc((size_t)S * cfg->block_max * bps);
+        if (!rawrows) v->d_raw = (unsigned char *)atkgpu_dmalloc((size_t)S * cfg->block_max * bps);
         v->d_taps = (float *)atkgpu_dmalloc(ntap * sizeof(float));
         v->d_out = (atkdsp_cf32 *)atkgpu_dmalloc((size_t)S * v->M * v->m_cap * sizeof(atkdsp_cf32));
         v->plan = atkgpu_fftplan_create(v->M);
-        ok = v->d_rows[0] && v->d_rows[1] && (mx || v->d_raw) && v->d_taps && v->d_out && v->plan;
+        ok = v->d_rows[0] && v->d_rows[1] && (rawrows || v->d_raw) && v->d_taps && v->d_out && v->plan;
     }
     const size_t dbytes = (size_t)S * (sizeof(atkgpu_unpack_job) + sizeof(atkgpu_copy_job)) + 256;
     for (int i = 0; ok && i < 2; ++i) ok = (v->desc_h[i] = atkgpu_hmalloc_mapped(dbytes, &v->desc_d[i])) != NULL;
@@ -204,7 +213,7 @@ atkgpu_pfbx *atkgpu_pfbx_create(const atkgpu_pfbx_cfg *cfg, unsigned S) {
 
 void *atkgpu_pfbx_input(atkgpu_pfbx *v, unsigned s) {
     if (!v || s >= v->S) return NULL;
-    if (!v->mx) return v->d_raw + (size_t)s * v->cfg.block_max * v->bps;
+    if (!v->rawrows) return v->d_raw + (size_t)s * v->cfg.block_max * v->bps;
     return v->d_rows[v->cur] + ((size_t)s * v->spitch + v->keep) * v->bps;
 }
 
@@ -219,11 +228,11 @@ static int build_desc(atkgpu_pfbx *v, int cur, size_t n) {
     if (v->desc_n[cur] && atkgpu_sync(v->stream)) return ATKDSP_E_FFT;    /* the set may still be in use */
     atkgpu_unpack_job *uj = (atkgpu_unpack_job *)v->desc_h[cur];
     atkgpu_copy_job *cj = (atkgpu_copy_job *)(uj + v->S);
-    const size_t es = v->mx ? (size_t)v->bps : sizeof(atkdsp_cf32);
+    const size_t es = v->rawrows ? (size_t)v->bps : sizeof(atkdsp_cf32);
     unsigned char *R = v->d_rows[cur], *T = v->d_rows[cur ^ 1];
     int ublk = 0, cblk = 0;
     for (unsigned s = 0; s < v->S; ++s) {
-        if (!v->mx) {
+        if (!v->rawrows) {
             atkgpu_unpack_job *j = &uj[s];
             j->raw = v->d_raw + (size_t)s * v->cfg.block_max * v->bps;
             j->dst = R + ((size_t)s * v->spitch + v->keep) * es;
@@ -239,7 +248,7 @@ static int build_desc(atkgpu_pfbx *v, int cur, size_t n) {
         c->src = R + ((size_t)s * v->spitch + n) * es;
         c->dst = T + (size_t)s * v->spitch * es;
         c->blk0 = cblk;
-        if (v->mx) {
+        if (v->rawrows) {
             c->n = (int)(v->keep * es);                   /* bytes */
             c->nblk = (c->n + 4095) / 4096;
         } else {
@@ -263,11 +272,11 @@ ptrdiff_t atkgpu_pfbx_run(atkgpu_pfbx *v, size_t n) {
     atkgpu_copy_job *cj = (atkgpu_copy_job *)(uj + v->S);
     char *dbase = (char *)v->desc_d[cur], *hbase = (char *)v->desc_h[cur];
 #define DEV(p) ((void *)(dbase + ((char *)(p) - hbase)))
-    const size_t es = v->mx ? (size_t)v->bps : sizeof(atkdsp_cf32);
+    const size_t es = v->rawrows ? (size_t)v->bps : sizeof(atkdsp_cf32);
     const size_t mp = round16(m);
     unsigned char *R = v->d_rows[cur];
     rc = atkgpu_event_record(v->ev[0], v->stream);
-    if (!rc && n && !v->mx) {
+    if (!rc && n && !v->rawrows) {
         int ublk = 0;
         for (unsigned s = 0; s < v->S; ++s) ublk += uj[s].nblk;
         rc = atkgpu_unpack_jobs(v->stream, DEV(uj), (int)v->S, ublk, 0, 0);
@@ -283,6 +292,10 @@ ptrdiff_t atkgpu_pfbx_run(atkgpu_pfbx *v, size_t n) {
             rc = atkgpu_pfb_mx(v->stream, base, v->spitch, (size_t)v->ov + avail, v->S, m, v->b, v->M, v->os,
                                (unsigned)v->kos, v->cfg.fmt, v->cfg.form, v->d_atab, v->d_tapsum, v->d_taps, zlo,
                                v->scale, v->d_offs, tw, NULL, 0, v->d_out, mp);
+        else if (v->cfg.form == ATKGPU_FORM_FP32RAW)
+            rc = atkgpu_pfb_fused_raw(v->stream, base, v->spitch, (size_t)v->ov + avail, v->S, m, v->b, v->M, v->os,
+                                      (unsigned)v->kos, v->cfg.fmt, v->cfg.off_re, v->cfg.off_im, zlo, v->d_taps,
+                                      tw, v->d_out, mp);
         else if (v->cfg.form == ATKGPU_FORM_FP32MM)
             rc = atkgpu_pfb_mm(v->stream, base, v->spitch, (size_t)v->ov + avail, v->S, m, v->b, v->M, v->os,
                                (unsigned)v->kos, v->d_taps, tw, NULL, 0, v->d_out, mp, 0);
@@ -299,7 +312,7 @@ ptrdiff_t atkgpu_pfbx_run(atkgpu_pfbx *v, size_t n) {
     if (!rc && n) {
         int cblk = 0;
         for (unsigned s = 0; s < v->S; ++s) cblk += cj[s].nblk;
-        rc = v->mx ? atkgpu_copy_bytes_jobs(v->stream, DEV(cj), (int)v->S, cblk)
+        rc = v->rawrows ? atkgpu_copy_bytes_jobs(v->stream, DEV(cj), (int)v->S, cblk)
                    : atkgpu_copy_jobs(v->stream, DEV(cj), (int)v->S, cblk);
     }
     if (!rc) rc = atkgpu_event_record(v->ev[4], v->stream);

@@ -6,9 +6,11 @@
     m = px.run(n)                    # one block of n samples for every stream
     y = px.read(s)                   # [nchan][m] complex64
 
-Forms: "fp32" (unpack, then the fused FP32 filterbank: the form the others
-are measured against), "fp32mm" (the same with the FP32 fold on the matrix
-cores: bit-identical to "fp32" for finite input), "bf16x2" / "bf16" (8-bit samples exact in BF16, taps
+Forms: "fp32" (unpack, then the fused FP32 filterbank), "fp32raw" (the
+fused FP32 filterbank reading the raw rows and converting each sample as it
+loads it: bit-identical to "fp32", and the like-for-like baseline of the
+matrix-core forms, which read the same raw rows), "fp32mm" (the FP32 fold on
+the matrix cores: bit-identical to "fp32" for finite input), "bf16x2" / "bf16" (8-bit samples exact in BF16, taps
 as hi + lo or one BF16 term), "fp16x2" / "fp16" (8- or 12-bit samples exact
 in FP16, taps scaled by a power of two and split likewise). Every form
 accumulates in FP32.
@@ -21,7 +23,7 @@ import numpy as np
 
 from . import AtkDspError, _Cf32, _check, _fmt, load
 
-FORMS = {"fp32": 0, "bf16x2": 1, "bf16": 2, "fp16x2": 3, "fp16": 4, "fp32mm": 5}
+FORMS = {"fp32": 0, "bf16x2": 1, "bf16": 2, "fp16x2": 3, "fp16": 4, "fp32mm": 5, "fp32raw": 6}
 
 
 class _Cfg(C.Structure):

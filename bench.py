@@ -55,13 +55,16 @@ def precision_forms(S: int = 256, n: int = 1_200_000, steps: int = 10):
     """The filterbank's matrix-core forms against the plain FP32 form on the
     C1 shape (2.4 MSPS cu8, 64 channels, 8 taps/branch, critically sampled):
     S streams of n samples each, resident in HBM. The FP32 form is unpack +
-    the fused FP32 filterbank; the matrix-core forms read the raw bytes."""
+    the fused FP32 filterbank; the FP32 raw form is the fused FP32 filterbank
+    reading the raw rows (bit-identical to the FP32 form); the matrix-core
+    forms read the same raw rows. Speedups are stated against both FP32
+    forms, whole run and filterbank kernel alone."""
     from atkdsp_amd import synth
     from atkdsp_amd.pfbx import PfbX
     base = np.frombuffer(synth.c1_bytes(n=n), np.uint8).reshape(-1, 2)
     raws = [np.roll(base, s * 7919, axis=0) for s in range(S)]
     out = {}
-    for form in ["fp32", "bf16x2", "fp16x2", "bf16"]:
+    for form in ["fp32", "fp32raw", "bf16x2", "fp16x2", "bf16"]:
         px = PfbX(form=form, fmt="cu8", fs=2.4e6, nchan=64, oversample=1, taps_per_poly=8, atten_db=81.2953,
                   streams=S, block_max=n, offset=0.001 + 0.002j)
         for s in range(S):
@@ -84,12 +87,17 @@ def precision_forms(S: int = 256, n: int = 1_200_000, steps: int = 10):
                      "filterbank_ms": round(pfb_ms, 4), "Msamples_per_s": round(S * n / (wall * 1e-3) / 1e6, 1)}
         px.close()
         print(f"[bench] precision form {form}: {out[form]}", file=sys.stderr, flush=True)
-    fp = out["fp32"]["device_ms"]
+    fp, fr = out["fp32"]["device_ms"], out["fp32raw"]["device_ms"]
+    kr = out["fp32raw"]["filterbank_ms"]
+    mx = [f for f in out if f not in ("fp32", "fp32raw")]
     return {"workload": f"{S} streams x {n} cu8 samples (2.4 MSPS), PFB 64 ch, 8 taps/branch, 1x, 81.3 dB",
             "forms": out,
             "speedup_vs_fp32": {f: round(fp / out[f]["device_ms"], 3) for f in out if f != "fp32"},
+            "speedup_vs_fp32raw": {f: round(fr / out[f]["device_ms"], 3) for f in mx},
+            "kernel_speedup_vs_fp32raw": {f: round(kr / out[f]["filterbank_ms"], 3) for f in mx},
             "note": "device_ms: HIP events around one whole run (unpack where there is one, filterbank, "
-                    "history copy); errors against the FP64 oracle are in tests/test_gpu_precision.py"}
+                    "history copy); filterbank_ms: the filterbank kernel alone; fp32raw reads the raw rows the "
+                    "matrix-core forms read; errors against the FP64 oracle are in tests/test_gpu_precision.py"}
 
 
 def drop_in(base: np.ndarray, calls: int = 20):

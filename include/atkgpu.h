@@ -221,12 +221,15 @@ ATKGPU_API int       atkgpu_server_detect_latest(atkgpu_server *v, const int *si
  * that path with the FP32 fold on the matrix cores (4x4x1 FP32 products,
  * one rounding a tap, taps in ascending order: bit-identical to the FP32
  * form for finite input; 256, 512 or 1024 channels at 2x, 256 or 512 at 1x,
- * up to 16 taps/branch).
+ * up to 16 taps/branch). ATKGPU_FORM_FP32RAW is the FP32 form reading the raw
+ * rows the matrix-core forms read, each sample converted as it is loaded
+ * (bit-identical to the FP32 form; 64 or 128 channels, 1x or 2x, up to 32 tap
+ * rows, cu8/ci8/ci16/ci16q11): the like-for-like baseline of those forms.
  * Supported shapes: 64 channels 1x with up to 8 taps/branch (every form and
  * format), 64 channels 1x up to 16 or 32 taps, 64 channels 2x up to 8 taps,
  * 128 channels 2x up to 8 taps (BF16 hi+lo for 8-bit, FP16 hi+lo for 12-bit). */
 enum { ATKGPU_FORM_FP32 = 0, ATKGPU_FORM_BF16X2 = 1, ATKGPU_FORM_BF16 = 2, ATKGPU_FORM_FP16X2 = 3,
-       ATKGPU_FORM_FP16 = 4, ATKGPU_FORM_FP32MM = 5 };
+       ATKGPU_FORM_FP16 = 4, ATKGPU_FORM_FP32MM = 5, ATKGPU_FORM_FP32RAW = 6 };
 typedef struct {
     int fmt;                   /* ATKDSP_FMT_CU8, _CI8 or _CI16Q11 (FP32 form: any) */
     double fs;
