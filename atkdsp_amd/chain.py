@@ -37,6 +37,7 @@ def _bind_chain(lib) -> None:
         "atkgpu_chain_run": (C.c_ssize_t, [vp, sz]),
         "atkgpu_chain_sync": (C.c_int, [vp]),
         "atkgpu_chain_timing": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_float)]),
+        "atkgpu_chain_set_timing": (C.c_int, [vp, C.c_int]),
         "atkgpu_chain_read_pfb": (C.c_ssize_t, [vp, C.c_uint, P(_Cf32), sz]),
         "atkgpu_chain_read_line": (C.c_int, [vp, C.c_uint, P(C.c_float), P(C.c_float), P(C.c_float)]),
         "atkgpu_chain_read_mean": (C.c_int, [vp, C.c_uint, P(C.c_double)]),
@@ -93,6 +94,10 @@ class Chain:
 
     def sync(self) -> None:
         _check(self._lib.atkgpu_chain_sync(self._h), "chain_sync")
+
+    def set_timing(self, on: bool) -> None:
+        """Record (default) or skip the per-run timing events timing() reads."""
+        _check(self._lib.atkgpu_chain_set_timing(self._h, int(bool(on))), "chain_set_timing")
 
     def timing(self):
         a, b, d = C.c_float(), C.c_float(), C.c_float()

@@ -20,6 +20,8 @@ struct atkgpu_chain {
     size_t keep, spitch, bps;
     void *stream;
     void *ev[4];
+    int timing;                /* record the timing events in each run */
+    int timed;                 /* the last run recorded them */
     atkdsp_cf32 *d_rows[2];
     int cur;
     unsigned char *d_raw;
@@ -67,6 +69,7 @@ atkgpu_chain *atkgpu_chain_create(const atkgpu_chain_cfg *cfg, unsigned S) {
     if (atkgpu_init() != 0) return NULL;
     atkgpu_chain *c = (atkgpu_chain *)calloc(1, sizeof *c);
     if (!c) return NULL;
+    c->timing = 1;
     c->cfg = *cfg;
     c->S = S;
     c->M = cfg->nchan;
@@ -173,7 +176,8 @@ ptrdiff_t atkgpu_chain_run(atkgpu_chain *c, size_t n) {
     int nfb = (int)((512 + c->S - 1) / c->S);
     if (nfb > c->nfb_max) nfb = c->nfb_max;
     if (nfb < 1) nfb = 1;
-    int rc = atkgpu_event_record(c->ev[0], c->stream);
+    const int tm = c->timing;
+    int rc = tm ? atkgpu_event_record(c->ev[0], c->stream) : 0;
     /* the display reads the raw blocks and unpacks them into the rows on the
      * way (with the block mean); a block that completes no frame is
      * unpacked on its own */
@@ -185,12 +189,12 @@ ptrdiff_t atkgpu_chain_run(atkgpu_chain *c, size_t n) {
     else if (!rc)
         rc = atkgpu_chain_unpack(c->stream, c->d_raw, g->block_max * c->bps, g->fmt, n, R + keep, c->spitch,
                                  c->d_off, c->d_upart, c->uparts, c->d_mean, g->dc_mode == 1, c->S);
-    if (!rc) rc = atkgpu_event_record(c->ev[1], c->stream);
+    if (!rc && tm) rc = atkgpu_event_record(c->ev[1], c->stream);
     if (!rc && m)
         rc = atkgpu_pfb_fused(c->stream, R + (keep - c->ov - c->lc), c->spitch, (size_t)c->ov + avail, c->S, m,
                               c->b, c->M, c->os, (unsigned)c->kos, c->d_taps, atkgpu_fftplan_twiddles(c->plan_m),
                               NULL, 0, c->d_out, (m + 15) & ~(size_t)15, 0);
-    if (!rc) rc = atkgpu_event_record(c->ev[2], c->stream);
+    if (!rc && tm) rc = atkgpu_event_record(c->ev[2], c->stream);
     if (!rc && frames) {
         const double seg_step = g->fs / (double)g->line_cols;
         rc = atkgpu_chain_post(c->stream, c->d_line, c->S, g->nfft, g->line_cols, c->d_dec, c->d_trace,
@@ -198,7 +202,7 @@ ptrdiff_t atkgpu_chain_run(atkgpu_chain *c, size_t n) {
                                g->trace_hz_per_bin);
     }
     if (!rc) rc = atkgpu_chain_keep(c->stream, R + n, c->spitch, T, keep, c->S);
-    if (!rc) rc = atkgpu_event_record(c->ev[3], c->stream);
+    if (!rc && tm) rc = atkgpu_event_record(c->ev[3], c->stream);
     if (rc) return ATKDSP_E_FFT;
     c->cur ^= 1;
     c->lc = (unsigned)(avail - m * c->hop);
@@ -207,6 +211,7 @@ ptrdiff_t atkgpu_chain_run(atkgpu_chain *c, size_t n) {
     c->last_m = m;
     c->last_frames = frames;
     c->ran = 1;
+    c->timed = tm;
     return (ptrdiff_t)m;
 }
 
@@ -215,8 +220,14 @@ int atkgpu_chain_sync(atkgpu_chain *c) {
     return atkgpu_sync(c->stream) ? ATKDSP_E_FFT : ATKDSP_OK;
 }
 
+int atkgpu_chain_set_timing(atkgpu_chain *c, int on) {
+    if (!c) return ATKDSP_E_ARG;
+    c->timing = on != 0;
+    return ATKDSP_OK;
+}
+
 int atkgpu_chain_timing(atkgpu_chain *c, float *ms_run, float *ms_pfb, float *ms_disp) {
-    if (!c || !c->ran) return ATKDSP_E_ARG;
+    if (!c || !c->ran || !c->timed) return ATKDSP_E_ARG;
     /* ev0 | display with the unpack, block mean | ev1 | filterbank | ev2 | decimate, stitch, keep | ev3 */
     float a = 0, b = 0, d = 0;
     if (atkgpu_event_elapsed_ms(c->ev[0], c->ev[3], &a) || atkgpu_event_elapsed_ms(c->ev[1], c->ev[2], &b) ||

@@ -68,8 +68,13 @@ ATKGPU_API int           atkgpu_chain_upload(atkgpu_chain *c, unsigned s, const 
 ATKGPU_API ptrdiff_t     atkgpu_chain_run(atkgpu_chain *c, size_t n);
 ATKGPU_API int           atkgpu_chain_sync(atkgpu_chain *c);
 /* device times of the last run, ms: whole run, the filterbank kernel alone,
- * the display reduction with the unpack and block mean */
+ * the display reduction with the unpack and block mean (ATKDSP_E_ARG when
+ * the last run was not timed) */
 ATKGPU_API int           atkgpu_chain_timing(atkgpu_chain *c, float *ms_run, float *ms_pfb, float *ms_disp);
+/* on != 0 (the default): each run records the four timing events that
+ * atkgpu_chain_timing reads; each costs the stream about 5 us, so a caller
+ * that does not read them may turn them off */
+ATKGPU_API int           atkgpu_chain_set_timing(atkgpu_chain *c, int on);
 /* results of the last run, copied to host (synchronising) */
 ATKGPU_API ptrdiff_t     atkgpu_chain_read_pfb(atkgpu_chain *c, unsigned s, atkdsp_cf32 *out, size_t cap);
 ATKGPU_API int           atkgpu_chain_read_line(atkgpu_chain *c, unsigned s, float *line_db, float *cols,

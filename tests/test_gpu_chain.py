@@ -61,3 +61,37 @@ def test_chain_matches_dropin_and_oracle(gpu, fmt, nchan, taps, n):
             O.stitch_max(cols, s * FS - FS / 2, FS / 1024, s * FS - FS / 2, s * FS + FS / 2, -FS / 2,
                          FS / 1024 * 4, traces[s])
             assert np.array_equal(trace.view(np.uint32), traces[s].view(np.uint32)), f"trace {s} {b}"
+
+
+def test_timing_events_optional(gpu):
+    """Runs without the timing events give the same outputs bit for bit;
+    timing() refuses a run that recorded none and reads the next one that did."""
+    from atkdsp_amd import synth
+    from atkdsp_amd.chain import Chain
+    n = 100_000
+    raws = [synth.quantize(synth.wideband_iq(7 + b, n, FS), gpu.FMT["ci16q11"]) for b in range(3)]
+    outs = []
+    for timed in (True, False):
+        ch = Chain(fmt="ci16q11", fs=FS, nchan=1024, oversample=2, taps_per_poly=16, nfft=4096, fft_hop=2048,
+                   line_cols=1024, trace_bins=8192, fc0_hz=0.0, fc_step_hz=FS, trace_lo_hz=-FS / 2,
+                   trace_hz_per_bin=FS / 1024 * 4, dc_mode=1, streams=1, block_max=n)
+        ch.set_timing(timed)
+        ys = []
+        for raw in raws:
+            ch.upload(0, raw)
+            ch.run(n)
+            ys.append(ch.read_pfb(0).copy())
+            ys.append(ch.read_line(0)[0].copy())
+        if timed:
+            assert all(t >= 0 for t in ch.timing())
+        else:
+            with pytest.raises(gpu.AtkDspError):
+                ch.timing()
+            ch.set_timing(True)
+            ch.upload(0, raws[0])
+            ch.run(n)
+            ch.sync()
+            assert all(t >= 0 for t in ch.timing())
+        outs.append(ys)
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
