@@ -60,6 +60,32 @@ typedef int (*atk_run_fn)(void *ctx, void *s, size_t i0, size_t cnt);
 int atk_pipe(void *s, void *s2, const void *in, size_t in_bpi, void *d_in, void *out, size_t out_bpi,
              const void *d_out, size_t n, size_t piece, atk_run_fn run, void *ctx);
 
+/* Work whose input is one run of caller bytes and whose output comes back in
+ * ranges, each its own set of host rows (a filterbank call: every range of
+ * hops is a column block of the channel-major output). For r = 0 .. nr-1,
+ * desc(ctx, r, &g) says how many input bytes range r needs on the device
+ * (g.need, non-decreasing), where its packed output sits (g.src, at most one
+ * staging piece, ATK_PIPE_PIECE bytes; ranges r and r+2 may share a buffer,
+ * never r and r+1) and where its rows go (g.dst, g.rows rows of g.width
+ * bytes, g.dpitch apart); run(ctx, s, r) queues its kernels on `s`. The input
+ * goes to d_in on `s` piece by piece as the ranges need it, range r's output
+ * comes back on `s2` while range r+1's input goes in and its kernels run, and
+ * the host writes range r-1's rows meanwhile. Returns once every row is
+ * written and all in_bytes are on the device (in `s` order); nonzero on an
+ * error (from desc or run, or a device error). Without staging, it falls back
+ * to one copy in, every range's kernels, and each range's rows out in turn. */
+#define ATK_PIPE_PIECE ((size_t)4 << 20)
+typedef struct {
+    size_t need;
+    const void *src;
+    char *dst;
+    size_t dpitch, width, rows;
+} atk_range;
+typedef int (*atk_desc_fn)(void *ctx, size_t r, atk_range *g);
+typedef int (*atk_range_fn)(void *ctx, void *s, size_t r);
+int atk_pipe_ranges(void *s, void *s2, void *d_in, const void *in, size_t in_bytes, size_t nr, atk_desc_fn desc,
+                    atk_range_fn run, void *ctx);
+
 /* channel records strongest first, with the reference's comparator and qsort */
 void atk_sort_channels(atkdsp_channel *c, size_t n);
 

@@ -25,6 +25,7 @@ struct atkdsp_pfb {
     int kos;                   /* > 0: the fused kernel runs this shape */
     atkgpu_fftplan *plan;      /* nchan-point            */
     void *stream;
+    void *stream2;             /* a single element's output comes back on it, range by range */
     size_t slab;               /* input samples per sub-call (multiple of hop) */
     size_t spitch;             /* samples per element in a slab buffer */
     atkdsp_cf32 *d_s[2];       /* slab buffers, max_nel * spitch */
@@ -101,6 +102,7 @@ static void pfb_free(atkdsp_pfb *p) {
     atkgpu_dfree(p->d_cal);
     atkgpu_fftplan_destroy(p->plan);
     if (p->stream) atkgpu_stream_destroy(p->stream);
+    if (p->stream2) atkgpu_stream_destroy(p->stream2);
     free(p->h);
     free(p->lo);
     free(p);
@@ -135,7 +137,10 @@ atkdsp_pfb *atkdsp_pfb_create(double fs, unsigned nchan, unsigned oversample, un
     p->hops_cap = (((slab + p->hop) / p->hop + 1) + 15) & ~(size_t)15;
     p->h = (double *)malloc(p->L * sizeof(double));
     p->lo = (atkdsp_cf32 *)calloc((size_t)max_nel * p->hop, sizeof(atkdsp_cf32));
-    if (!p->h || !p->lo || atkgpu_stream_create(&p->stream)) { pfb_free(p); return NULL; }
+    if (!p->h || !p->lo || atkgpu_stream_create(&p->stream) || atkgpu_stream_create(&p->stream2)) {
+        pfb_free(p);
+        return NULL;
+    }
     p->plan = atkgpu_fftplan_create(nchan);
     const size_t nm = (size_t)max_nel * p->hops_cap * nchan;
     p->kos = atkgpu_pfb_fused_kos(nchan, oversample, taps_per_poly);
@@ -219,6 +224,48 @@ int atkdsp_pfb_describe(const atkdsp_pfb *p, char *buf, size_t cap) {
 
 /* One sub-call over ins[e][pos, pos+c): writes its m_s hops at column i_done
  * of the caller's [nel][nchan][m] output. */
+/* One element's sub-call through the fused kernel in ranges of R hops, each
+ * range's output (nchan rows of at most R hops, one staging piece) coming
+ * back while the next range's input goes in (atk_pipe_ranges). Every hop
+ * depends only on its own window, so the ranges give the one-launch output. */
+typedef struct {
+    atkdsp_pfb *p;
+    atkdsp_cf32 *S;            /* the slab: window coordinate 0 */
+    size_t ms, R, R16, avail;
+    atkdsp_cf32 *out;          /* the call's first output hop of this sub-call */
+    size_t m;                  /* output row pitch, hops */
+} pfb_pipe_ctx;
+
+static int pfb_pipe_desc(void *ctx, size_t r, atk_range *g) {
+    const pfb_pipe_ctx *x = (const pfb_pipe_ctx *)ctx;
+    const size_t h0 = r * x->R, h1 = h0 + x->R < x->ms ? h0 + x->R : x->ms;
+    const size_t hop = x->p->hop, lc = x->p->lc;
+    /* the range's last window ends at sample h1*hop + ov of the slab; the
+     * call's input starts at ov + lc */
+    g->need = (h1 * hop > lc ? h1 * hop - lc : 0) * sizeof(atkdsp_cf32);
+    g->src = x->p->d_fold + (r & 1) * (size_t)x->p->nchan * x->R;
+    g->dst = (char *)(x->out + h0);
+    g->dpitch = x->m * sizeof(atkdsp_cf32);
+    g->width = (h1 - h0) * sizeof(atkdsp_cf32);
+    g->rows = x->p->nchan;
+    return 0;
+}
+
+static int pfb_pipe_run(void *ctx, void *s, size_t r) {
+    const pfb_pipe_ctx *x = (const pfb_pipe_ctx *)ctx;
+    atkdsp_pfb *p = x->p;
+    const size_t h0 = r * x->R, h = (h0 + x->R < x->ms ? h0 + x->R : x->ms) - h0;
+    atkdsp_cf32 *o = p->d_out + (r & 1) * (size_t)p->nchan * x->R16;
+    int rc = atkgpu_pfb_fused(s, x->S + h0 * p->hop, p->spitch, (size_t)p->ov + x->avail - h0 * p->hop, 1, h,
+                              (long long)p->b + (long long)h0, p->nchan, p->os, (unsigned)p->kos, p->d_taps,
+                              atkgpu_fftplan_twiddles(p->plan), p->cal_nel ? (const void *)p->d_cal : NULL,
+                              p->cal_nel, o, x->R16, 0);
+    if (!rc)
+        rc = atkgpu_d2d_2d(s, p->d_fold + (r & 1) * (size_t)p->nchan * x->R, h * sizeof(atkdsp_cf32), o,
+                           x->R16 * sizeof(atkdsp_cf32), h * sizeof(atkdsp_cf32), p->nchan);
+    return rc;
+}
+
 static int pfb_sub(atkdsp_pfb *p, const atkdsp_cf32 *const *ins, unsigned nel, size_t pos, size_t c,
                    atkdsp_cf32 *out, size_t m, size_t i_done, size_t *made) {
     const unsigned M = p->nchan, hop = p->hop, ov = p->ov;
@@ -235,6 +282,15 @@ static int pfb_sub(atkdsp_pfb *p, const atkdsp_cf32 *const *ins, unsigned nel, s
     /* device output pitch: whole 128-byte lines for the fused kernel */
     const size_t mp = p->kos ? (ms + 15) & ~(size_t)15 : ms;
     int rc = 0;
+    /* one element with more hops than a staging piece of output holds: in ranges */
+    const size_t R = (ATK_PIPE_PIECE / ((size_t)M * sizeof(atkdsp_cf32))) & ~(size_t)15;
+    if (p->kos && nel == 1 && R >= 16 && ms > R && 2 * R <= (size_t)p->max_nel * p->hops_cap) {
+        if (p->lc) rc = atkgpu_h2d(p->stream, S + ov, p->lo, p->lc * sizeof(atkdsp_cf32));
+        pfb_pipe_ctx x = {p, S, ms, R, R, avail, out + i_done, m};
+        if (!rc) rc = atk_pipe_ranges(p->stream, p->stream2, S + ov + p->lc, ins[0] + pos,
+                                      c * sizeof(atkdsp_cf32), (ms + R - 1) / R, pfb_pipe_desc, pfb_pipe_run, &x);
+        goto history;
+    }
     for (unsigned e = 0; e < nel && !rc; ++e) {
         atkdsp_cf32 *se = S + (size_t)e * p->spitch;
         if (p->lc) rc = atkgpu_h2d(p->stream, se + ov, p->lo + (size_t)e * hop, p->lc * sizeof(atkdsp_cf32));
@@ -262,6 +318,7 @@ static int pfb_sub(atkdsp_pfb *p, const atkdsp_cf32 *const *ins, unsigned nel, s
     }
     if (!rc) rc = atk_get_rows(p->stream, out + i_done, m * sizeof(atkdsp_cf32), packed, ms * sizeof(atkdsp_cf32),
                                (size_t)nel * M);
+history:;
     /* next history: s[ms hop, ms hop + ov) into the other slab buffer */
     atkdsp_cf32 *T = p->d_s[p->cur ^ 1];
     for (unsigned e = 0; e < nel && !rc; ++e)

@@ -24,7 +24,9 @@
 #define XF_PIECE ((size_t)4 << 20)
 #define XF_NBUF 4
 #define XF_NENG 4
+#ifndef XF_HELPERS
 #define XF_HELPERS 3                      /* the calling thread makes four   */
+#endif
 #define XF_SPLIT ((size_t)256 << 10)      /* smaller copies stay on one thread */
 #define XF_STAGE_MIN ((size_t)1 << 20)    /* smaller copies go to the runtime directly */
 
@@ -359,6 +361,74 @@ int atk_pipe(void *s, void *s2, const void *in, size_t in_bpi, void *d_in, void 
             team_copy(&jo);
         }
     }
+    engine_release(e);
+    return rc;
+}
+
+/* ---- input in one run, output in ranges ---------------------------------------------- */
+/* Pinned pieces 0 and 1 carry the input (alternately), 2 and 3 the ranges'
+ * outputs (range r in piece 2 + r%2). A piece is rewritten only once the host
+ * has waited for its last copy: the input piece two uploads back, the output
+ * piece of range r-2, which the host wrote out before queuing range r. */
+static int pipe_upload(xf_engine *e, void *s, char *d_in, const char *in, size_t *done, size_t upto, size_t *k) {
+    int rc = 0;
+    while (*done < upto && !rc) {
+        const size_t n = upto - *done < XF_PIECE ? upto - *done : XF_PIECE;
+        const int b = (int)(*k & 1);
+        rc = piece_wait(e, b);
+        if (rc) break;
+        const xf_job j = {e->buf[b], in + *done, n, n, n, 1};
+        team_copy(&j);
+        rc = atkgpu_h2d(s, d_in + *done, e->buf[b], n);
+        if (!rc) rc = piece_arm(e, b, s);
+        *done += n;
+        ++*k;
+    }
+    return rc;
+}
+
+static int pipe_drain(xf_engine *e, const atk_range *g, int b) {
+    int rc = piece_wait(e, b);
+    if (!rc && g->rows) {
+        const xf_job j = {g->dst, e->buf[b], g->dpitch, g->width, g->width, g->rows};
+        team_copy(&j);
+    }
+    return rc;
+}
+
+int atk_pipe_ranges(void *s, void *s2, void *d_in, const void *in, size_t in_bytes, size_t nr, atk_desc_fn desc,
+                    atk_range_fn run, void *ctx) {
+    xf_engine *e = (s2 && nr > 1) ? engine_claim() : NULL;
+    int rc = 0;
+    if (!e) {
+        rc = atk_put(s, d_in, in, in_bytes);
+        for (size_t r = 0; r < nr && !rc; ++r) {
+            atk_range g;
+            rc = desc(ctx, r, &g);
+            if (!rc) rc = run(ctx, s, r);
+            if (!rc && g.rows) rc = atk_get_rows(s, g.dst, g.dpitch, g.src, g.width, g.rows);
+        }
+        return rc;
+    }
+    size_t done = 0, k = 0;
+    atk_range g[2];
+    for (size_t r = 0; r < nr && !rc; ++r) {
+        atk_range *cur = &g[r & 1];
+        rc = desc(ctx, r, cur);
+        if (!rc && cur->width * cur->rows > XF_PIECE) rc = -1;
+        if (!rc) rc = pipe_upload(e, s, (char *)d_in, (const char *)in, &done,
+                                  cur->need < in_bytes ? cur->need : in_bytes, &k);
+        if (!rc) rc = run(ctx, s, r);
+        const int bo = 2 + (int)(r & 1);
+        if (!rc) rc = atkgpu_event_record(e->kev[r & 1], s);
+        if (!rc) rc = atkgpu_stream_wait_event(s2, e->kev[r & 1]);
+        if (!rc) rc = piece_wait(e, bo);                      /* range r-2's rows are out */
+        if (!rc && cur->rows) rc = atkgpu_d2h(s2, e->buf[bo], cur->src, cur->width * cur->rows);
+        if (!rc) rc = piece_arm(e, bo, s2);
+        if (!rc && r > 0) rc = pipe_drain(e, &g[(r - 1) & 1], 2 + (int)((r - 1) & 1));
+    }
+    if (!rc) rc = pipe_drain(e, &g[(nr - 1) & 1], 2 + (int)((nr - 1) & 1));
+    if (!rc) rc = pipe_upload(e, s, (char *)d_in, (const char *)in, &done, in_bytes, &k);
     engine_release(e);
     return rc;
 }
