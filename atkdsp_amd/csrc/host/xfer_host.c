@@ -7,7 +7,7 @@
  * device-to-host copy of 32 MB took 10.6 ms). Here a large copy moves in
  * 4 MiB pieces through a ring of pinned buffers: the DMA of one piece (about
  * 56 GB/s each way) overlaps the host copy of the next, and the host copies
- * are split over a small team of threads (about 90 GB/s with four).
+ * are split over a small team of threads.
  *
  * Engines: a few, each a ring of XF_NBUF pinned pieces with an event per
  * piece, claimed by CAS like the device slots (one caller per engine at a
@@ -24,8 +24,14 @@
 #define XF_PIECE ((size_t)4 << 20)
 #define XF_NBUF 4
 #define XF_NENG 4
+/* helpers beside the calling thread: up to XF_HELPERS, and fewer than the
+ * library's thread count (atkdsp_get_threads: the host's cores unless the
+ * caller set it), counted at the first large copy. Host copies of pageable
+ * memory saturate near eight threads on an MI355X host (16 cores): a 50 ms
+ * 1024-channel filterbank call took 1.05-1.08 ms with eight threads,
+ * 1.19-1.22 ms with four and 1.08-1.11 ms with twelve */
 #ifndef XF_HELPERS
-#define XF_HELPERS 3                      /* the calling thread makes four   */
+#define XF_HELPERS 7
 #endif
 #define XF_SPLIT ((size_t)256 << 10)      /* smaller copies stay on one thread */
 #define XF_STAGE_MIN ((size_t)1 << 20)    /* smaller copies go to the runtime directly */
@@ -105,7 +111,8 @@ static void team_start(void) {
     pthread_attr_init(&at);
     pthread_attr_setdetachstate(&at, PTHREAD_CREATE_DETACHED);
     int n = 0;
-    for (int i = 1; i <= XF_HELPERS; ++i) {
+    const int want = atkdsp_get_threads() - 1 < XF_HELPERS ? atkdsp_get_threads() - 1 : XF_HELPERS;
+    for (int i = 1; i <= want; ++i) {
         pthread_t t;
         if (pthread_create(&t, &at, helper, (void *)(intptr_t)i) != 0) break;
         ++n;
