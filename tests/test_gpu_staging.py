@@ -112,3 +112,37 @@ def test_filterbank_rows_longer_than_a_piece(gpu):
     y = np.concatenate([p.process(x[a:a + blk]) for a in range(0, n, blk)], axis=1)
     assert whole.shape == y.shape == (4, n // 4)
     assert np.array_equal(whole.view(np.uint32), y.view(np.uint32))
+
+
+@pytest.mark.parametrize("nchan,os_,K,cal", [(1024, 2, 16, False), (1024, 2, 16, True), (64, 1, 8, False),
+                                             (2048, 2, 12, False)])
+def test_filterbank_output_in_ranges(gpu, nchan, os_, K, cal):
+    """A one-element call with more hops than one staging piece of output holds
+    runs in ranges of hops, each range's output coming back while the next
+    range's input goes in. It equals the same samples in blocks too small for
+    ranges bit for bit, with a leftover carried into the call, a cal table,
+    and a call of one range and one hop more."""
+    hop = nchan // os_
+    rows = (4 << 20) // (nchan * 8) // 16 * 16          # hops a range holds
+    rng = np.random.default_rng(nchan + K)
+    n1 = 2_000_003
+    n2 = rows * hop + hop                               # one range and one hop more, with the leftover
+    x = (rng.standard_normal(n1 + n2 + 777) + 1j * rng.standard_normal(n1 + n2 + 777)).astype(np.complex64)
+    x *= np.float32(0.2)
+    tab = (np.exp(1j * rng.uniform(0, 2 * np.pi, nchan)) * rng.uniform(0.5, 2.0, nchan)).astype(np.complex64)
+
+    def make():
+        p = gpu.Pfb(40e6, nchan, os_, K, 80.0)
+        if cal:
+            p.set_cal(tab, 1)
+        return p
+
+    p = make()
+    parts = [p.process(x[:777]), p.process(x[777:777 + n1]), p.process(x[777 + n1:])]
+    assert parts[1].shape[1] > rows and parts[2].shape[1] > rows
+    whole = np.concatenate([a for a in parts if a.shape[1]], axis=1)
+    q = make()
+    blk = 20 * hop + 13                                 # a few dozen hops a call: one range, no pipeline
+    y = np.concatenate([b for b in (q.process(x[a:a + blk]) for a in range(0, x.size, blk)) if b.shape[1]], axis=1)
+    assert whole.shape == y.shape
+    assert np.array_equal(whole.view(np.uint32), y.view(np.uint32))
