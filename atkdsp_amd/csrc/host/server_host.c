@@ -61,6 +61,14 @@ int atkgpu_pfb_prototype(double *h, unsigned nchan, unsigned K, double atten_db)
 #ifndef NINFLIGHT
 #define NINFLIGHT 3            /* batches launched and not yet complete */
 #endif
+/* ordering events: batch k records oev[k % NOEV] after its compute, and batch
+ * k + 2's ingest waits on it. Batch k + NOEV, which records it again, is
+ * launched only once batch k + NOEV - NINFLIGHT = k + 3 is complete, and so
+ * after batch k + 2's ingest (and its wait) has run: an event is never
+ * recorded again while a wait on it may still be pending */
+#ifndef NOEV
+#define NOEV (2 * NINFLIGHT)
+#endif
 #define POLL_NS 10000          /* completion poll interval */
 #define QCAP 256               /* pending chunks per stream */
 /* chunk latencies: a histogram of log-spaced buckets, each 1/256 wider
@@ -164,7 +172,6 @@ typedef struct {
 
 typedef struct {
     void *event;               /* the batch's compute is done (polled by the completion thread) */
-    void *oevent;              /* the same point, for the dispatcher's stream waits only */
     void *uevent;              /* its ingest is done */
     void *devent;              /* its display lines are done */
     int n;
@@ -195,6 +202,7 @@ struct atkgpu_server {
     void *qs_h, *qs_d;                           /* mapped, per stream: line pointers, records, counts, floors */
     atkgpu_fftplan *plans[NPLANS];               /* twiddles for 64..4096 points */
     batch_t batch[NINFLIGHT];
+    void *oev[NOEV];                             /* compute done, for the dispatcher's stream waits only */
     unsigned b_submit, b_done;                   /* batches launched / completed */
     unsigned rr;                                 /* round-robin start */
     pthread_mutex_t mu;
@@ -276,9 +284,10 @@ atkgpu_server *atkgpu_server_create(size_t hbm_budget, unsigned max_streams, uns
              (v->q_h = atkgpu_hmalloc_mapped(QREC_CAP * sizeof(atkgpu_chan) + 64, &v->q_d)) &&
              (v->qs_h = atkgpu_hmalloc_mapped(qset_bytes(max_streams), &v->qs_d));
     for (int i = 0; ok && i < NPLANS; ++i) ok = (v->plans[i] = atkgpu_fftplan_create(64u << i)) != NULL;
+    for (int i = 0; ok && i < NOEV; ++i) ok = !atkgpu_event_create_order(&v->oev[i]);
     for (int i = 0; ok && i < NINFLIGHT; ++i) {
         batch_t *b = &v->batch[i];
-        ok = !atkgpu_event_create_order(&b->event) && !atkgpu_event_create_order(&b->oevent) &&
+        ok = !atkgpu_event_create_order(&b->event) &&
              !atkgpu_event_create_order(&b->uevent) &&
              !atkgpu_event_create_order(&b->devent) &&
              (b->items = (bitem *)calloc(max_batch, sizeof(bitem))) &&
@@ -338,9 +347,9 @@ void atkgpu_server_destroy(atkgpu_server *v) {
     if (v->stream) atkgpu_sync(v->stream);
     for (unsigned i = 0; v->st && i < v->max_streams; ++i)
         if (v->st[i].used) stream_free(&v->st[i]);
+    for (int i = 0; i < NOEV; ++i) atkgpu_event_destroy(v->oev[i]);
     for (int i = 0; i < NINFLIGHT; ++i) {
         atkgpu_event_destroy(v->batch[i].event);
-        atkgpu_event_destroy(v->batch[i].oevent);
         atkgpu_event_destroy(v->batch[i].uevent);
         atkgpu_event_destroy(v->batch[i].devent);
         atkgpu_dfree(v->batch[i].dP);
@@ -700,7 +709,7 @@ static int launch_batch(atkgpu_server *v, batch_t *bt, unsigned k) {
      * on; the completion thread polls the other one. */
     void *ust = INGEST_STREAM ? v->ustream : v->stream;
     if (!rc && INGEST_STREAM && k >= 2)
-        rc = atkgpu_stream_wait_event(v->ustream, v->batch[(k - 2) % NINFLIGHT].oevent);
+        rc = atkgpu_stream_wait_event(v->ustream, v->oev[(k - 2) % NOEV]);
     if (!rc && nu) rc = atkgpu_unpack_jobs(ust, DEV(uj), nu, nu, 1, INGEST_GRID);
     if (!rc) rc = atkgpu_event_record(bt->uevent, ust);
     if (!rc && INGEST_STREAM) rc = atkgpu_stream_wait_event(v->stream, bt->uevent);
@@ -803,7 +812,7 @@ static int launch_batch(atkgpu_server *v, batch_t *bt, unsigned k) {
 #undef DEV
     if (!rc && nb && DISPLAY_STREAM) rc = atkgpu_event_record(bt->devent, v->dstream);
     if (!rc && nb && DISPLAY_STREAM) rc = atkgpu_stream_wait_event(v->stream, bt->devent);
-    if (!rc) rc = atkgpu_event_record(bt->oevent, v->stream);
+    if (!rc) rc = atkgpu_event_record(v->oev[k % NOEV], v->stream);
     if (!rc) rc = atkgpu_event_record(bt->event, v->stream);
     return rc;
 }
