@@ -44,6 +44,27 @@ def make_streams(S: int):
     return base, [np.roll(base, s * 7919, axis=0) for s in range(S)]
 
 
+# the whole chain per input sample at the wideband shape: the compulsory bytes
+# (the raw ci16 sample read once, the 2x cf32 filterbank output written once,
+# the display lines negligible) and the useful FP32 work (fold 16 taps x 2
+# branches x complex = 128, the 1024-point FFT per 512 samples = 100, its
+# rotation 12, the 4096-point display FFT per 2048 samples = 120, window,
+# power and max-hold 12)
+CHAIN_BYTES_PER_SAMPLE = 4 + 16
+CHAIN_FLOP_PER_SAMPLE = 128 + 100 + 12 + 120 + 12
+VALU_FP32_PEAK_TFLOPS = 78.6      # MI355X FP32 vector, without packed FP32
+
+
+def chain_roofline(samples_per_s: float) -> dict:
+    gbs = samples_per_s * CHAIN_BYTES_PER_SAMPLE / 1e9
+    tfl = samples_per_s * CHAIN_FLOP_PER_SAMPLE / 1e12
+    return {"bytes_per_sample": CHAIN_BYTES_PER_SAMPLE, "achieved_GBps": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "flop_per_sample": CHAIN_FLOP_PER_SAMPLE,
+            "achieved_TFLOPs": round(tfl, 2), "valu_frac": round(tfl / VALU_FP32_PEAK_TFLOPS, 4),
+            "note": "the whole chain against its compulsory bytes and its useful FP32 operations per input "
+                    "sample; the VALU peak is the non-packed FP32 rate (packed FP32 is not used)"}
+
+
 def pfb_bytes_per_launch(S: int, n: int, m: int) -> float:
     """Compulsory bytes of one filterbank launch: every input sample and the
     history read once as cf32, every output written once as cf32."""
@@ -631,6 +652,7 @@ def main() -> int:
                      "traffic_source": traffic_src,
                      "kernel": "k_pfb_fused<1024,2,32>",
                      "bytes_per_launch": pfb_b},
+        "chain_roofline": chain_roofline(value),
         "cpu_baseline": cpu,
         **extras,
         "build": atkdsp_amd.build_info(),
