@@ -38,6 +38,7 @@ extern "C" int atkgpu_pfb_fused(void *s, const void *sbuf, size_t spitch, size_t
     if (!m || !nel) return 0;
     const int G = group_size(M, os);
     if (!G) return -1;
+    if (opitch > PFB_OPITCH_MAX) return -1;
     PfbFusedArgs a;
     a.jobs = nullptr;
     a.njobs = 0;

@@ -10,9 +10,13 @@
 //   FFT    the G fold rows go to LDS; teams of M/PT threads run the M-point
 //          team FFT (fft_core.hpp) in registers with LDS exchanges.
 //   rotate exp(-2 pi i k shift/M), shift = ((b+1)*HOP) mod M (int64 hop
-//          counter; at 2x this is (-1)^k on odd hops), then the cal factor.
+//          counter; at 2x this is (-1)^k on alternate hops, applied before
+//          the transform as a circular shift of its input by M/2: the fold
+//          writes the two branches of such a hop to each other's LDS slots),
+//          then the cal factor.
 //   store  bins go back to LDS and leave channel-major, G consecutive hops of
-//          a channel per lane group, out[(e*M + k)*opitch + i].
+//          a channel, two hops a lane as 16-byte buffer stores from a scalar
+//          base per channel pass, out[(e*M + k)*opitch + i].
 //
 // Taps are zero-padded from K*OS to KOS (an instantiated size): a zero tap
 // adds an exact zero, so results equal the unpadded sum for finite input.
@@ -64,11 +68,18 @@ struct PfbJob {
     cpx *out;              // channel k, hop i at out[k*opitch + i]
     const float *taps;
     long long b0;
-    int nvalid, m, opitch, blk0, nblk;
+    int nvalid, m, opitch, blk0, nblk;   // opitch at most PFB_OPITCH_MAX
     int pad_;
 };
 
 enum { PFB_IN_CF32 = -1 };
+
+// The output pitch a launch accepts (hops): the store's lane offsets, up to
+// 128 channel rows of a pass, are 32-bit byte offsets.
+#define PFB_OPITCH_MAX ((size_t)1 << 20)
+
+typedef unsigned pfb_u2 __attribute__((ext_vector_type(2)));
+typedef unsigned pfb_u4 __attribute__((ext_vector_type(4)));
 
 struct PfbFusedArgs {
     const PfbJob *jobs;    // non-null: a ragged batch, one descriptor per block
@@ -320,11 +331,18 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
             else
                 pfb_fold_pass<M, OS, KOS, true, TP::HREG, IN>(ar, ai, s, taps, hk, base, nlast, u, cre, cim, kk,
                                                               zlo);
+            // at 2x a hop whose b+1 is odd takes (-1)^k after the transform
+            // (reference src/channelizer.c:275-276 with shift = M/2); the
+            // transform of its input circularly shifted by M/2 gives the same
+            // values (an exactly zero bin keeps +0 where a negation gives -0):
+            // branch r's fold goes to branch 1-r's slot
 #pragma unroll
-            for (int p = 0; p < GP; ++p)
+            for (int p = 0; p < GP; ++p) {
+                const bool sw = OS == 2 && (((i0 + sub * GP + p) & 1) == (flip_base ? 1 : 0));
 #pragma unroll
                 for (int r = 0; r < OS; ++r)
-                    lds[(sub * GP + p) * ROW + lpad(r * HOP + u)] = cmake(ar[p][r], ai[p][r]);
+                    lds[(sub * GP + p) * ROW + lpad((sw ? OS - 1 - r : r) * HOP + u)] = cmake(ar[p][r], ai[p][r]);
+            }
         };
         if constexpr (TP::HREG) {
 #pragma unroll
@@ -334,7 +352,7 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
             for (int c = 0; c < S::COLS; ++c) pass(c, nullptr);
         }
         __syncthreads();
-        // ---- FFT, rotation, cal: team tm transforms hops tm, tm+TEAMS, ... ----
+        // ---- FFT and cal: team tm transforms hops tm, tm+TEAMS, ... ----
         const int tm = tid / S::NT, j = tid % S::NT;
         // a team inside one wavefront synchronises its stages wave-locally;
         // the rows it reads were published by the block barrier above
@@ -347,31 +365,51 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
             team_load<M, S::PT>(v, row, j);
             team_sync<WV>();
             team_fft_from<M, S::PT, 0, WV>(v, row, twl, j, 1);
-            // at 2x the rotation is (-1)^k on hops whose b+1 is odd
-            // (reference src/channelizer.c:275-276 with shift = M/2)
-            const bool flip = (OS == 2) && (((i0 + p) & 1) == (flip_base ? 1 : 0));
 #pragma unroll
             for (int r = 0; r < S::PT; ++r) {
                 const int k = j + r * S::NT;
                 cpx y = v[r];
-                if (flip && (k & 1)) y = cmake(-y.x, -y.y);
                 if (cal) y = cmul(y, cal[k]);
                 row[k] = y;
             }
             if constexpr (!WV) __syncthreads();
         }
         if constexpr (WV) __syncthreads();       // every row done before the store phase
-        // ---- channel-major store: LPC lanes per channel, consecutive hops ----
+        // ---- channel-major store: lane (channel lc of a pass, hops 2*lp and
+        // 2*lp+1), one 16-byte buffer store per pass at a per-block lane
+        // offset from a scalar base, so a store costs no vector arithmetic ----
         {
-            constexpr int CH_PER_PASS = 256 / S::LPC;
-            const int lp = tid % S::LPC, lc = tid / S::LPC;
-#pragma unroll 4
-            for (int k0 = 0; k0 < M; k0 += CH_PER_PASS) {
-                const int k = k0 + lc;
-                cpx *ok = oute + k * opitch + i0;
+            static_assert(G % 2 == 0 && S::LPC == G, "a group's hops are whole lane pairs of one pass");
+            constexpr int LP2 = G / 2;                   // lanes per channel
+            constexpr int CH_PER_PASS = 256 / LP2;
+            constexpr int Q = 4;                         // passes whose bins are read together
+            static_assert(M % (Q * CH_PER_PASS) == 0, "whole pass sets");
+            const int lp = tid % LP2, lc = tid / LP2;
+            const int voff = (lc * opitch + 2 * lp) * 8;
+            const bool two = i0 + 2 * lp + 1 < m, one = i0 + 2 * lp < m;
+#pragma unroll 1
+            for (int k0 = 0; k0 < M; k0 += Q * CH_PER_PASS) {
+                cpx v0[Q], v1[Q];
 #pragma unroll
-                for (int p = lp; p < G; p += S::LPC)
-                    if (i0 + p < m) ok[p] = lds[p * ROW + k];
+                for (int q = 0; q < Q; ++q) {
+                    v0[q] = lds[(2 * lp) * ROW + k0 + q * CH_PER_PASS + lc];
+                    v1[q] = lds[(2 * lp + 1) * ROW + k0 + q * CH_PER_PASS + lc];
+                }
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    // records cover the lane offsets of this pass; the hop
+                    // tests above keep every store inside rows < m
+                    const __amdgpu_buffer_rsrc_t orr = __builtin_amdgcn_make_buffer_rsrc(
+                        (void *)(oute + (long long)(k0 + q * CH_PER_PASS) * opitch + i0), (short)0, 0x7fffffff,
+                        0x00020000);
+                    if (two) {
+                        const pfb_u4 w = {__builtin_bit_cast(unsigned, v0[q].x), __builtin_bit_cast(unsigned, v0[q].y),
+                                          __builtin_bit_cast(unsigned, v1[q].x), __builtin_bit_cast(unsigned, v1[q].y)};
+                        __builtin_amdgcn_raw_buffer_store_b128(w, orr, voff, 0, 0);
+                    } else if (one) {
+                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pfb_u2, v0[q]), orr, voff, 0, 0);
+                    }
+                }
             }
         }
         __syncthreads();

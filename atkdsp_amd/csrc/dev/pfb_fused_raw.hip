@@ -50,7 +50,7 @@ extern "C" int atkgpu_pfb_fused_raw(void *s, const void *raw, size_t spitch, siz
                                     float off_im, int zlo, const float *taps, const void *tw, void *out,
                                     size_t opitch) {
     if (!m || !nel) return 0;
-    if (!atkgpu_pfb_fused_raw_supported(M, os, kos, fmt)) return -1;
+    if (!atkgpu_pfb_fused_raw_supported(M, os, kos, fmt) || opitch > PFB_OPITCH_MAX) return -1;
     const int G = atkgpu_pfb_group(M, os);
     PfbFusedArgs a = {};
     a.s = (const cpx *)raw;
