@@ -189,8 +189,16 @@ FFT_HD void team_stage_w(cpx *v, cpx *lds, const cpx *w, int j, int Ns, bool las
 template <int N, int PT>
 FFT_HD void team_load(cpx *v, const cpx *lds, int j) {
     constexpr int NT = N / PT;
+    if constexpr (NT % 16 == 0) {
+        // lpad(j + r*NT) = lpad(j) + r*(NT + NT/16) for j >= 0: one address,
+        // the rest immediate offsets
+        const cpx *b = lds + lpad(j);
 #pragma unroll
-    for (int r = 0; r < PT; ++r) v[r] = lds[lpad(j + r * NT)];
+        for (int r = 0; r < PT; ++r) v[r] = b[r * (NT + NT / 16)];
+    } else {
+#pragma unroll
+        for (int r = 0; r < PT; ++r) v[r] = lds[lpad(j + r * NT)];
+    }
 }
 
 // ---- generic mixed-radix Stockham stage (runtime radix) ----------------------
