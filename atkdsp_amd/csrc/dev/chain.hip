@@ -210,17 +210,84 @@ __device__ __forceinline__ cpx raw_conv(const unsigned char *raw, unsigned i, fl
     return o;
 }
 
+// Bytes of one raw sample of format FMT.
+template <int FMT>
+__host__ __device__ constexpr int raw_bps() {
+    return FMT == FMT_CU8 || FMT == FMT_CI8 ? 2 : FMT == FMT_CF32 ? 8 : 4;
+}
+
 // HALF: fhop = N/2, so a frame inside the block (and not the last) owns
 // exactly its first PT/2 sample slots
 template <int FMT, bool HALF>
 struct DispRawSrc {
     cpx *row;
     const unsigned char *raw;
+    __amdgpu_buffer_rsrc_t rr;      // the block's raw bytes
     int fhop, dlen, last;
     float cre, cim, kk, ore, oim;
+    // fm: k is a power of two and c' is zero or c'k normal, so ((float)b - c')*k
+    // is computed as fma((float)b, k, -c'k): c'k is exact, and scaling by a
+    // power of two commutes with rounding, so both give the same float (the
+    // result is never subnormal: b - c' is 0, or at least 1/2 in magnitude
+    // when |c'| < 1/2, or a nonzero multiple of ulp(c') >= 2^-24 otherwise)
+    bool fm;
+    float dre, dim;
     double sr, si;
     __device__ __forceinline__ cpx conv(int pos) const {
         return raw_conv<FMT>(raw, (unsigned)(pos - dlen), cre, cim, kk, ore, oim);
+    }
+    // the raw word of sample i0 + r*NT of the block, through the buffer (a
+    // lane offset and a constant offset per r: no vector address arithmetic)
+    template <int NT>
+    __device__ __forceinline__ unsigned bword(unsigned i0, int r) const {
+        constexpr int B = raw_bps<FMT>();
+        const int vo = (int)(i0 * B), so = r * NT * B;
+        if constexpr (B == 2) return __builtin_amdgcn_raw_buffer_load_b16(rr, vo, so, 0);
+        else return __builtin_amdgcn_raw_buffer_load_b32(rr, vo, so, 0);
+    }
+    __device__ __forceinline__ void wparts(unsigned w, float &x, float &y) const {
+        if constexpr (FMT == FMT_CU8) {
+            x = (float)(w & 0xffu);
+            y = (float)((w >> 8) & 0xffu);
+        } else if constexpr (FMT == FMT_CI8) {
+            x = (float)(signed char)(w & 0xffu);
+            y = (float)(signed char)((w >> 8) & 0xffu);
+        } else {
+            x = (float)(short)(w & 0xffffu);
+            y = (float)(short)(w >> 16);
+        }
+    }
+    // the frame's PT samples j + r*NT, all inside the block (sample i0 first)
+    template <int PT, int NT>
+    __device__ __forceinline__ void bload(unsigned i0, cpx *v) const {
+#pragma clang fp contract(off)
+        if constexpr (FMT == FMT_CF32) {
+#pragma unroll
+            for (int r = 0; r < PT; ++r) {
+                const cpx b = __builtin_bit_cast(
+                    cpx, __builtin_amdgcn_raw_buffer_load_b64(rr, (int)(i0 * 8), r * NT * 8, 0));
+                v[r] = cmake(b.x - ore, b.y - oim);
+            }
+        } else {
+            unsigned w[PT];
+#pragma unroll
+            for (int r = 0; r < PT; ++r) w[r] = bword<NT>(i0, r);
+            if (FMT != FMT_CU8 && fm) {
+#pragma unroll
+                for (int r = 0; r < PT; ++r) {
+                    float x, y;
+                    wparts(w[r], x, y);
+                    v[r] = cmake(__builtin_fmaf(x, kk, -dre), __builtin_fmaf(y, kk, -dim));
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < PT; ++r) {
+                    float x, y;
+                    wparts(w[r], x, y);
+                    v[r] = cmake((x - cre) * kk, (y - cim) * kk);
+                }
+            }
+        }
     }
     template <int PT, int NT>
     __device__ __forceinline__ void load(int f, int j, cpx *v) const {
@@ -228,8 +295,7 @@ struct DispRawSrc {
         if (f * fhop >= dlen) {
             // the whole frame is in this block: raw loads only
             const unsigned i0 = (unsigned)(p0 - dlen);
-#pragma unroll
-            for (int r = 0; r < PT; ++r) v[r] = raw_conv<FMT>(raw, i0 + r * NT, cre, cim, kk, ore, oim);
+            bload<PT, NT>(i0, v);
         } else {
 #pragma unroll
             for (int r = 0; r < PT; ++r) {
@@ -298,6 +364,17 @@ __global__ void __launch_bounds__(256) k_disp_unpack(DispRawArgs a) {
     // the folded constants exactly as src/convert.c:47, :57, :68 (IEEE division)
     src.cre = a.c0 + src.ore / a.kk;
     src.cim = a.c0 + src.oim / a.kk;
+    src.rr = __builtin_amdgcn_make_buffer_rsrc((void *)src.raw, (short)0, a.n * raw_bps<FMT>(), 0x00020000);
+    {
+#pragma clang fp contract(off)
+        const unsigned kb = __builtin_bit_cast(unsigned, a.kk);
+        const bool pow2 = (kb & 0x807fffffu) == 0 && (kb >> 23) != 0 && (kb >> 23) != 255;
+        src.dre = src.cre * a.kk;
+        src.dim = src.cim * a.kk;
+        // c' = 0 exactly, or c'k normal (a c'k that underflowed is not exact)
+        auto ok = [](float c, float d) { return c == 0.0f || __builtin_fabsf(d) >= 1.17549435e-38f; };
+        src.fm = pow2 && ok(src.cre, src.dre) && ok(src.cim, src.dim);
+    }
     src.sr = 0.0;
     src.si = 0.0;
     const int f0 = fb * a.fpb;
