@@ -49,13 +49,13 @@ struct PfbShape {
     // hops a subset folds: at least enough that every FFT team has a hop
     static constexpr int GP = SUB * GP0 >= TEAMS ? GP0 : TEAMS / SUB;
     static constexpr int G = SUB * GP;                       // hops per group
-    static constexpr int LPC = G >= 64 ? 64 : G;             // store lanes per channel
     // an LDS row holds one hop: the FFT's padded layout (lpad) during the
-    // transform, natural bin order after it. In the store phase half a
-    // wavefront reads LPC' = min(LPC, 32) hops x 32/LPC' channels, 8-byte
-    // elements at p*ROW + k; a row stride of 32/LPC' (mod 32) elements puts
-    // those 32 reads on 32 distinct bank pairs
-    static constexpr int RMOD = 32 / (LPC < 32 ? LPC : 32);
+    // transform, natural bin order after it. In the store phase a lane reads
+    // the two hops 2p, 2p+1 it stores, one per instruction: half a wavefront
+    // reads L = min(G/2, 32) rows 2p x 32/L channels, 8-byte elements at
+    // 2p*ROW + k. For G <= 32 a row stride of 32/G (mod 32) elements puts
+    // those 32 reads on 32 distinct bank pairs (2p*32/G = p*32/L)
+    static constexpr int RMOD = 32 / (G < 32 ? G : 32);
     static constexpr int ROW = lpad_size<M>() + ((RMOD - lpad_size<M>() % 32) % 32 + 32) % 32;
     static_assert(NT <= 256 && G % TEAMS == 0, "shape");
 };
@@ -379,7 +379,7 @@ __global__ void __launch_bounds__(256, 2) k_pfb_fused(PfbFusedArgs a) {
         // 2*lp+1), one 16-byte buffer store per pass at a per-block lane
         // offset from a scalar base, so a store costs no vector arithmetic ----
         {
-            static_assert(G % 2 == 0 && S::LPC == G, "a group's hops are whole lane pairs of one pass");
+            static_assert(G % 2 == 0 && G <= 64, "a group's hops are whole lane pairs of one pass");
             constexpr int LP2 = G / 2;                   // lanes per channel
             constexpr int CH_PER_PASS = 256 / LP2;
             constexpr int Q = 4;                         // passes whose bins are read together
