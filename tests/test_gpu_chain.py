@@ -23,7 +23,8 @@ def line_close(a_db, b_db, tol_db=3e-3, floor=1e-9):
 
 
 @pytest.mark.parametrize("fmt,nchan,taps,n", [("ci16q11", 1024, 16, 200_000), ("cu8", 256, 12, 131_071),
-                                               ("cf32", 2048, 12, 300_001)])
+                                               ("cf32", 2048, 12, 300_001), ("ci8", 512, 8, 150_001),
+                                               ("ci16", 1024, 16, 100_003)])
 def test_chain_matches_dropin_and_oracle(gpu, fmt, nchan, taps, n):
     from atkdsp_amd import synth
     from atkdsp_amd.chain import Chain
