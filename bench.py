@@ -116,6 +116,7 @@ def precision_forms(S: int = 256, n: int = 1_200_000, steps: int = 10):
             "speedup_vs_fp32": {f: round(fp / out[f]["device_ms"], 3) for f in out if f != "fp32"},
             "speedup_vs_fp32raw": {f: round(fr / out[f]["device_ms"], 3) for f in mx},
             "kernel_speedup_vs_fp32raw": {f: round(kr / out[f]["filterbank_ms"], 3) for f in mx},
+            **mx_busy(),
             "note": "device_ms: HIP events around one whole run (unpack where there is one, filterbank, "
                     "history copy); filterbank_ms: the filterbank kernel alone; fp32raw reads the raw rows the "
                     "matrix-core forms read; errors against the FP64 oracle are in tests/test_gpu_precision.py"}
@@ -329,6 +330,19 @@ def nco_phase_run(seconds: float = 10.0, fs: float = 40e6, p: int = 1234, q: int
     return {"samples": nrun * chunk, "runs": nrun, "chunk": chunk, "f_over_fs": f"{p}/{q}", "ms": round(ms, 2),
             "Msamples_per_s": round(nrun * chunk / (ms * 1e-3) / 1e6, 1),
             "final_phase_err_rad": float(err[-1]), "max_phase_err_rad": float(err.max())}
+
+
+def mx_busy():
+    """Matrix-core busy fraction of each matrix-core form at this workload,
+    from the committed rocprofv3 counter summary (the counters cannot be read
+    from inside the timed run)."""
+    f = ROOT / "profiles" / "pfb_mx_c1_mfma.json"
+    if not f.exists():
+        return {}
+    d = json.loads(f.read_text())
+    busy = {k: round(v["mfma_busy_cycles"] / (v["grbm_gui_active"] / d["xcds"] * d["simds"]), 4)
+            for k, v in d["forms"].items()}
+    return {"mfma_busy": busy, "mfma_busy_source": f"profiles/{f.name}: {d['source']}"}
 
 
 def pfb_traffic():
