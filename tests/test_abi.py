@@ -134,3 +134,24 @@ def test_reference_binding_would_load_it():
     lib = C.CDLL(str(LIB))
     lib.atkdsp_build_info.restype = C.c_char_p
     assert b"abi 26" in lib.atkdsp_build_info()
+
+
+def test_filterbank_launch_refuses_an_output_pitch_beyond_its_offsets():
+    """The fused filterbank's store uses 32-bit lane offsets, so its launchers
+    refuse an output pitch above 2^20 hops, before any device call (so this
+    runs without a device); a call with no hops is a no-op that succeeds."""
+    lib = C.CDLL(str(LIB))
+    f = lib.atkgpu_pfb_fused
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_uint, C.c_size_t, C.c_longlong, C.c_uint,
+                  C.c_uint, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p, C.c_size_t,
+                  C.c_size_t]
+    big = (1 << 20) + 1
+    assert f(None, None, 1 << 22, 1 << 22, 1, 16, 0, 1024, 2, 32, None, None, None, 0, None, big, 0) == -1
+    assert f(None, None, 1 << 22, 1 << 22, 1, 0, 0, 1024, 2, 32, None, None, None, 0, None, big, 0) == 0
+    r = lib.atkgpu_pfb_fused_raw
+    r.restype = C.c_int
+    r.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_uint, C.c_size_t, C.c_longlong, C.c_uint,
+                  C.c_uint, C.c_uint, C.c_int, C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.c_size_t]
+    assert r(None, None, 1 << 22, 1 << 22, 1, 16, 0, 64, 1, 8, 0, 0.0, 0.0, 0, None, None, None, big) == -1
