@@ -55,11 +55,14 @@ extern "C" int atkgpu_pfb_fused(void *s, const void *sbuf, size_t spitch, size_t
     a.opitch = (long long)opitch;
     a.ocol = (long long)ocol;
     const long long ngroups = ((long long)m + G - 1) / G;
-    // about 2048 blocks in all: four rounds of the 512 that are resident at
-    // once (two per CU on 256 CUs); measured 2.287 ms at 128 x 2,000,000
-    // samples against 2.315 ms with 4096 blocks (7.75 rounds), 2.318 ms with
-    // 1024 and 2.361 ms with 512
-    long long gpb = ((long long)nel * ngroups + 2047) / 2048;
+    // about 2048 blocks in all at 256 channels and more: four rounds of the
+    // 512 that are resident at once (two per CU on 256 CUs); measured at
+    // 128 x 2,000,000 samples into 1024 channels against 1024, 1536, 3072
+    // and 4096 blocks. The narrow shapes (64 and 128 channels, memory-bound,
+    // small groups) run best with about 8192 (64 channels at 256 x 1,200,000
+    // samples: 1.00 against 1.05 ms)
+    const long long nbt = M <= 128 ? 8192 : 2048;
+    long long gpb = ((long long)nel * ngroups + nbt - 1) / nbt;
     if (gpb < 1) gpb = 1;
     a.groups_per_block = (int)gpb;
     a.nblk_per_el = (int)((ngroups + gpb - 1) / gpb);
